@@ -1,0 +1,114 @@
+// Launch-side declarations shared by the DiffSoup HIP translation units.
+// Kernels live in raster.hip, multires.hip, encoding.hip, edge_grad.hip and
+// shade.hip; the extern "C" boundary in capi.hip validates arguments and
+// calls these launchers on the caller's stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ds {
+
+constexpr int kMaxViews = 64;
+
+// Per-launch description of the views packed into one batch.  Passed to the
+// kernels by value, so a captured graph keeps its own copy.  Views may have
+// different resolutions; view b's pixels occupy [pix_base[b], pix_base[b+1])
+// of every packed per-pixel buffer and its tiles [tile_base[b], tile_base[b+1]).
+struct Views {
+    int n;
+    int tile;                      // tile edge in pixels: 8, 16 or 32
+    int H[kMaxViews], W[kMaxViews];
+    int tiles_x[kMaxViews];
+    int tile_base[kMaxViews + 1];
+    int pix_base[kMaxViews + 1];
+};
+
+// Per-(view, triangle) setup record: clip-space corners and the conservative
+// pixel rectangle (half-open; empty when culled).
+struct __align__(16) TriRecord {
+    float4 p0, p1, p2;
+    int4 rect;                     // y0, y1, x0, x1
+};
+
+// Device-side counters written by the raster launches (int64 each).
+enum Stat : int {
+    kStatCovered = 0,   // fragments inside and with depth in [-1, 1], before the mask
+    kStatPairs = 1,     // (triangle, tile) pairs produced by binning
+    kStatOverflow = 2,  // 1 when pairs exceeded the bin capacity (fallback path ran)
+    kStatBadIndex = 3,  // 1 when a triangle referenced a vertex outside [0, V)
+    kStatCount = 8,
+};
+
+// Workspace carve-up for one raster launch.
+struct RasterWorkspace {
+    TriRecord* recs;     // [B * T]
+    int* tile_count;     // [n_tiles]
+    int* tile_off;       // [n_tiles + 1]
+    int* bins;           // [capacity]
+    long long* stats;    // [kStatCount]
+    int64_t capacity;
+};
+
+size_t raster_workspace_bytes(int B, int T, int n_tiles, int64_t capacity);
+RasterWorkspace raster_workspace_carve(void* ws, int B, int T, int n_tiles, int64_t capacity);
+
+// Mask modes for the depth test.
+enum MaskMode : int { kMaskDeterministic = 0, kMaskHash = 1 };
+
+struct AlphaSource {
+    const float* alpha;   // [T, S] post-sigmoid lattice opacities at one level
+    int level;
+    int S;
+};
+
+hipError_t launch_raster_bin(const Views& v, int V, const float* pos, int T, const int* tri,
+                             const RasterWorkspace& w, hipStream_t s);
+hipError_t launch_raster_resolve(const Views& v, int T, const AlphaSource& a, int mask_mode,
+                                 uint64_t seed, float* rast, const RasterWorkspace& w, hipStream_t s);
+
+// Training-step variant of the resolve: background pixels get their colour,
+// MSE gradient and L1 written here, covered pixels are appended to a list.
+struct BackgroundShade {
+    const float* target;   // [P, 3]
+    float* color;          // [P, 3]
+    float* grad_color;     // [P, 3]
+    float* pix_loss;       // [P]
+    float* loss_part;      // [n_tiles] per-tile sum of squared background error
+    int* covered;          // [P] covered pixel indices (unordered)
+    int* n_covered;        // device counter, zeroed before the launch
+    float grad_scale;      // dL/dcolor = grad_scale * (color - target)
+    float background;      // constant background colour
+};
+hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource& a, int mask_mode,
+                                       uint64_t seed, float* rast, const RasterWorkspace& w,
+                                       const BackgroundShade& bg, hipStream_t s);
+hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const float* rast,
+                              const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
+                              hipStream_t s);
+hipError_t launch_pixel_l1(long long P, int C, const float* color, const float* target,
+                           float* pix_loss, hipStream_t s);
+
+// Multires lattice gathers and scatters.
+hipError_t launch_multires_color_fwd(long long P, const float* rast, int lo, int hi, int T, int C,
+                                     const float* feat, int concat, float* out, hipStream_t s);
+hipError_t launch_multires_color_bwd(long long P, const float* rast, int lo, int hi, int T, int C,
+                                     float* grad_feat, int concat, const float* grad_out,
+                                     hipStream_t s);
+hipError_t launch_accumulate_fwd(int T, int lo, int hi, int target, int C, const float* feat,
+                                 float* out, hipStream_t s);
+hipError_t launch_accumulate_bwd(int T, int lo, int hi, int target, int C, float* grad_feat,
+                                 const float* grad_out, hipStream_t s);
+
+hipError_t launch_sh2(const Views& v, const float* rast, const float* inv_mvp, float* enc,
+                      hipStream_t s);
+
+hipError_t launch_edge_grad(const Views& v, int C, const float* color, const float* grad_color,
+                            const float* rast, int V, const float* pos, float* grad_pos,
+                            int T, const int* tri, hipStream_t s);
+
+hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s);
+hipError_t launch_fill_f32(float* p, long long n, float value, hipStream_t s);
+
+}  // namespace ds

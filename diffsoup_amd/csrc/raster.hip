@@ -1,0 +1,532 @@
+// Tile-binned rasterizer with stochastic opacity masking and a packed
+// depth|triangle-id test, plus the opacity auxiliary gradient that re-walks
+// the same bins.
+//
+// Pipeline per launch (all on one stream, no host synchronisation):
+//   k_clear   zero the per-tile counters and the stats block
+//   k_setup   per (view, triangle): cull, conservative rectangle, record,
+//             count the tiles the rectangle touches
+//   k_scan    exclusive scan of the per-tile counts (one workgroup)
+//   k_emit    per (view, triangle): claim a slot in each touched tile's bin
+//   k_resolve per tile: stage bin records in LDS, each lane owns one pixel
+//             and keeps the smallest packed key of the fragments that pass
+//             the mask; one coalesced float4 write of rast per pixel.
+// Because every pixel is owned by exactly one lane, the z-buffer lives in
+// registers and needs no atomics; the min over keys makes the result
+// independent of bin order.  When the (triangle, tile) pairs exceed the bin
+// capacity the resolve kernel walks every triangle of the view instead,
+// which is slower but gives the same image.
+//
+// Reference behaviour reproduced: src/cuda/rasterize.cu:43-137 (rects),
+// :170-277 (fragments), :319-365 (depth test), :923-971 (opacity aux),
+// src/cuda/multires.cu:130-156 (per-fragment alpha), and
+// python/diffsoup/multires.py:52-102 (the composition).
+#include "ds_kernels.h"
+#include "ds_math.h"
+
+namespace ds {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int view_of_tile(const Views& v, int tile) {
+    int b = 0;
+    while (b + 1 < v.n && v.tile_base[b + 1] <= tile) ++b;
+    return b;
+}
+
+__global__ void k_fill_i32(int* p, long long n, int value) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+__global__ void k_fill_f32(float* p, long long n, float value) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+__global__ void k_clear(int* tile_count, int n_tiles, long long* stats) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_tiles) tile_count[i] = 0;
+    if (i < kStatCount) stats[i] = 0;
+}
+
+__device__ __forceinline__ void tile_span(const int4 r, int ts, int& tx0, int& tx1, int& ty0, int& ty1) {
+    tx0 = r.z / ts; tx1 = (r.w - 1) / ts;
+    ty0 = r.x / ts; ty1 = (r.y - 1) / ts;
+}
+
+__global__ void k_setup(Views v, int V, const float4* __restrict__ pos, int T,
+                        const int* __restrict__ tri, TriRecord* __restrict__ recs,
+                        int* __restrict__ tile_count, long long* __restrict__ stats) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= (long long)v.n * T) return;
+    const int b = static_cast<int>(i / T);
+    const int t = static_cast<int>(i - (long long)b * T);
+    const int i0 = tri[3 * t + 0], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+    TriRecord r;
+    r.rect = make_int4(0, 0, 0, 0);
+    if (i0 < 0 || i0 >= V || i1 < 0 || i1 >= V || i2 < 0 || i2 >= V) {
+        r.p0 = r.p1 = r.p2 = make_float4(0.f, 0.f, 0.f, 1.f);
+        recs[i] = r;
+        stats[kStatBadIndex] = 1;
+        return;
+    }
+    const float4* P = pos + (long long)b * V;
+    r.p0 = P[i0]; r.p1 = P[i1]; r.p2 = P[i2];
+    const Rect q = triangle_rect(r.p0, r.p1, r.p2, v.H[b], v.W[b]);
+    r.rect = make_int4(q.y0, q.y1, q.x0, q.x1);
+    recs[i] = r;
+    if (q.y1 <= q.y0) return;
+    int tx0, tx1, ty0, ty1;
+    tile_span(r.rect, v.tile, tx0, tx1, ty0, ty1);
+    int* cnt = tile_count + v.tile_base[b];
+    const int tw = v.tiles_x[b];
+    for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&cnt[ty * tw + tx], 1);
+}
+
+// Exclusive scan of n ints by one 1024-thread workgroup, 4 items per thread
+// per round.  off[n] receives the total; totals above `capacity` set the
+// overflow stat so that k_emit writes nothing and k_resolve falls back.
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int* __restrict__ off,
+                                               int n, long long capacity, long long* stats) {
+    __shared__ long long wave_sum[1024 / kWave];
+    __shared__ long long carry_s;
+    const int tid = threadIdx.x, lane = tid % kWave, wv = tid / kWave;
+    if (tid == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 4 * 1024) {
+        const int i = base + 4 * tid;
+        int v0 = i + 0 < n ? cnt[i + 0] : 0;
+        int v1 = i + 1 < n ? cnt[i + 1] : 0;
+        int v2 = i + 2 < n ? cnt[i + 2] : 0;
+        int v3 = i + 3 < n ? cnt[i + 3] : 0;
+        const long long s = (long long)v0 + v1 + v2 + v3;
+        long long inc = s;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const long long o = __shfl_up(inc, d, kWave);
+            if (lane >= d) inc += o;
+        }
+        if (lane == kWave - 1) wave_sum[wv] = inc;
+        __syncthreads();
+        long long wpre = 0, total = 0;
+        for (int k = 0; k < 1024 / kWave; ++k) {
+            const long long ws = wave_sum[k];
+            if (k < wv) wpre += ws;
+            total += ws;
+        }
+        const long long carry = carry_s;
+        long long e = carry + wpre + inc - s;
+        if (i + 0 < n) off[i + 0] = static_cast<int>(e);
+        e += v0;
+        if (i + 1 < n) off[i + 1] = static_cast<int>(e);
+        e += v1;
+        if (i + 2 < n) off[i + 2] = static_cast<int>(e);
+        e += v2;
+        if (i + 3 < n) off[i + 3] = static_cast<int>(e);
+        __syncthreads();
+        if (tid == 0) carry_s = carry + total;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long long total = carry_s;
+        off[n] = total > capacity ? static_cast<int>(capacity) : static_cast<int>(total);
+        stats[kStatPairs] = total;
+        if (total > capacity) stats[kStatOverflow] = 1;
+    }
+}
+
+__global__ void k_emit(Views v, int T, const TriRecord* __restrict__ recs, int* __restrict__ tile_count,
+                       const int* __restrict__ tile_off, int* __restrict__ bins,
+                       const long long* __restrict__ stats) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= (long long)v.n * T) return;
+    if (stats[kStatOverflow]) return;
+    const int4 r = recs[i].rect;
+    if (r.y <= r.x) return;
+    const int b = static_cast<int>(i / T);
+    const int t = static_cast<int>(i - (long long)b * T);
+    int tx0, tx1, ty0, ty1;
+    tile_span(r, v.tile, tx0, tx1, ty0, ty1);
+    const int base = v.tile_base[b];
+    const int tw = v.tiles_x[b];
+    for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx) {
+            const int tile = base + ty * tw + tx;
+            const int slot = atomicSub(&tile_count[tile], 1) - 1;
+            bins[tile_off[tile] + slot] = t;
+        }
+}
+
+// Opacity at one lattice level (src/cuda/multires.cu:130-156 with C = 1).
+__device__ __forceinline__ float lattice_alpha(const AlphaSource& a, int t, float b0, float b1) {
+    const Corners c = lattice_corners(b0, b1, static_cast<uint32_t>(a.level));
+    const float* src = a.alpha + (long long)t * a.S;
+    float alpha = 0.f;
+    alpha += c.w[0] * src[c.idx[0]];
+    alpha += c.w[1] * src[c.idx[1]];
+    alpha += c.w[2] * src[c.idx[2]];
+    return alpha;
+}
+
+// Shared traversal state for one tile: which triangles to visit and where
+// this lane's pixel sits.
+struct TileWalk {
+    int b, px, py, qx0, qy0;
+    float ndc_x, ndc_y;
+    int start, n;
+    bool overflow;
+};
+
+template <int TS>
+__device__ __forceinline__ TileWalk tile_walk(const Views& v, int T, const int* tile_off,
+                                              const long long* stats) {
+    TileWalk w;
+    const int tile = blockIdx.x;
+    w.b = view_of_tile(v, tile);
+    const int lt = tile - v.tile_base[w.b];
+    const int tx = lt % v.tiles_x[w.b], ty = lt / v.tiles_x[w.b];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    constexpr int QPR = TS / 8;
+    w.qx0 = tx * TS + (wave % QPR) * 8;
+    w.qy0 = ty * TS + (wave / QPR) * 8;
+    w.px = w.qx0 + lane % 8;
+    w.py = w.qy0 + lane / 8;
+    w.ndc_x = pixel_ndc(w.px, v.W[w.b]);
+    w.ndc_y = pixel_ndc(w.py, v.H[w.b]);
+    w.overflow = stats[kStatOverflow] != 0;
+    if (w.overflow) {
+        w.start = 0;
+        w.n = T;
+    } else {
+        w.start = tile_off[tile];
+        w.n = tile_off[tile + 1] - w.start;
+    }
+    return w;
+}
+
+template <int TS>
+struct TileStage {
+    float4 p0[TS * TS], p1[TS * TS], p2[TS * TS];
+    int4 rect[TS * TS];
+    int tri[TS * TS];
+};
+
+template <int TS>
+__device__ __forceinline__ void stage_chunk(TileStage<TS>& st, const TileWalk& w, int c0, int m, int T,
+                                            const int* bins, const TriRecord* recs) {
+    if (static_cast<int>(threadIdx.x) < m) {
+        const int t = w.overflow ? c0 + threadIdx.x : bins[w.start + c0 + threadIdx.x];
+        const TriRecord r = recs[(long long)w.b * T + t];
+        st.p0[threadIdx.x] = r.p0;
+        st.p1[threadIdx.x] = r.p1;
+        st.p2[threadIdx.x] = r.p2;
+        st.rect[threadIdx.x] = r.rect;
+        st.tri[threadIdx.x] = t;
+    }
+}
+
+// Wave-uniform test: does rectangle r touch this wave's 8x8 pixel quad?
+__device__ __forceinline__ bool quad_hit(int4 r, int qx0, int qy0) {
+    const int y0 = __builtin_amdgcn_readfirstlane(r.x), y1 = __builtin_amdgcn_readfirstlane(r.y);
+    const int x0 = __builtin_amdgcn_readfirstlane(r.z), x1 = __builtin_amdgcn_readfirstlane(r.w);
+    return !(x1 <= qx0 || x0 >= qx0 + 8 || y1 <= qy0 || y0 >= qy0 + 8);
+}
+
+__device__ __forceinline__ bool in_rect(int4 r, int px, int py) {
+    return px >= r.z && px < r.w && py >= r.x && py < r.y;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long x) {
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+template <int TS, bool SHADE>
+__global__ void __launch_bounds__(TS * TS)
+k_resolve(Views v, int T, AlphaSource a, int mask_mode, unsigned long long seed,
+          const TriRecord* __restrict__ recs, const int* __restrict__ tile_off,
+          const int* __restrict__ bins, long long* __restrict__ stats, float4* __restrict__ rast,
+          BackgroundShade bg) {
+    constexpr int NT = TS * TS;
+    __shared__ TileStage<TS> st;
+    __shared__ float wave_loss[NT / kWave];
+    const TileWalk w = tile_walk<TS>(v, T, tile_off, stats);
+
+    long long best = kKeyEmpty;
+    float bb0 = 0.f, bb1 = 0.f, bd = 0.f;
+    int bt = -1;
+    long long covered = 0;
+
+    for (int c0 = 0; c0 < w.n; c0 += NT) {
+        const int m = min(NT, w.n - c0);
+        __syncthreads();
+        stage_chunk<TS>(st, w, c0, m, T, bins, recs);
+        __syncthreads();
+        for (int j = 0; j < m; ++j) {
+            const int4 r = st.rect[j];
+            if (!quad_hit(r, w.qx0, w.qy0)) continue;
+            if (!in_rect(r, w.px, w.py)) continue;
+            Hit h;
+            if (!cover(w.ndc_y, w.ndc_x, st.p0[j], st.p1[j], st.p2[j], h)) continue;
+            float d;
+            if (!fragment_depth(h, d)) continue;
+            ++covered;
+            if (!depth_testable(d)) continue;
+            const int t = st.tri[j];
+            const long long key = depth_key(d, static_cast<uint32_t>(t));
+            if (key >= best) continue;  // cannot win whether or not the mask keeps it
+            const float alpha = lattice_alpha(a, t, h.b0, h.b1);
+            const float u = mask_mode == kMaskHash
+                                ? mask_threshold(seed, w.b, t, w.py, w.px)
+                                : 0.5f;
+            if (alpha < u) continue;
+            best = key; bb0 = h.b0; bb1 = h.b1; bd = d; bt = t;
+        }
+    }
+
+    const bool inside = w.px < v.W[w.b] && w.py < v.H[w.b];
+    const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
+    if (inside) {
+        rast[p] = bt >= 0 ? make_float4(bb0, bb1, bd, static_cast<float>(bt + 1))
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if constexpr (SHADE) {
+        // Background pixels are shaded here (constant colour, MSE term and its
+        // gradient, per-pixel L1 for the opacity gradient); covered pixels are
+        // appended to the list the MLP kernel consumes.
+        float l = 0.f;
+        if (inside && bt < 0) {
+            const float* tg = bg.target + p * 3;
+            float* col = bg.color + p * 3;
+            float* gc = bg.grad_color + p * 3;
+            float l1 = 0.f;
+            for (int c = 0; c < 3; ++c) {
+                const float d = bg.background - tg[c];
+                col[c] = bg.background;
+                gc[c] = bg.grad_scale * d;
+                l1 += fabsf(d);
+                l += d * d;
+            }
+            bg.pix_loss[p] = l1;
+        }
+        const bool cov = inside && bt >= 0;
+        const unsigned long long m = __ballot(cov);
+        const int lane = threadIdx.x % kWave;
+        int base = 0;
+        if (lane == 0 && m) base = atomicAdd(bg.n_covered, __popcll(m));
+        base = __shfl(base, 0, kWave);
+        if (cov) {
+            const int rank = __popcll(m & ((1ull << lane) - 1ull));
+            bg.covered[base + rank] = static_cast<int>(p);
+        }
+        for (int d = kWave / 2; d > 0; d >>= 1) l += __shfl_xor(l, d, kWave);
+        if (lane == 0) wave_loss[threadIdx.x / kWave] = l;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float sum = 0.f;
+            for (int k = 0; k < NT / kWave; ++k) sum += wave_loss[k];
+            bg.loss_part[blockIdx.x] = sum;
+        }
+    }
+    covered = wave_sum_ll(covered);
+    if (threadIdx.x % kWave == 0 && covered)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&stats[kStatCovered]),
+                  static_cast<unsigned long long>(covered));
+}
+
+// Score-function gradient of the per-pixel L1 loss w.r.t. fragment opacity,
+// scattered onto the opacity lattice (src/cuda/rasterize.cu:923-971 and
+// src/cuda/multires.cu:178-202).  Every fragment of the fragment list that is
+// not strictly behind the visible surface contributes; fragments are
+// regenerated from the bins instead of being stored.
+template <int TS>
+__global__ void __launch_bounds__(TS * TS)
+k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
+              const int* __restrict__ tile_off, const int* __restrict__ bins,
+              const long long* __restrict__ stats, const float4* __restrict__ rast,
+              const float* __restrict__ pix_loss, float* __restrict__ grad_alpha) {
+    constexpr int NT = TS * TS;
+    __shared__ TileStage<TS> st;
+    const TileWalk w = tile_walk<TS>(v, T, tile_off, stats);
+    const bool inside_view = w.px < v.W[w.b] && w.py < v.H[w.b];
+    float depth_pix = 0.f, loss = 0.f;
+    int tri_pix = -1;
+    if (inside_view) {
+        const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
+        const float4 rp = rast[p];
+        depth_pix = rp.z;
+        tri_pix = static_cast<int>(rp.w) - 1;
+        loss = pix_loss[p];
+    }
+    constexpr float eps = 1e-5f;
+    for (int c0 = 0; c0 < w.n; c0 += NT) {
+        const int m = min(NT, w.n - c0);
+        __syncthreads();
+        stage_chunk<TS>(st, w, c0, m, T, bins, recs);
+        __syncthreads();
+        for (int j = 0; j < m; ++j) {
+            const int4 r = st.rect[j];
+            if (!quad_hit(r, w.qx0, w.qy0)) continue;
+            if (!in_rect(r, w.px, w.py)) continue;
+            Hit h;
+            if (!cover(w.ndc_y, w.ndc_x, st.p0[j], st.p1[j], st.p2[j], h)) continue;
+            float d;
+            if (!fragment_depth(h, d)) continue;
+            if (tri_pix >= 0 && d > depth_pix) continue;
+            const int t = st.tri[j];
+            const float alpha = lattice_alpha(a, t, h.b0, h.b1);
+            float g;
+            if (tri_pix >= 0 && t == tri_pix) {
+                g = loss / (alpha > eps ? alpha : eps);
+            } else {
+                const float om = 1.f - alpha;
+                g = -(loss / (om > eps ? om : eps));
+            }
+            const Corners c = lattice_corners(h.b0, h.b1, static_cast<uint32_t>(a.level));
+            float* dst = grad_alpha + (long long)t * a.S;
+            unsafeAtomicAdd(&dst[c.idx[0]], g * c.w[0]);
+            unsafeAtomicAdd(&dst[c.idx[1]], g * c.w[1]);
+            unsafeAtomicAdd(&dst[c.idx[2]], g * c.w[2]);
+        }
+    }
+}
+
+__global__ void k_pixel_l1(long long P, int C, const float* __restrict__ color,
+                           const float* __restrict__ target, float* __restrict__ pix_loss) {
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float l = 0.f;
+    for (int c = 0; c < C; ++c) l += fabsf(color[p * C + c] - target[p * C + c]);
+    pix_loss[p] = l;
+}
+
+inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+}  // namespace
+
+size_t raster_workspace_bytes(int B, int T, int n_tiles, int64_t capacity) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    return up(sizeof(long long) * kStatCount) + up(sizeof(TriRecord) * (size_t)B * T) +
+           up(sizeof(int) * (size_t)n_tiles) + up(sizeof(int) * ((size_t)n_tiles + 1)) +
+           up(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
+}
+
+RasterWorkspace raster_workspace_carve(void* ws, int B, int T, int n_tiles, int64_t capacity) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    char* p = static_cast<char*>(ws);
+    RasterWorkspace w;
+    w.stats = reinterpret_cast<long long*>(p);           p += up(sizeof(long long) * kStatCount);
+    w.recs = reinterpret_cast<TriRecord*>(p);            p += up(sizeof(TriRecord) * (size_t)B * T);
+    w.tile_count = reinterpret_cast<int*>(p);            p += up(sizeof(int) * (size_t)n_tiles);
+    w.tile_off = reinterpret_cast<int*>(p);              p += up(sizeof(int) * ((size_t)n_tiles + 1));
+    w.bins = reinterpret_cast<int*>(p);
+    w.capacity = capacity;
+    return w;
+}
+
+hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned g = static_cast<unsigned>(std::min<long long>(blocks_for(n, 256), 65536));
+    k_fill_i32<<<g, 256, 0, s>>>(p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, long long n, float value, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned g = static_cast<unsigned>(std::min<long long>(blocks_for(n, 256), 65536));
+    k_fill_f32<<<g, 256, 0, s>>>(p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_bin(const Views& v, int V, const float* pos, int T, const int* tri,
+                             const RasterWorkspace& w, hipStream_t s) {
+    const int n_tiles = v.tile_base[v.n];
+    k_clear<<<blocks_for(std::max(n_tiles, (int)kStatCount), 256), 256, 0, s>>>(w.tile_count, n_tiles,
+                                                                                  w.stats);
+    const long long BT = (long long)v.n * T;
+    if (BT > 0) {
+        k_setup<<<blocks_for(BT, 256), 256, 0, s>>>(v, V, reinterpret_cast<const float4*>(pos), T, tri,
+                                                    w.recs, w.tile_count, w.stats);
+    }
+    k_scan<<<1, 1024, 0, s>>>(w.tile_count, w.tile_off, n_tiles, w.capacity, w.stats);
+    if (BT > 0) {
+        k_emit<<<blocks_for(BT, 256), 256, 0, s>>>(v, T, w.recs, w.tile_count, w.tile_off, w.bins,
+                                                   w.stats);
+    }
+    return hipGetLastError();
+}
+
+template <bool SHADE>
+static hipError_t resolve_dispatch(const Views& v, int T, const AlphaSource& a, int mask_mode,
+                                   uint64_t seed, float* rast, const RasterWorkspace& w,
+                                   const BackgroundShade& bg, hipStream_t s) {
+    const int n_tiles = v.tile_base[v.n];
+    if (n_tiles == 0) return hipSuccess;
+    float4* out = reinterpret_cast<float4*>(rast);
+    switch (v.tile) {
+        case 8:
+            k_resolve<8, SHADE><<<n_tiles, 64, 0, s>>>(v, T, a, mask_mode, seed, w.recs, w.tile_off,
+                                                       w.bins, w.stats, out, bg);
+            break;
+        case 16:
+            k_resolve<16, SHADE><<<n_tiles, 256, 0, s>>>(v, T, a, mask_mode, seed, w.recs, w.tile_off,
+                                                         w.bins, w.stats, out, bg);
+            break;
+        case 32:
+            k_resolve<32, SHADE><<<n_tiles, 1024, 0, s>>>(v, T, a, mask_mode, seed, w.recs, w.tile_off,
+                                                          w.bins, w.stats, out, bg);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_resolve(const Views& v, int T, const AlphaSource& a, int mask_mode,
+                                 uint64_t seed, float* rast, const RasterWorkspace& w, hipStream_t s) {
+    return resolve_dispatch<false>(v, T, a, mask_mode, seed, rast, w, BackgroundShade{}, s);
+}
+
+hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource& a, int mask_mode,
+                                       uint64_t seed, float* rast, const RasterWorkspace& w,
+                                       const BackgroundShade& bg, hipStream_t s) {
+    return resolve_dispatch<true>(v, T, a, mask_mode, seed, rast, w, bg, s);
+}
+
+hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const float* rast,
+                              const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
+                              hipStream_t s) {
+    const int n_tiles = v.tile_base[v.n];
+    if (n_tiles == 0) return hipSuccess;
+    const float4* r4 = reinterpret_cast<const float4*>(rast);
+    switch (v.tile) {
+        case 8:
+            k_opacity_aux<8><<<n_tiles, 64, 0, s>>>(v, T, a, w.recs, w.tile_off, w.bins, w.stats, r4,
+                                                    pix_loss, grad_alpha);
+            break;
+        case 16:
+            k_opacity_aux<16><<<n_tiles, 256, 0, s>>>(v, T, a, w.recs, w.tile_off, w.bins, w.stats, r4,
+                                                      pix_loss, grad_alpha);
+            break;
+        case 32:
+            k_opacity_aux<32><<<n_tiles, 1024, 0, s>>>(v, T, a, w.recs, w.tile_off, w.bins, w.stats,
+                                                       r4, pix_loss, grad_alpha);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pixel_l1(long long P, int C, const float* color, const float* target,
+                           float* pix_loss, hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    k_pixel_l1<<<blocks_for(P, 256), 256, 0, s>>>(P, C, color, target, pix_loss);
+    return hipGetLastError();
+}
+
+}  // namespace ds

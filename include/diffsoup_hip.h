@@ -1,0 +1,151 @@
+/*
+ * diffsoup_hip.h - C-ABI of libdiffsoup_hip.so, the MI355X implementation of
+ * DiffSoup's differentiable triangle-soup rasterizer.
+ *
+ * Conventions shared by every entry point:
+ *  - pointers are device pointers unless the parameter is documented as a
+ *    host array (the per-view H / W resolution arrays);
+ *  - `stream` is a hipStream_t (pass torch.cuda.current_stream().cuda_stream);
+ *    all work is enqueued on it, nothing synchronises the host;
+ *  - nothing allocates device memory; scratch space is a caller-owned,
+ *    256-byte aligned workspace sized by ds_raster_workspace_size;
+ *  - the return value is DS_OK (0) or a negative DS_ERR_* code, with a
+ *    message in ds_last_error() (thread-local); nothing calls exit();
+ *  - views of one batch may differ in resolution: view b's pixels occupy
+ *    [sum_{b'<b} H[b']W[b'], ...) of every packed per-pixel buffer;
+ *  - rast holds per pixel (b0, b1, depth = z/w, triangle id + 1), background
+ *    all zeros, exactly as the reference's rasterization output.
+ *
+ * Each declaration names the reference interface it replaces
+ * (kenji-tojo/diffsoup, path:line).
+ */
+#ifndef DIFFSOUP_HIP_H
+#define DIFFSOUP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DS_OK 0
+#define DS_ERR_INVALID -1   /* bad shape, size, pointer or option            */
+#define DS_ERR_HIP -2       /* a HIP launch error                            */
+#define DS_ERR_WORKSPACE -4 /* workspace smaller than ds_*_workspace_size    */
+#define DS_ERR_UNSUPPORTED -5 /* a configuration without a compiled kernel   */
+
+/* Device-side counters at the start of the raster workspace (int64 each). */
+#define DS_STAT_COVERED 0  /* fragments inside with depth in [-1,1], pre-mask */
+#define DS_STAT_PAIRS 1    /* (triangle, tile) pairs produced by binning       */
+#define DS_STAT_OVERFLOW 2 /* 1 if pairs exceeded the bin capacity             */
+#define DS_STAT_BAD_INDEX 3 /* 1 if a triangle indexed a vertex outside [0,V)  */
+
+/* Library version string.  Replaces _core.__version__ (src/main.cpp:26). */
+const char* ds_version(void);
+
+/* Message of the last failing call on this thread.  Replaces the
+ * CUDA_CHECK print-and-exit (src/cuda/cuda_common.cuh:19-27) and the
+ * std::runtime_error shape checks (src/main.cpp:183-185). */
+const char* ds_last_error(void);
+
+/* Lattice samples per triangle at one level: 3 at level 0, else
+ * (2^(L-1)+1)(2^L+1).  Same as feats_at_level (src/cuda/multires.cuh:8-10,
+ * python/diffsoup/multires.py:16-23). */
+uint32_t ds_feats_at_level(int level);
+
+/* Number of screen tiles of edge `tile` (8, 16 or 32) over B views with host
+ * resolution arrays H[B], W[B]; -1 on invalid input. */
+int64_t ds_raster_tile_count(int B, const int* H, const int* W, int tile);
+
+/* Bytes of workspace for ds_rasterize_fwd / ds_opacity_aux_bwd with room for
+ * `capacity` (triangle, tile) pairs.  When a batch produces more pairs the
+ * kernels still return the exact image through a slower path and set
+ * DS_STAT_OVERFLOW; grow the capacity for the next call. */
+size_t ds_raster_workspace_size(int B, const int* H, const int* W, int T, int tile,
+                                int64_t capacity);
+
+/* Byte offset of the int64 stats block (DS_STAT_*) inside the workspace. */
+int ds_raster_stats_offset(void);
+
+/* Rasterize B views of one triangle soup with per-fragment multires opacity,
+ * stochastic masking and a depth test.  pos [B,V,4] clip space (16-byte
+ * aligned), tri [T,3] int32, alpha_src [T, feats_at_level(level)] post-sigmoid
+ * opacities, rast [P,4] output.  stochastic != 0 keeps a fragment when
+ * alpha >= u with u a hash of (seed, view, triangle, row, column);
+ * stochastic == 0 uses u = 0.5.
+ * Replaces _core.compute_triangle_rects, compute_fragments,
+ * filter_valid_fragments, multires_triangle_alpha and depth_test
+ * (src/main.cpp:30-108,174-191) as composed by
+ * rasterize_multires_triangle_alpha (python/diffsoup/multires.py:52-102). */
+int ds_rasterize_fwd(int B, const int* H, const int* W, int V, const float* pos, int T,
+                     const int* tri, int level, const float* alpha_src, int stochastic,
+                     uint64_t seed, int tile, float* rast, void* workspace,
+                     size_t workspace_bytes, int64_t capacity, void* stream);
+
+/* Opacity auxiliary gradient: regenerates every fragment, and for those not
+ * strictly behind the visible surface adds the score-function gradient
+ * +l/max(a,1e-5) (visible triangle) or -l/max(1-a,1e-5) (others) onto
+ * grad_alpha [T, feats_at_level(level)] (accumulated, not scaled), with l the
+ * per-pixel loss pix_loss [P].  reuse_bins != 0 reuses the bins that the
+ * previous ds_rasterize_fwd left in the same workspace for the same pos/tri.
+ * Replaces _core.backward_opacity_aux_loss + backward_multires_triangle_alpha
+ * (src/main.cpp:134-154,193-210) as composed by _OpacityAuxLossFn.forward
+ * (python/diffsoup/rasterize.py:148-190). */
+int ds_opacity_aux_bwd(int B, const int* H, const int* W, int V, const float* pos, int T,
+                       const int* tri, int level, const float* alpha_src, int tile,
+                       const float* rast, const float* pix_loss, float* grad_alpha,
+                       int reuse_bins, void* workspace, size_t workspace_bytes,
+                       int64_t capacity, void* stream);
+
+/* pix_loss[p] = sum_c |color[p,c] - target[p,c]| in channel order: the
+ * per-pixel L1 inside backward_opacity_aux_loss_kernel
+ * (src/cuda/rasterize.cu:955-960). */
+int ds_pixel_l1(int64_t P, int C, const float* color, const float* target, float* pix_loss,
+                void* stream);
+
+/* Per-pixel multires feature gather over levels [lo, hi] of feat
+ * [T, S_tot, C].  concat == 0 sums the levels into out [P, C] (the
+ * reference); concat != 0 writes out [P, (hi-lo+1)*C], one slice per level.
+ * Background pixels are written as zeros.  Replaces
+ * _core.multires_triangle_color (src/main.cpp:212-232). */
+int ds_multires_color_fwd(int64_t P, const float* rast, int lo, int hi, int T, int C,
+                          const float* feat, int concat, float* out, void* stream);
+
+/* Adjoint of ds_multires_color_fwd, accumulated into grad_feat with atomics.
+ * Replaces _core.backward_multires_triangle_color (src/main.cpp:234-254). */
+int ds_multires_color_bwd(int64_t P, const float* rast, int lo, int hi, int T, int C,
+                          float* grad_feat, int concat, const float* grad_out, void* stream);
+
+/* Evaluate the level-summed multires function of feat [T, S_tot(lo..hi), C]
+ * at the vertices of the target-level lattice: out [T, feats_at_level(target), C].
+ * Replaces _core.accumulate_to_level_forward (src/main.cpp:258-280). */
+int ds_accumulate_fwd(int T, int lo, int hi, int target, int C, const float* feat, float* out,
+                      void* stream);
+
+/* Adjoint of ds_accumulate_fwd accumulated into grad_feat.  Replaces
+ * _core.accumulate_to_level_backward (src/main.cpp:282-304). */
+int ds_accumulate_bwd(int T, int lo, int hi, int target, int C, float* grad_feat,
+                      const float* grad_out, void* stream);
+
+/* Degree-2 real SH basis of the per-pixel view direction, enc [P, 9], zero on
+ * background; inv_mvp [B,4,4] row-major.  Replaces _core.encode_view_dir_sh2
+ * (src/main.cpp:158-170). */
+int ds_sh2(int B, const int* H, const int* W, const float* rast, const float* inv_mvp, float* enc,
+           void* stream);
+
+/* Edge gradients w.r.t. clip-space positions, accumulated into grad_pos
+ * [B,V,4]; color and grad_color are [P, C].  Replaces
+ * _core.backward_edge_grad (src/main.cpp:112-130). */
+int ds_edge_grad_bwd(int B, const int* H, const int* W, int C, const float* color,
+                     const float* grad_color, const float* rast, int V, const float* pos,
+                     float* grad_pos, int T, const int* tri, void* stream);
+
+/* Fill n floats with value (a kernel, so it is safe inside graph capture). */
+int ds_fill_f32(float* p, int64_t n, float value, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFSOUP_HIP_H */

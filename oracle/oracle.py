@@ -1,0 +1,269 @@
+"""CPU oracle for DiffSoup's rasterizer hot path (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker or the timed CPU
+baseline, never as the thing measured or shipped.  The product path
+(``diffsoup_amd``) must fail loudly when its HIP library is missing rather
+than fall back to anything here.
+
+The numeric kernels live in ``ds_oracle.c`` (plain C, strict IEEE fp32, no
+contraction), wrapped here with ctypes over NumPy arrays.  The compositions
+below restate the reference's Python layer:
+
+* ``rasterize`` - python/diffsoup/multires.py:52-102 with
+  python/diffsoup/rasterize.py:40-126 (rects -> fragments -> compaction ->
+  opacity -> threshold -> depth test), with the counter-based hash threshold
+  in place of torch.rand and the compaction order fixed to
+  (view, triangle, rectangle position).
+* ``opacity_aux_grad`` - python/diffsoup/rasterize.py:148-196.
+* ``shade_reference`` - ColorMLP (python/diffsoup/multires.py:109-207) and the
+  training-step composition of examples/02_synthetic.py:354-380, evaluated in
+  float64 with torch on the CPU.
+
+Pinning: see the oracle tests (tests/test_oracle_*.py) - closed forms,
+brute force, the SplitMix64 known-answer vector and FP64 finite differences.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libds_oracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_c = ctypes
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.dso_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        L.dso_splitmix64_next.restype = ctypes.c_uint64
+        L.dso_mask_threshold.argtypes = [_c.c_uint64, _c.c_uint32, _c.c_uint32, _c.c_uint32, _c.c_uint32]
+        L.dso_mask_threshold.restype = _c.c_float
+        L.dso_triangle_rects.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _c.c_int, _i32p, _i32p, _i64p]
+        L.dso_triangle_rects.restype = _c.c_int64
+        L.dso_fragments.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _c.c_int, _i32p, _i32p, _i32p, _f32p]
+        L.dso_fragments.restype = _c.c_int64
+        L.dso_intersect.argtypes = [_c.c_float, _c.c_float, _f32p, _f32p, _f32p] + [ctypes.POINTER(_c.c_float)] * 4
+        L.dso_intersect.restype = _c.c_int
+        L.dso_feats_at_level.argtypes = [_c.c_uint32]
+        L.dso_feats_at_level.restype = _c.c_uint32
+        L.dso_interp.argtypes = [_c.c_float, _c.c_float, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p]
+        L.dso_multires_alpha.argtypes = [_c.c_int64, _f32p, _c.c_int, _c.c_int, _f32p, _f32p]
+        L.dso_mask_thresholds.argtypes = [_c.c_int64, _i32p, _f32p, _c.c_int, _c.c_uint64, _f32p]
+        L.dso_depth_test.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int64, _i32p, _f32p, _f32p, _f32p, _f32p]
+        L.dso_multires_color.argtypes = [_c.c_int64, _f32p, _c.c_int, _c.c_int, _c.c_int, _f32p, _c.c_int, _f32p]
+        L.dso_multires_color_bwd.argtypes = [_c.c_int64, _f32p, _c.c_int, _c.c_int, _c.c_int, _f32p, _c.c_int, _f32p]
+        L.dso_accumulate_fwd.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p]
+        L.dso_accumulate_bwd.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p]
+        L.dso_sh2.argtypes = [_c.c_int, _c.c_int, _c.c_int, _f32p, _f32p, _f32p]
+        L.dso_edge_grad.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p, _f32p, _c.c_int, _f32p, _f32p, _c.c_int, _i32p]
+        L.dso_opacity_aux.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p, _f32p, _c.c_int64,
+                                      _i32p, _f32p, _f32p, _c.c_int, _c.c_int, _f32p]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def feats_at_level(level: int) -> int:
+    return int(lib().dso_feats_at_level(level))
+
+
+def feats_in_levels(lo: int, hi: int) -> int:
+    return sum(feats_at_level(l) for l in range(lo, hi + 1))
+
+
+def splitmix64(seed: int, n: int) -> list[int]:
+    st = ctypes.c_uint64(seed)
+    return [int(lib().dso_splitmix64_next(ctypes.byref(st))) for _ in range(n)]
+
+
+def mask_threshold(seed: int, view: int, tri: int, y: int, x: int) -> float:
+    return float(lib().dso_mask_threshold(seed, view, tri, y, x))
+
+
+def intersect(ndc_y, ndc_x, p0, p1, p2):
+    out = [ctypes.c_float() for _ in range(4)]
+    ok = lib().dso_intersect(ndc_y, ndc_x, _f32(p0), _f32(p1), _f32(p2), *[ctypes.byref(o) for o in out])
+    return bool(ok), tuple(o.value for o in out)
+
+
+def triangle_rects(H, W, pos, tri):
+    """Reference rects (src/cuda/rasterize.cu:43-137): [B*T,4] (h0,h_len,w0,w_len), counts."""
+    pos = _f32(pos)
+    tri = _i32(tri)
+    B, V, _ = pos.shape
+    T = tri.shape[0]
+    rects = np.zeros((B * T, 4), np.int32)
+    counts = np.zeros(B * T, np.int64)
+    total = lib().dso_triangle_rects(H, W, B, V, pos, T, tri, rects, counts)
+    return rects, counts, int(total)
+
+
+def fragments(H, W, pos, tri):
+    """Valid fragments in canonical order: frag_pix [N,3] (view,y,x), frag_attrs [N,4]."""
+    pos = _f32(pos)
+    tri = _i32(tri)
+    B, V, _ = pos.shape
+    T = tri.shape[0]
+    rects, _, total = triangle_rects(H, W, pos, tri)
+    fp = np.zeros((max(total, 1), 3), np.int32)
+    fa = np.zeros((max(total, 1), 4), np.float32)
+    n = lib().dso_fragments(H, W, B, V, pos, T, tri, rects, fp, fa)
+    return fp[:n].copy(), fa[:n].copy()
+
+
+def multires_alpha(frag_attrs, level, alpha_src):
+    alpha_src = _f32(alpha_src).reshape(alpha_src.shape[0], -1)
+    N = frag_attrs.shape[0]
+    out = np.zeros(max(N, 1), np.float32)
+    lib().dso_multires_alpha(N, _f32(frag_attrs), level, alpha_src.shape[1], alpha_src, out)
+    return out[:N]
+
+
+def thresholds(frag_pix, frag_attrs, stochastic, seed):
+    N = frag_pix.shape[0]
+    out = np.zeros(max(N, 1), np.float32)
+    lib().dso_mask_thresholds(N, _i32(frag_pix), _f32(frag_attrs), int(bool(stochastic)), seed, out)
+    return out[:N]
+
+
+def rasterize(resolution, pos, tri, level, alpha_src, stochastic=True, seed=0, return_frags=False):
+    """rast [B,H,W,4] for one resolution shared by all views."""
+    H, W = resolution
+    pos = _f32(pos)
+    B = pos.shape[0]
+    fp, fa = fragments(H, W, pos, tri)
+    alpha = multires_alpha(fa, level, alpha_src)
+    thr = thresholds(fp, fa, stochastic, seed)
+    rast = np.zeros((B, H, W, 4), np.float32)
+    lib().dso_depth_test(B, H, W, fp.shape[0], _i32(fp) if fp.size else np.zeros((1, 3), np.int32), _f32(fa) if fa.size else np.zeros((1, 4), np.float32),
+                         alpha if alpha.size else np.zeros(1, np.float32), thr if thr.size else np.zeros(1, np.float32), rast)
+    if return_frags:
+        return rast, (fp, fa, alpha, thr)
+    return rast
+
+
+def multires_color(rast, lo, hi, feat, concat=False):
+    rast = _f32(rast)
+    feat = _f32(feat)
+    C = feat.shape[2]
+    P = rast.size // 4
+    width = (hi - lo + 1) * C if concat else C
+    out = np.zeros((P, width), np.float32)
+    lib().dso_multires_color(P, rast, lo, hi, C, feat, int(concat), out)
+    return out.reshape(rast.shape[:-1] + (width,))
+
+
+def multires_color_bwd(rast, lo, hi, feat_shape, grad_out, concat=False):
+    rast = _f32(rast)
+    g = np.zeros(feat_shape, np.float32)
+    P = rast.size // 4
+    lib().dso_multires_color_bwd(P, rast, lo, hi, feat_shape[2], g, int(concat), _f32(grad_out))
+    return g
+
+
+def accumulate_fwd(lo, hi, target, feat):
+    feat = _f32(feat)
+    T, _, C = feat.shape
+    out = np.zeros((T, feats_at_level(target), C), np.float32)
+    lib().dso_accumulate_fwd(T, lo, hi, target, C, feat, out)
+    return out
+
+
+def accumulate_bwd(lo, hi, target, feat_shape, grad_out):
+    g = np.zeros(feat_shape, np.float32)
+    T, _, C = feat_shape
+    lib().dso_accumulate_bwd(T, lo, hi, target, C, g, _f32(grad_out))
+    return g
+
+
+def sh2(rast, inv_mvp):
+    rast = _f32(rast)
+    B, H, W, _ = rast.shape
+    out = np.zeros((B, H, W, 9), np.float32)
+    lib().dso_sh2(B, H, W, rast, _f32(inv_mvp), out)
+    return out
+
+
+def edge_grad(color, grad_color, rast, pos, tri):
+    color = _f32(color)
+    B, H, W, C = color.shape
+    pos = _f32(pos)
+    V = pos.shape[1]
+    tri = _i32(tri)
+    gp = np.zeros_like(pos)
+    lib().dso_edge_grad(B, H, W, C, color, _f32(grad_color), _f32(rast), V, pos, gp, tri.shape[0], tri)
+    return gp
+
+
+def opacity_aux_grad(color, target, rast, pos, tri, level, alpha_src):
+    """Unscaled grad w.r.t. alpha_src [T,S] (python/diffsoup/rasterize.py:162-181)."""
+    color = _f32(color)
+    B, H, W, C = color.shape
+    alpha_src = _f32(alpha_src).reshape(alpha_src.shape[0], -1)
+    fp, fa = fragments(H, W, pos, tri)
+    alpha = multires_alpha(fa, level, alpha_src)
+    g = np.zeros_like(alpha_src)
+    N = fp.shape[0]
+    if N:
+        lib().dso_opacity_aux(B, H, W, C, color, _f32(target), _f32(rast), N, _i32(fp), _f32(fa), alpha, level,
+                              alpha_src.shape[1], g)
+    return g
+
+
+def covered_fragments(H, W, pos, tri) -> int:
+    return int(fragments(H, W, pos, tri)[0].shape[0])
+
+
+# ---------------------------------------------------------------------------
+# Shading reference in float64 (torch on the CPU): ColorMLP + composite + MSE.
+# ---------------------------------------------------------------------------
+
+def shade_reference(x, covered, weights, target, background=1.0):
+    """x [P, Din] features, covered [P] bool, weights = [(W1,b1),(W2,b2),(W3,b3)]
+    (torch layout, W [out,in]); returns color [P,3] and loss (MSE mean over
+    P*3) in float64, plus the MLP as a torch function for autograd checks.
+
+    ColorMLP.forward (python/diffsoup/multires.py:185-207): covered pixels get
+    (1 - x[:,3]) * x[:,:3] + x[:,3] * sigmoid(mlp(x)); the driver composites
+    a white background (examples/02_synthetic.py:367-368).
+    """
+    import torch
+
+    xt = torch.as_tensor(np.asarray(x, np.float64))
+    cov = torch.as_tensor(np.asarray(covered, bool))
+    h = xt
+    Ws = [(torch.as_tensor(np.asarray(w, np.float64)), torch.as_tensor(np.asarray(b, np.float64))) for w, b in weights]
+    for i, (w, b) in enumerate(Ws):
+        h = h @ w.T + b
+        h = torch.relu(h) if i < len(Ws) - 1 else torch.sigmoid(h)
+    res = xt[:, 3:4]
+    blended = (1.0 - res) * xt[:, :3] + res * h
+    color = torch.where(cov[:, None], blended, torch.full_like(blended, background))
+    tgt = torch.as_tensor(np.asarray(target, np.float64))
+    loss = ((color - tgt) ** 2).mean()
+    return color.numpy(), float(loss)
