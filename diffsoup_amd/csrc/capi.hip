@@ -80,6 +80,11 @@ int check_levels(int lo, int hi) {
 
 }  // namespace
 
+namespace ds {
+int step_make_views(const ds_step_desc* d, Views& v) { return make_views(d->B, d->H, d->W, d->tile, v); }
+int step_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace ds
+
 extern "C" {
 
 const char* ds_version(void) { return "diffsoup-mi355x 0.1.0"; }

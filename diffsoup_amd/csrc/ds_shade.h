@@ -1,0 +1,39 @@
+// Launch-side interface of the fused shading kernels (shade.hip) and the
+// training-step helpers (step.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ds {
+
+struct ShadeArgs {
+    const float4* rast;     // [P] (b0, b1, depth, id + 1)
+    const int* covered;     // [P] covered pixel indices
+    const int* n_covered;   // device count of covered pixels
+    const float* feat;      // [T, S_tot, C]
+    float* grad_feat;       // [T, S_tot, C] accumulated
+    int S_tot;
+    int lo;                 // first feature level
+    const float* blobs;     // per-lane weight operands (k_blobs)
+    const float* b1;
+    const float* b2;
+    const float* b3;
+    const float* target;    // [P, 3]
+    float* color;           // [P, 3]
+    float* grad_color;      // [P, 3]
+    float* pix_loss;        // [P]
+    float grad_scale;       // 2 / (3 P)
+    float* partials;        // [nblocks, nparam]
+};
+
+bool shade_supported(int nl, int c, int dh);
+size_t shade_blob_floats(int nl, int c, int dh);
+int shade_nparam(int nl, int c, int dh);
+hipError_t launch_shade_blobs(int nl, int c, int dh, bool bf16, const float* w1, const float* w2,
+                              const float* w3, float* blobs, hipStream_t s);
+hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_reduce_partials(const float* partials, int nblocks, int nparam, float* out, hipStream_t s);
+
+}  // namespace ds

@@ -1,0 +1,500 @@
+// Fused per-pixel shading for the training step: multires feature gather ->
+// pixel MLP (Linear-ReLU-Linear-ReLU-Linear-Sigmoid) -> residual blend ->
+// MSE against the target, and the whole adjoint (MLP input and weight
+// gradients, feature-lattice scatter) in the same pass.
+//
+// Work unit: one wave shades a tile of 16 covered pixels (the list the
+// training-step resolve kernel appends).  Every layer is an MFMA product with
+// the layer's output units on the M axis and the 16 pixels on the N axis.  In
+// the accumulator layout of v_mfma_f32_16x16x{4f32,16bf16} lane l holds rows
+// 4(l/16) + r, r = 0..3, of pixel l % 16, which is exactly the B-operand
+// layout the next layer needs (with the K order of the f32 variant permuted),
+// so activations stay in registers from the feature gather to the MLP input
+// gradient.  Weights are pre-arranged per lane ("blobs") once per step and
+// staged in LDS.  Weight gradients contract over pixels, so the per-pixel
+// activations and deltas are transposed through a per-wave LDS tile and fed
+// to MFMAs whose accumulators persist across all tiles a wave shades; one
+// partial per workgroup is reduced by k_reduce_partials.
+//
+// Reference behaviour: ColorMLP (python/diffsoup/multires.py:109-207),
+// multires_triangle_color (src/cuda/multires.cu:224-312, here per level
+// concatenated), the training-step composition of
+// examples/02_synthetic.py:354-388 with an MSE loss, and the feature scatter
+// of src/cuda/multires.cu:70-128.
+#include "ds_kernels.h"
+#include "ds_math.h"
+#include "ds_shade.h"
+
+namespace ds {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Compile-time shape of one shading configuration.
+template <int NL_, int C_, int DH_, bool BF16_>
+struct Cfg {
+    static constexpr int NL = NL_, C = C_, DH = DH_;
+    static constexpr bool BF16 = BF16_;
+    static constexpr int DIN = NL * C;
+    static constexpr int NI = (DIN + 15) / 16;  // input tiles
+    static constexpr int NH = DH / 16;          // hidden tiles
+    static constexpr int DINP = 16 * NI;
+    // blob layer table: (out tiles, in tiles) for L1, L2, L3, L3T, L2T, L1T
+    static constexpr int L1 = 0, L2 = NH * NI, L3 = L2 + NH * NH, L3T = L3 + NH, L2T = L3T + NH,
+                         L1T = L2T + NH * NH, NTILES = L1T + NI * NH;
+    // operand units of one (m, t) tile pair: 4 f32 steps or one bf16 step of 4 values
+    static constexpr int TILE_ELEMS = 4 * kWave;
+    static constexpr int BLOB_ELEMS = NTILES * TILE_ELEMS;
+    static constexpr int NPARAM = DH * DIN + DH * DH + 3 * DH + DH + DH + 3 + 1;
+    // per-wave LDS staging for the weight-gradient transposes (floats)
+    static constexpr int SX = DINP + 4, SH = DH + 4, SD3 = 16 + 4;
+    static constexpr int STAGE = 16 * (SX + 4 * SH + SD3);
+    static_assert(C % 4 == 0, "channels per level must be a multiple of 4");
+    static_assert(DH % 16 == 0, "hidden width must be a multiple of 16");
+};
+
+// One MFMA step: acc += A * B on a 16x16 tile.  For f32 an operand unit is a
+// single float (K = 4 per step over 4 lane groups), for bf16 four values
+// (K = 16 per step).
+template <bool BF16>
+struct Mma;
+
+template <>
+struct Mma<false> {
+    // in: the input tile t of the chained activations (D layout, 4 floats)
+    // blob: per-lane values for (m, t): 4 consecutive steps r, 64 lanes each
+    __device__ static __forceinline__ f32x4 tile(const float* blob, int lane, f32x4 in, f32x4 acc) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(blob[0 * kWave + lane], in[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(blob[1 * kWave + lane], in[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(blob[2 * kWave + lane], in[2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(blob[3 * kWave + lane], in[3], acc, 0, 0, 0);
+        return acc;
+    }
+    // weight-gradient step over 16 pixels: A[i][k] = a_lds(k = pixel, i), B[k][j] = b_lds(pixel, j)
+    __device__ static __forceinline__ f32x4 outer(const float* A, int sa, const float* Bm, int sb, int lane,
+                                                  f32x4 acc) {
+        const int i = lane % 16, g = lane / 16;
+        for (int s = 0; s < 4; ++s) {
+            const int px = 4 * s + g;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[px * sa + i], Bm[px * sb + i], acc, 0, 0, 0);
+        }
+        return acc;
+    }
+};
+
+template <>
+struct Mma<true> {
+    __device__ static __forceinline__ i16x4 pack(f32x4 v) {
+        i16x4 r;
+        r[0] = static_cast<short>(bf16_rne(v[0]));
+        r[1] = static_cast<short>(bf16_rne(v[1]));
+        r[2] = static_cast<short>(bf16_rne(v[2]));
+        r[3] = static_cast<short>(bf16_rne(v[3]));
+        return r;
+    }
+    __device__ static __forceinline__ f32x4 tile(const float* blob, int lane, f32x4 in, f32x4 acc) {
+        const i16x4 a = reinterpret_cast<const i16x4*>(blob)[lane];
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, pack(in), acc, 0, 0, 0);
+    }
+    __device__ static __forceinline__ f32x4 outer(const float* A, int sa, const float* Bm, int sb, int lane,
+                                                  f32x4 acc) {
+        const int i = lane % 16, g = lane / 16;
+        f32x4 a, b;
+        for (int e = 0; e < 4; ++e) {
+            a[e] = A[(4 * g + e) * sa + i];
+            b[e] = Bm[(4 * g + e) * sb + i];
+        }
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pack(a), pack(b), acc, 0, 0, 0);
+    }
+};
+
+// Offsets (in floats) of a layer tile pair inside the LDS blob copy.  A bf16
+// tile pair holds 64 lanes x 4 bf16 = 128 floats' worth of storage but is
+// laid out in a 256-float slot so that both precisions share the table.
+template <class K>
+__device__ __forceinline__ const float* blob_at(const float* blobs, int layer_base, int m, int nin, int t) {
+    return blobs + (layer_base + m * nin + t) * K::TILE_ELEMS;
+}
+
+// y[m] = bias + sum_t W(m, t) * in[t] for NOUT output tiles and NIN input tiles.
+template <class K, int NOUT, int NIN>
+__device__ __forceinline__ void dense(const float* blobs, int base, int lane, const f32x4 (&in)[NIN],
+                                      f32x4 (&out)[NOUT]) {
+    for (int m = 0; m < NOUT; ++m)
+        for (int t = 0; t < NIN; ++t)
+            out[m] = Mma<K::BF16>::tile(blob_at<K>(blobs, base, m, NIN, t), lane, in[t], out[m]);
+}
+
+template <class K>
+__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+k_shade(ShadeArgs a) {
+    constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
+    extern __shared__ float smem[];
+    float* blobs = smem;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int g = lane / 16, j = lane % 16;
+    float* stage = smem + K::BLOB_ELEMS + wave * K::STAGE;
+    float* sX = stage;
+    float* sH1 = sX + 16 * K::SX;
+    float* sH2 = sH1 + 16 * K::SH;
+    float* sD1 = sH2 + 16 * K::SH;
+    float* sD2 = sD1 + 16 * K::SH;
+    float* sD3 = sD2 + 16 * K::SH;
+
+    // stage the weight blobs
+    {
+        const float4* src = reinterpret_cast<const float4*>(a.blobs);
+        float4* dst = reinterpret_cast<float4*>(blobs);
+        for (int i = threadIdx.x; i < K::BLOB_ELEMS / 4; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+
+    f32x4 gW1[NH][NI], gW2[NH][NH], gW3[NH], gb1[NH], gb2[NH], gb3;
+    for (int m = 0; m < NH; ++m) {
+        for (int n = 0; n < NI; ++n) gW1[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int n = 0; n < NH; ++n) gW2[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        gW3[m] = gb1[m] = gb2[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    gb3 = f32x4{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+
+    // level offsets inside a triangle's feature row
+    uint32_t loff[K::NL];
+    {
+        uint32_t o = 0;
+        for (int k = 0; k < K::NL; ++k) { loff[k] = o; o += feats_at_level(a.lo + k); }
+    }
+
+    const int n = *a.n_covered;
+    const int n_waves = gridDim.x * kWavesPerBlock;
+    for (int tile = blockIdx.x * kWavesPerBlock + wave; tile * 16 < n; tile += n_waves) {
+        const int idx = tile * 16 + j;
+        const bool valid = idx < n;
+        int p = 0, t = 0;
+        float b0 = 0.f, b1 = 0.f;
+        if (valid) {
+            p = a.covered[idx];
+            const float4 r4 = a.rast[p];
+            b0 = r4.x; b1 = r4.y;
+            t = static_cast<int>(r4.w) - 1;
+        }
+        const float* frow = a.feat + (long long)t * a.S_tot * C;
+
+        // ---- gather: lane (g, j) holds features 16 tt + 4 g + r of pixel j
+        f32x4 x[NI];
+        for (int tt = 0; tt < NI; ++tt) {
+            x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            const int f0 = 16 * tt + 4 * g;
+            if (valid && f0 < DIN) {
+                const int lv = f0 / C, c0 = f0 % C;
+                const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
+                const float* fl = frow + (long long)loff[lv] * C + c0;
+                const float4 v0 = *reinterpret_cast<const float4*>(fl + cr.idx[0] * C);
+                const float4 v1 = *reinterpret_cast<const float4*>(fl + cr.idx[1] * C);
+                const float4 v2 = *reinterpret_cast<const float4*>(fl + cr.idx[2] * C);
+                float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+                e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
+                e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;
+                e2 += cr.w[0] * v0.z; e2 += cr.w[1] * v1.z; e2 += cr.w[2] * v2.z;
+                e3 += cr.w[0] * v0.w; e3 += cr.w[1] * v1.w; e3 += cr.w[2] * v2.w;
+                x[tt] = f32x4{e0, e1, e2, e3};
+            }
+        }
+
+        // ---- forward
+        f32x4 h1[NH], h2[NH], y[1];
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) { h1[m][r] = a.b1[16 * m + 4 * g + r]; h2[m][r] = a.b2[16 * m + 4 * g + r]; }
+        for (int r = 0; r < 4; ++r) y[0][r] = (g == 0 && r < 3) ? a.b3[r] : 0.f;
+        dense<K, NH, NI>(blobs, K::L1, lane, x, h1);
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) h1[m][r] = fmaxf(h1[m][r], 0.f);
+        dense<K, NH, NH>(blobs, K::L2, lane, h1, h2);
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) h2[m][r] = fmaxf(h2[m][r], 0.f);
+        dense<K, 1, NH>(blobs, K::L3, lane, h2, y);
+
+        // ---- blend, loss and output gradient (lanes of group 0 own pixel j)
+        f32x4 d3[1] = {f32x4{0.f, 0.f, 0.f, 0.f}};
+        float dxr[4] = {0.f, 0.f, 0.f, 0.f};
+        if (g == 0 && valid) {
+            const float res = x[0][3];
+            const float* tg = a.target + (long long)p * 3;
+            float* col = a.color + (long long)p * 3;
+            float* gc = a.grad_color + (long long)p * 3;
+            float l1 = 0.f, dres = 0.f;
+            for (int c = 0; c < 3; ++c) {
+                const float s = sigmoidf(y[0][c]);
+                const float rgb = x[0][c];
+                const float out = (1.0f - res) * rgb + res * s;
+                const float d = out - tg[c];
+                loss += d * d;
+                l1 += fabsf(d);
+                const float go = a.grad_scale * d;
+                col[c] = out;
+                gc[c] = go;
+                dxr[c] = go * (1.0f - res);
+                dres += go * (s - rgb);
+                d3[0][c] = go * res * s * (1.0f - s);
+            }
+            dxr[3] = dres;
+            a.pix_loss[p] = l1;
+        }
+
+        // ---- backward through the MLP
+        f32x4 dh2[NH], dh1[NH], dx[NI];
+        for (int m = 0; m < NH; ++m) dh2[m] = dh1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int m = 0; m < NI; ++m) dx[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        dense<K, NH, 1>(blobs, K::L3T, lane, d3, dh2);
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) dh2[m][r] = h2[m][r] > 0.f ? dh2[m][r] : 0.f;
+        dense<K, NH, NH>(blobs, K::L2T, lane, dh2, dh1);
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) dh1[m][r] = h1[m][r] > 0.f ? dh1[m][r] : 0.f;
+        dense<K, NI, NH>(blobs, K::L1T, lane, dh1, dx);
+        if (g == 0)
+            for (int r = 0; r < 4; ++r) dx[0][r] += dxr[r];
+
+        // ---- scatter the input gradient onto the feature lattice
+        if (valid) {
+            float* grow = a.grad_feat + (long long)t * a.S_tot * C;
+            for (int tt = 0; tt < NI; ++tt) {
+                const int f0 = 16 * tt + 4 * g;
+                if (f0 >= DIN) continue;
+                const int lv = f0 / C, c0 = f0 % C;
+                const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
+                float* gl = grow + (long long)loff[lv] * C + c0;
+                for (int i = 0; i < 3; ++i)
+                    for (int r = 0; r < 4; ++r)
+                        unsafeAtomicAdd(&gl[cr.idx[i] * C + r], dx[tt][r] * cr.w[i]);
+            }
+        }
+
+        // ---- bias gradients stay per lane until the end
+        for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
+        gb3 += d3[0];
+
+        // ---- weight gradients: transpose through LDS, contract over the 16 pixels
+        for (int tt = 0; tt < NI; ++tt)
+            for (int r = 0; r < 4; ++r) sX[j * K::SX + 16 * tt + 4 * g + r] = x[tt][r];
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) {
+                const int u = 16 * m + 4 * g + r;
+                sH1[j * K::SH + u] = h1[m][r];
+                sH2[j * K::SH + u] = h2[m][r];
+                sD1[j * K::SH + u] = dh1[m][r];
+                sD2[j * K::SH + u] = dh2[m][r];
+            }
+        for (int r = 0; r < 4; ++r) sD3[j * K::SD3 + 4 * g + r] = d3[0][r];
+        wave_lds_sync();
+        for (int m = 0; m < NH; ++m) {
+            for (int nn = 0; nn < NI; ++nn)
+                gW1[m][nn] = Mma<K::BF16>::outer(sD1 + 16 * m, K::SH, sX + 16 * nn, K::SX, lane, gW1[m][nn]);
+            for (int nn = 0; nn < NH; ++nn)
+                gW2[m][nn] = Mma<K::BF16>::outer(sD2 + 16 * m, K::SH, sH1 + 16 * nn, K::SH, lane, gW2[m][nn]);
+            gW3[m] = Mma<K::BF16>::outer(sD3, K::SD3, sH2 + 16 * m, K::SH, lane, gW3[m]);
+        }
+        wave_lds_sync();
+    }
+
+    // ---- reduce the bias gradients over the 16 pixels of each lane group
+    for (int m = 0; m < NH; ++m)
+        for (int r = 0; r < 4; ++r)
+            for (int d = 8; d > 0; d >>= 1) {
+                gb1[m][r] += __shfl_xor(gb1[m][r], d, 16);
+                gb2[m][r] += __shfl_xor(gb2[m][r], d, 16);
+            }
+    for (int r = 0; r < 4; ++r)
+        for (int d = 8; d > 0; d >>= 1) gb3[r] += __shfl_xor(gb3[r], d, 16);
+    for (int d = 32; d > 0; d >>= 1) loss += __shfl_xor(loss, d, kWave);
+
+    // ---- workgroup reduction into LDS (reusing the staging area), one partial per block
+    __syncthreads();
+    float* red = smem + K::BLOB_ELEMS;
+    for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) red[i] = 0.f;
+    __syncthreads();
+    float* rW1 = red;
+    float* rW2 = rW1 + DH * DIN;
+    float* rW3 = rW2 + DH * DH;
+    float* rb1 = rW3 + 3 * DH;
+    float* rb2 = rb1 + DH;
+    float* rb3 = rb2 + DH;
+    float* rl = rb3 + 3;
+    for (int m = 0; m < NH; ++m) {
+        for (int r = 0; r < 4; ++r) {
+            const int u = 16 * m + 4 * g + r;   // accumulator row
+            for (int nn = 0; nn < NI; ++nn) {
+                const int f = 16 * nn + j;
+                if (f < DIN) atomicAdd(&rW1[u * DIN + f], gW1[m][nn][r]);
+            }
+            for (int nn = 0; nn < NH; ++nn) atomicAdd(&rW2[u * DH + 16 * nn + j], gW2[m][nn][r]);
+            if (j == 0) {
+                atomicAdd(&rb1[u], gb1[m][r]);
+                atomicAdd(&rb2[u], gb2[m][r]);
+            }
+        }
+        // gW3 rows are output channels 4 g + r (only 0..2 exist), columns hidden units
+        if (g == 0)
+            for (int r = 0; r < 3; ++r) atomicAdd(&rW3[r * DH + 16 * m + j], gW3[m][r]);
+    }
+    if (g == 0 && j == 0)
+        for (int r = 0; r < 3; ++r) atomicAdd(&rb3[r], gb3[r]);
+    if (lane == 0) atomicAdd(rl, loss);
+    __syncthreads();
+    float* out = a.partials + (long long)blockIdx.x * K::NPARAM;
+    for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) out[i] = red[i];
+}
+
+// Per-lane weight blobs for one configuration, from the fp32 master weights
+// W1 [DH, DIN], W2 [DH, DH], W3 [3, DH].  Entry (layer, m, t, lane, e) is the
+// A operand for output row 16 m + lane % 16 and input row 16 t + 4 (lane / 16) + e.
+template <class K>
+__global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ w2,
+                        const float* __restrict__ w3, float* __restrict__ blobs) {
+    constexpr int NI = K::NI, NH = K::NH, DIN = K::DIN, DH = K::DH;
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= K::NTILES * kWave) return;
+    const int tile = id / kWave, lane = id % kWave;
+    int base, nin, layer;
+    if (tile < K::L2) { layer = 0; base = K::L1; nin = NI; }
+    else if (tile < K::L3) { layer = 1; base = K::L2; nin = NH; }
+    else if (tile < K::L3T) { layer = 2; base = K::L3; nin = NH; }
+    else if (tile < K::L2T) { layer = 3; base = K::L3T; nin = 1; }
+    else if (tile < K::L1T) { layer = 4; base = K::L2T; nin = NH; }
+    else { layer = 5; base = K::L1T; nin = NH; }
+    const int m = (tile - base) / nin, t = (tile - base) % nin;
+    const int o = 16 * m + lane % 16;
+    float v[4];
+    for (int e = 0; e < 4; ++e) {
+        const int i = 16 * t + 4 * (lane / 16) + e;
+        float w = 0.f;
+        switch (layer) {
+            case 0: w = (i < DIN) ? w1[o * DIN + i] : 0.f; break;
+            case 1: w = w2[o * DH + i]; break;
+            case 2: w = (o < 3) ? w3[o * DH + i] : 0.f; break;
+            case 3: w = (i < 3) ? w3[i * DH + o] : 0.f; break;
+            case 4: w = w2[i * DH + o]; break;
+            case 5: w = (o < DIN) ? w1[i * DIN + o] : 0.f; break;
+        }
+        v[e] = w;
+    }
+    float* dst = blobs + tile * K::TILE_ELEMS;
+    if constexpr (K::BF16) {
+        unsigned short* d16 = reinterpret_cast<unsigned short*>(dst);
+        for (int e = 0; e < 4; ++e) d16[lane * 4 + e] = bf16_rne(v[e]);
+    } else {
+        for (int e = 0; e < 4; ++e) dst[e * kWave + lane] = v[e];
+    }
+}
+
+__global__ void k_reduce_partials(const float* __restrict__ partials, int nblocks, int nparam,
+                                  float* __restrict__ out) {
+    // blockDim = (64, 16): x over 64 parameters, y strides over the partial blocks
+    __shared__ float acc[16][65];
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    float s = 0.f;
+    if (k < nparam)
+        for (int b = threadIdx.y; b < nblocks; b += 16) s += partials[(long long)b * nparam + k];
+    acc[threadIdx.y][threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.y == 0 && k < nparam) {
+        float t = 0.f;
+        for (int r = 0; r < 16; ++r) t += acc[r][threadIdx.x];
+        out[k] = t;
+    }
+}
+
+template <class K>
+hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
+    const size_t lds = sizeof(float) * (K::BLOB_ELEMS + kWavesPerBlock * K::STAGE);
+    const size_t red = sizeof(float) * (K::BLOB_ELEMS + K::NPARAM);
+    const size_t bytes = lds > red ? lds : red;
+    static bool opted_in = false;
+    if (bytes > 65536 && !opted_in) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 static_cast<int>(bytes));
+        if (e != hipSuccess) return e;
+        opted_in = true;
+    }
+    k_shade<K><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a);
+    return hipGetLastError();
+}
+
+template <class K>
+hipError_t blobs_dispatch(const float* w1, const float* w2, const float* w3, float* blobs, hipStream_t s) {
+    const int n = K::NTILES * kWave;
+    k_blobs<K><<<(n + 255) / 256, 256, 0, s>>>(w1, w2, w3, blobs);
+    return hipGetLastError();
+}
+
+// The compiled configurations: (levels, channels per level, hidden width).
+#define DS_SHADE_CONFIGS(X) \
+    X(1, 8, 16)             \
+    X(4, 8, 32)             \
+    X(1, 8, 32)             \
+    X(4, 4, 32)             \
+    X(2, 8, 32)
+
+}  // namespace
+
+bool shade_supported(int nl, int c, int dh) {
+#define X(NL, C, DH) if (nl == NL && c == C && dh == DH) return true;
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return false;
+}
+
+size_t shade_blob_floats(int nl, int c, int dh) {
+#define X(NL, C, DH) if (nl == NL && c == C && dh == DH) return Cfg<NL, C, DH, false>::BLOB_ELEMS;
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return 0;
+}
+
+int shade_nparam(int nl, int c, int dh) { return dh * nl * c + dh * dh + 3 * dh + dh + dh + 3 + 1; }
+
+hipError_t launch_shade_blobs(int nl, int c, int dh, bool bf16, const float* w1, const float* w2,
+                              const float* w3, float* blobs, hipStream_t s) {
+#define X(NL, C, DH)                                                                   \
+    if (nl == NL && c == C && dh == DH)                                                \
+        return bf16 ? blobs_dispatch<Cfg<NL, C, DH, true>>(w1, w2, w3, blobs, s)       \
+                    : blobs_dispatch<Cfg<NL, C, DH, false>>(w1, w2, w3, blobs, s);
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s) {
+#define X(NL, C, DH)                                                                   \
+    if (nl == NL && c == C && dh == DH)                                                \
+        return bf16 ? shade_dispatch<Cfg<NL, C, DH, true>>(a, nblocks, s)              \
+                    : shade_dispatch<Cfg<NL, C, DH, false>>(a, nblocks, s);
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_reduce_partials(const float* partials, int nblocks, int nparam, float* out, hipStream_t s) {
+    dim3 block(64, 16);
+    k_reduce_partials<<<(nparam + 63) / 64, block, 0, s>>>(partials, nblocks, nparam, out);
+    return hipGetLastError();
+}
+
+}  // namespace ds

@@ -1,0 +1,265 @@
+// One fused training step (forward + backward) on the caller's stream:
+//
+//   k_step_prep     sigmoid of the opacity logits, zero the gradient
+//                   accumulators and the covered-pixel counter
+//   k_blobs         per-lane MLP weight operands (shade.hip)
+//   k_project       clip = MVP [x y z 1] for every (view, vertex)
+//   raster bin      setup / scan / emit (raster.hip)
+//   resolve+shade   depth test, background shading, covered-pixel list
+//   k_shade         gather -> MLP -> blend -> MSE, MLP adjoint, feature scatter
+//   edge grad       silhouette / intersection gradients into clip positions
+//   opacity aux     score-function opacity gradient (same bins, no re-binning)
+//   reductions      MLP weight gradients and the loss
+//   k_finalize      clip -> world vertex gradients, sigmoid backward
+//
+// No allocation, no host synchronisation: everything lives in the caller's
+// workspace and output buffers, so the sequence can be captured in a graph.
+// Composition follows examples/02_synthetic.py:354-388 with MSE in place of
+// L1 + SSIM and the features concatenated per level.
+#include "../../include/diffsoup_hip.h"
+#include "ds_kernels.h"
+#include "ds_math.h"
+#include "ds_shade.h"
+
+namespace ds {
+namespace {
+
+__global__ void k_step_prep(const float* __restrict__ logits, float* __restrict__ alpha, long long n_alpha,
+                            float* __restrict__ gfeat, long long n_feat, float* __restrict__ galpha,
+                            float* __restrict__ gpos, long long n_pos, int* __restrict__ n_covered) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long i = i0; i < n_alpha; i += stride) {
+        alpha[i] = 1.0f / (1.0f + expf(-logits[i]));
+        galpha[i] = 0.f;
+    }
+    float4* gf4 = reinterpret_cast<float4*>(gfeat);
+    for (long long i = i0; i < n_feat / 4; i += stride) gf4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (long long i = (n_feat / 4) * 4 + i0; i < n_feat; i += stride) gfeat[i] = 0.f;
+    float4* gp4 = reinterpret_cast<float4*>(gpos);
+    for (long long i = i0; i < n_pos / 4; i += stride) gp4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i0 == 0) *n_covered = 0;
+}
+
+// clip[b, v] = M_b [x, y, z, 1], row-major M (examples/utils.py:412-426)
+__global__ void k_project(int B, int V, const float* __restrict__ verts, const float* __restrict__ mvp,
+                          float4* __restrict__ pos) {
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= (long long)B * V) return;
+    const int b = static_cast<int>(id / V);
+    const int v = static_cast<int>(id - (long long)b * V);
+    const float x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
+    const float* m = mvp + 16 * b;
+    float o[4];
+    for (int i = 0; i < 4; ++i) o[i] = m[4 * i] * x + m[4 * i + 1] * y + m[4 * i + 2] * z + m[4 * i + 3];
+    pos[id] = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// grad_verts[v] = sum_b (M_b^T grad_pos[b, v])[0:3]; grad_logits = scale * g * s (1 - s)
+__global__ void k_finalize(int B, int V, const float* __restrict__ mvp, const float4* __restrict__ gpos,
+                           float* __restrict__ gverts, long long n_alpha, const float* __restrict__ alpha,
+                           const float* __restrict__ galpha, float aux_scale, float* __restrict__ glogits) {
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id < V) {
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        for (int b = 0; b < B; ++b) {
+            const float4 g = gpos[(long long)b * V + id];
+            const float* m = mvp + 16 * b;
+            gx += m[0] * g.x + m[4] * g.y + m[8] * g.z + m[12] * g.w;
+            gy += m[1] * g.x + m[5] * g.y + m[9] * g.z + m[13] * g.w;
+            gz += m[2] * g.x + m[6] * g.y + m[10] * g.z + m[14] * g.w;
+        }
+        gverts[3 * id] = gx;
+        gverts[3 * id + 1] = gy;
+        gverts[3 * id + 2] = gz;
+    }
+    if (id < n_alpha) {
+        const float s = alpha[id];
+        glogits[id] = aux_scale * galpha[id] * (s * (1.0f - s));
+    }
+}
+
+// loss = (sum of background partials + shaded-pixel loss) * scale; one workgroup.
+__global__ void __launch_bounds__(1024) k_loss(const float* __restrict__ bg, int n_bg,
+                                               const float* __restrict__ fg, float scale,
+                                               float* __restrict__ loss) {
+    __shared__ float red[16];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n_bg; i += blockDim.x) s += bg[i];
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int k = 0; k < (int)(blockDim.x / 64); ++k) t += red[k];
+        *loss = (t + *fg) * scale;
+    }
+}
+
+inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct StepLayout {
+    size_t raster, gpos, galpha, covered, ncov, bgloss, blobs, partials, total;
+    int n_tiles, nblocks, nparam;
+    long long P;
+};
+
+int shade_blocks(long long P) {
+    long long nb = (P + 63) / 64;
+    if (nb > 512) nb = 512;
+    if (nb < 1) nb = 1;
+    return static_cast<int>(nb);
+}
+
+bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
+    const int nl = d->feat_hi - d->feat_lo + 1;
+    if (!shade_supported(nl, d->channels, d->hidden)) return false;
+    L.n_tiles = v.tile_base[v.n];
+    L.P = v.pix_base[v.n];
+    L.nblocks = shade_blocks(L.P);
+    L.nparam = shade_nparam(nl, d->channels, d->hidden);
+    const long long Sa = feats_at_level(d->alpha_level);
+    size_t o = 0;
+    L.raster = o; o += up256(raster_workspace_bytes(d->B, d->T, L.n_tiles, d->capacity));
+    L.gpos = o; o += up256(sizeof(float) * 4 * (size_t)d->B * d->V);
+    L.galpha = o; o += up256(sizeof(float) * (size_t)d->T * Sa);
+    L.covered = o; o += up256(sizeof(int) * (size_t)L.P);
+    L.ncov = o; o += 256;
+    L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
+    L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(nl, d->channels, d->hidden));
+    L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
+    L.total = o;
+    return true;
+}
+
+}  // namespace
+
+// Exported through capi.hip's error helpers.
+int step_make_views(const ds_step_desc* d, Views& v);
+int step_fail(int code, const char* msg);
+
+}  // namespace ds
+
+extern "C" {
+
+int64_t ds_step_mlp_params(int levels, int channels, int hidden) {
+    return ds::shade_nparam(levels, channels, hidden);
+}
+
+size_t ds_step_workspace_size(const ds_step_desc* d) {
+    ds::Views v;
+    if (!d || ds::step_make_views(d, v) != DS_OK) return 0;
+    ds::StepLayout L;
+    if (!ds::layout(d, v, L)) return 0;
+    return L.total;
+}
+
+int ds_train_step(const ds_step_desc* d, void* stream) {
+    using namespace ds;
+    if (!d) return step_fail(DS_ERR_INVALID, "null step descriptor");
+    Views v;
+    int rc = step_make_views(d, v);
+    if (rc != DS_OK) return rc;
+    const int nl = d->feat_hi - d->feat_lo + 1;
+    if (d->feat_lo < 0 || nl < 1 || d->feat_hi > 10 || d->alpha_level < 0 || d->alpha_level > 10)
+        return step_fail(DS_ERR_INVALID, "level range invalid");
+    if (d->T < 0 || d->V < 0 || d->T > (1 << 24) - 1) return step_fail(DS_ERR_INVALID, "bad T or V");
+    StepLayout L;
+    if (!layout(d, v, L)) return step_fail(DS_ERR_UNSUPPORTED, "no compiled shading kernel for this (levels, channels, hidden)");
+    if (!d->workspace || d->workspace_bytes < L.total)
+        return step_fail(DS_ERR_WORKSPACE, "workspace smaller than ds_step_workspace_size");
+    if (reinterpret_cast<uintptr_t>(d->workspace) & 255u) return step_fail(DS_ERR_INVALID, "workspace must be 256-byte aligned");
+    const void* req[] = {d->verts, d->mvp, d->tri, d->alpha_logits, d->feat, d->w1, d->b1, d->w2, d->b2,
+                         d->w3, d->b3, d->target, d->pos, d->alpha, d->rast, d->color, d->grad_color,
+                         d->pix_loss, d->grad_verts, d->grad_alpha_logits, d->grad_feat, d->grad_mlp, d->loss};
+    for (const void* p : req)
+        if (!p && d->T > 0) return step_fail(DS_ERR_INVALID, "null pointer in step descriptor");
+    if ((reinterpret_cast<uintptr_t>(d->pos) | reinterpret_cast<uintptr_t>(d->rast) |
+         reinterpret_cast<uintptr_t>(d->feat) | reinterpret_cast<uintptr_t>(d->grad_feat)) & 15u)
+        return step_fail(DS_ERR_INVALID, "pos, rast, feat and grad_feat must be 16-byte aligned");
+
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(d->workspace);
+    RasterWorkspace rw = raster_workspace_carve(ws + L.raster, d->B, d->T, L.n_tiles, d->capacity);
+    float* gpos = reinterpret_cast<float*>(ws + L.gpos);
+    float* galpha = reinterpret_cast<float*>(ws + L.galpha);
+    int* covered = reinterpret_cast<int*>(ws + L.covered);
+    int* ncov = reinterpret_cast<int*>(ws + L.ncov);
+    float* bgloss = reinterpret_cast<float*>(ws + L.bgloss);
+    float* blobs = reinterpret_cast<float*>(ws + L.blobs);
+    float* partials = reinterpret_cast<float*>(ws + L.partials);
+
+    const long long Sa = feats_at_level(d->alpha_level);
+    const long long S_tot = feats_in_levels(d->feat_lo, d->feat_hi);
+    const long long n_alpha = (long long)d->T * Sa;
+    const long long n_feat = (long long)d->T * S_tot * d->channels;
+    const long long n_pos = 4LL * d->B * d->V;
+    const float inv_n = 1.0f / (3.0f * static_cast<float>(L.P));
+
+    hipError_t e;
+#define DS_TRY(expr, what) \
+    if ((e = (expr)) != hipSuccess) return step_fail(DS_ERR_HIP, what);
+    auto mark = [&](int stage, bool begin) -> hipError_t {
+        if (d->timed_stage != stage) return hipSuccess;
+        void* ev = begin ? d->ev_begin : d->ev_end;
+        return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
+    };
+
+    const long long prep_n = std::max(std::max(n_alpha, n_feat / 4), std::max(n_pos / 4, 1LL));
+    DS_TRY(mark(DS_STAGE_PREP, true), "event");
+    k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
+        d->alpha_logits, d->alpha, n_alpha, d->grad_feat, n_feat, galpha, gpos, n_pos, ncov);
+    DS_TRY(hipGetLastError(), "step prep");
+    DS_TRY(mark(DS_STAGE_PREP, false), "event");
+    DS_TRY(mark(DS_STAGE_BLOBS, true), "event");
+    DS_TRY(launch_shade_blobs(nl, d->channels, d->hidden, d->bf16 != 0, d->w1, d->w2, d->w3, blobs, s), "blobs");
+    DS_TRY(mark(DS_STAGE_BLOBS, false), "event");
+    DS_TRY(mark(DS_STAGE_PROJECT, true), "event");
+    if ((long long)d->B * d->V > 0) {
+        k_project<<<blocks_for((long long)d->B * d->V, 256), 256, 0, s>>>(d->B, d->V, d->verts, d->mvp,
+                                                                        reinterpret_cast<float4*>(d->pos));
+        DS_TRY(hipGetLastError(), "project");
+    }
+    DS_TRY(mark(DS_STAGE_PROJECT, false), "event");
+    DS_TRY(mark(DS_STAGE_BIN, true), "event");
+    DS_TRY(launch_raster_bin(v, d->V, d->pos, d->T, d->tri, rw, s), "raster bin");
+    DS_TRY(mark(DS_STAGE_BIN, false), "event");
+    AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(Sa)};
+    BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, bgloss, covered, ncov,
+                       2.0f * inv_n, d->background};
+    DS_TRY(mark(DS_STAGE_RESOLVE, true), "event");
+    DS_TRY(launch_raster_resolve_shade(v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, d->seed,
+                                       d->rast, rw, bg, s),
+           "resolve");
+    DS_TRY(mark(DS_STAGE_RESOLVE, false), "event");
+    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), covered, ncov, d->feat, d->grad_feat,
+                 static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->target, d->color,
+                 d->grad_color, d->pix_loss, 2.0f * inv_n, partials};
+    DS_TRY(mark(DS_STAGE_SHADE, true), "event");
+    DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade");
+    DS_TRY(mark(DS_STAGE_SHADE, false), "event");
+    DS_TRY(mark(DS_STAGE_EDGE, true), "event");
+    DS_TRY(launch_edge_grad(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri, s),
+           "edge grad");
+    DS_TRY(mark(DS_STAGE_EDGE, false), "event");
+    DS_TRY(mark(DS_STAGE_AUX, true), "event");
+    DS_TRY(launch_opacity_aux(v, d->T, a, d->rast, d->pix_loss, galpha, rw, s), "opacity aux");
+    DS_TRY(mark(DS_STAGE_AUX, false), "event");
+    DS_TRY(mark(DS_STAGE_FINAL, true), "event");
+    DS_TRY(launch_reduce_partials(partials, L.nblocks, L.nparam, d->grad_mlp, s), "reduce");
+    k_loss<<<1, 1024, 0, s>>>(bgloss, L.n_tiles, d->grad_mlp + (L.nparam - 1), inv_n, d->loss);
+    DS_TRY(hipGetLastError(), "loss");
+    const long long nf = std::max<long long>(d->V, n_alpha);
+    if (nf > 0) {
+        k_finalize<<<blocks_for(nf, 256), 256, 0, s>>>(d->B, d->V, d->mvp, reinterpret_cast<const float4*>(gpos),
+                                                       d->grad_verts, n_alpha, d->alpha, galpha, inv_n,
+                                                       d->grad_alpha_logits);
+        DS_TRY(hipGetLastError(), "finalize");
+    }
+    DS_TRY(mark(DS_STAGE_FINAL, false), "event");
+#undef DS_TRY
+    return DS_OK;
+}
+
+}  // extern "C"

@@ -141,6 +141,84 @@ int ds_edge_grad_bwd(int B, const int* H, const int* W, int C, const float* colo
                      const float* grad_color, const float* rast, int V, const float* pos,
                      float* grad_pos, int T, const int* tri, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Fused training step.  One call enqueues a full forward + backward of the
+ * reference's training iteration (examples/02_synthetic.py:354-388) for B
+ * views of possibly different resolution: projection, rasterization with
+ * stochastic opacity, per-level concatenated multires features, the pixel
+ * MLP (Linear-ReLU-Linear-ReLU-Linear-Sigmoid, hidden width `hidden`, input
+ * (feat_hi-feat_lo+1)*channels, 3 outputs) with the residual blend of
+ * ColorMLP (python/diffsoup/multires.py:185-207), a constant background,
+ * the MSE loss against `target`, the opacity auxiliary gradient, edge
+ * gradients, and all parameter gradients.  bf16 != 0 runs the MLP on bf16
+ * MFMA with fp32 accumulation; everything else is fp32.
+ * Replaces the Python composition of the reference's ops; it has no single
+ * reference counterpart.  All pointers are device pointers except H and W.
+ * ------------------------------------------------------------------------ */
+typedef struct ds_step_desc {
+    int B;                    /* views                                       */
+    const int* H;             /* host [B]                                    */
+    const int* W;             /* host [B]                                    */
+    int tile;                 /* screen tile edge: 8, 16 or 32               */
+    int V;                    /* vertices                                    */
+    const float* verts;       /* [V,3] world positions                       */
+    const float* mvp;         /* [B,4,4] row-major                           */
+    int T;                    /* triangles                                   */
+    const int* tri;           /* [T,3]                                       */
+    int alpha_level;          /* opacity lattice level                       */
+    const float* alpha_logits;/* [T, feats_at_level(alpha_level)]            */
+    int stochastic;           /* hash mask (1) or threshold 0.5 (0)          */
+    uint64_t seed;
+    int feat_lo, feat_hi;     /* feature levels, concatenated in the MLP     */
+    int channels;             /* channels per level                          */
+    const float* feat;        /* [T, S_tot, channels]                        */
+    int hidden;               /* MLP hidden width                            */
+    int bf16;                 /* MLP precision                               */
+    const float* w1; const float* b1;  /* [hidden, in], [hidden]             */
+    const float* w2; const float* b2;  /* [hidden, hidden], [hidden]         */
+    const float* w3; const float* b3;  /* [3, hidden], [3]                   */
+    const float* target;      /* [P,3] packed over views                     */
+    float background;         /* background colour                           */
+    float* pos;               /* out [B,V,4] clip positions                  */
+    float* alpha;             /* out [T,S_a] sigmoid(alpha_logits)           */
+    float* rast;              /* out [P,4]                                   */
+    float* color;             /* out [P,3]                                   */
+    float* grad_color;        /* out [P,3] dLoss/dcolor                      */
+    float* pix_loss;          /* out [P] per-pixel L1                        */
+    float* grad_verts;        /* out [V,3]                                   */
+    float* grad_alpha_logits; /* out [T,S_a]                                 */
+    float* grad_feat;         /* out [T,S_tot,channels]                      */
+    float* grad_mlp;          /* out [ds_step_mlp_params]: dW1,dW2,dW3,db1,db2,db3,loss_sum */
+    float* loss;              /* out [1] mean squared error                  */
+    void* workspace;          /* 256-byte aligned                            */
+    size_t workspace_bytes;
+    int64_t capacity;         /* (triangle, tile) bin capacity               */
+    int timed_stage;          /* DS_STAGE_* bracketed by the events, or 0    */
+    void* ev_begin;           /* hipEvent_t recorded before that stage       */
+    void* ev_end;             /* hipEvent_t recorded after it                */
+} ds_step_desc;
+
+/* Stages of ds_train_step that timed_stage can select. */
+#define DS_STAGE_PREP 1     /* sigmoid, zero accumulators                    */
+#define DS_STAGE_BLOBS 2    /* MLP weight operands                           */
+#define DS_STAGE_PROJECT 3  /* clip-space projection                         */
+#define DS_STAGE_BIN 4      /* triangle setup, tile counts, scan, bin emit   */
+#define DS_STAGE_RESOLVE 5  /* depth test + background shading               */
+#define DS_STAGE_SHADE 6    /* gather + MLP + loss + adjoint + scatter       */
+#define DS_STAGE_EDGE 7     /* edge gradients                                */
+#define DS_STAGE_AUX 8      /* opacity auxiliary gradient                    */
+#define DS_STAGE_FINAL 9    /* reductions, loss, vertex / logit gradients    */
+
+/* Length of the grad_mlp vector for a configuration. */
+int64_t ds_step_mlp_params(int levels, int channels, int hidden);
+
+/* Workspace bytes for ds_train_step; 0 if the configuration is invalid or
+ * has no compiled shading kernel. */
+size_t ds_step_workspace_size(const ds_step_desc* d);
+
+/* Enqueue one fused forward + backward step (see above). */
+int ds_train_step(const ds_step_desc* d, void* stream);
+
 /* Fill n floats with value (a kernel, so it is safe inside graph capture). */
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 

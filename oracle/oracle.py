@@ -267,3 +267,77 @@ def shade_reference(x, covered, weights, target, background=1.0):
     tgt = torch.as_tensor(np.asarray(target, np.float64))
     loss = ((color - tgt) ** 2).mean()
     return color.numpy(), float(loss)
+
+
+# ---------------------------------------------------------------------------
+# Whole training step on the CPU (the timed CPU baseline and the fused-step
+# checker): raster -> concatenated gather -> MLP + blend + MSE (torch, CPU) ->
+# MLP adjoint -> feature scatter -> edge gradients -> opacity gradient.
+# Composition: examples/02_synthetic.py:354-388 with an MSE loss.
+# ---------------------------------------------------------------------------
+
+def project(verts, mvp):
+    """clip = mvp @ [v, 1] evaluated in float32 in the order m0 x + m1 y + m2 z + m3."""
+    v = np.asarray(verts, np.float32)
+    m = np.asarray(mvp, np.float32)
+    out = np.empty((m.shape[0], v.shape[0], 4), np.float32)
+    for i in range(4):
+        out[..., i] = (m[:, None, i, 0] * v[None, :, 0] + m[:, None, i, 1] * v[None, :, 1]
+                       + m[:, None, i, 2] * v[None, :, 2] + m[:, None, i, 3])
+    return out
+
+
+def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, grad_color_in=None):
+    """Full forward + backward of the fused step for equal-resolution views.
+
+    Returns a dict with rast, color, grad_color, loss and the gradients of the
+    MLP (torch layout), features, world vertices and opacity logits.  With
+    color_in / grad_color_in given, the edge and opacity gradients use those
+    images (to isolate kernels from upstream rounding)."""
+    import torch
+
+    dtype = dtype or torch.float64
+    s = soup
+    H, W = s.resolutions[0]
+    B = s.B
+    if pos is None:
+        pos = project(s.verts, s.mvp)
+    if alpha is None:
+        alpha = (1.0 / (1.0 + np.exp(-s.alpha_logits.astype(np.float64)))).astype(np.float32)
+    rast = rasterize((H, W), pos, s.tri, s.alpha_level, alpha, True, seed).reshape(-1, 4)
+    lo, hi = s.levels
+    P = rast.shape[0]
+    X = multires_color(rast, lo, hi, s.feat, True).reshape(P, -1)
+    covered = rast[:, 3] > 0
+    idx = torch.from_numpy(np.nonzero(covered)[0])
+    Xt = torch.tensor(X[covered], dtype=dtype, requires_grad=True)
+    Ws = [(torch.tensor(w, dtype=dtype, requires_grad=True), torch.tensor(b, dtype=dtype, requires_grad=True))
+          for w, b in s.mlp]
+    h = Xt
+    for i, (w, b) in enumerate(Ws):
+        h = h @ w.T + b
+        h = torch.relu(h) if i < len(Ws) - 1 else torch.sigmoid(h)
+    res = Xt[:, 3:4]
+    blend = (1 - res) * Xt[:, :3] + res * h
+    tgt = torch.tensor(s.target, dtype=dtype)
+    color = torch.ones(P, 3, dtype=dtype).index_put((idx,), blend)
+    loss = ((color - tgt) ** 2).mean()
+    loss.backward()
+    dX = np.zeros(X.shape, np.float32)
+    dX[covered] = Xt.grad.numpy()
+    grad_feat = multires_color_bwd(rast, lo, hi, s.feat.shape, dX, True)
+    color32 = color.detach().numpy().astype(np.float32)
+    grad_color32 = (2.0 * (color.detach() - tgt) / color.numel()).numpy().astype(np.float32)
+    ci = color32 if color_in is None else color_in
+    gi = grad_color32 if grad_color_in is None else grad_color_in
+    gpos = edge_grad(ci.reshape(B, H, W, 3), gi.reshape(B, H, W, 3), rast.reshape(B, H, W, 4), pos, s.tri)
+    gverts = np.einsum("bij,bvi->vj", s.mvp.astype(np.float64), gpos.astype(np.float64))[:, :3]
+    ga = opacity_aux_grad(ci.reshape(B, H, W, 3), s.target.reshape(B, H, W, 3), rast.reshape(B, H, W, 4), pos,
+                          s.tri, s.alpha_level, alpha)
+    galog = ga / (3 * P) * alpha * (1 - alpha)
+    gmlp = np.concatenate([Ws[0][0].grad.numpy().ravel(), Ws[1][0].grad.numpy().ravel(),
+                           Ws[2][0].grad.numpy().ravel(), Ws[0][1].grad.numpy(), Ws[1][1].grad.numpy(),
+                           Ws[2][1].grad.numpy()])
+    return dict(rast=rast, color=color.detach().numpy(), grad_color=grad_color32, loss=float(loss),
+                grad_feat=grad_feat, grad_verts=gverts, grad_alpha_logits=galog, grad_mlp=gmlp,
+                covered=covered, pos=pos, alpha=alpha)

@@ -1,0 +1,133 @@
+"""GPU parity of the fused training step (ds_train_step) against the CPU
+oracle for the raster / gather / edge / opacity parts and a float64 torch
+evaluation of the pixel MLP, blend and MSE loss, plus float64 central finite
+differences on the continuous parameters.
+
+Tolerances: fp32 MLP - colour within 1e-5 of float64, parameter gradients
+within 1e-4 of their largest magnitude; bf16 MLP (fp32 accumulation) -
+colour within 1e-3 of float64 (the image-error bar), gradients within 2e-2.
+The raster output must be bit-identical in both precisions."""
+import numpy as np
+import pytest
+import torch
+
+from diffsoup_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(got, ref, rel, what):
+    scale = max(np.abs(ref).max(), 1e-12)
+    err = np.abs(got - ref).max()
+    assert err <= rel * scale, f"{what}: max err {err:.3e} > {rel:.1e} x {scale:.3e}"
+
+
+def _run(dev, s, bf16, seed=5):
+    from diffsoup_amd import fused
+    prm = fused.SoupParams.from_soup(s, dev)
+    nl = s.levels[1] - s.levels[0] + 1
+    st = fused.FusedStep(s.resolutions, s.T, s.verts.shape[0], s.alpha_level, s.levels, s.feat.shape[2],
+                         s.mlp_dims[1], bf16=bf16, device=dev)
+    loss = st.run(prm, torch.from_numpy(s.mvp).to(dev), torch.from_numpy(s.target).to(dev), seed)
+    torch.cuda.synchronize()
+    out = {k: getattr(st, k).cpu().numpy() for k in ("pos", "alpha", "rast", "color", "grad_color",
+                                                     "pix_loss", "grad_verts", "grad_alpha_logits",
+                                                     "grad_feat", "grad_mlp")}
+    out["loss"] = float(loss.item())
+    out["stats"] = st.stats().cpu().numpy()
+    del nl
+    return out
+
+
+@pytest.fixture(scope="module")
+def small():
+    return scenes.small_random_soup(0)
+
+
+@pytest.fixture(scope="module")
+def medium():
+    return scenes.sphere_torus_soup(0)
+
+
+@pytest.mark.parametrize("which,bf16", [("small", False), ("medium", False), ("medium", True)])
+def test_fused_step_matches_reference(dev, small, medium, which, bf16):
+    s = small if which == "small" else medium
+    print(f"fused step {s.name} bf16={bf16}", flush=True)
+    g = _run(dev, s, bf16)
+    # projection against float64
+    pos64 = np.einsum("bij,nj->bni", s.mvp.astype(np.float64),
+                      np.concatenate([s.verts, np.ones((s.verts.shape[0], 1), np.float32)], 1).astype(np.float64))
+    np.testing.assert_allclose(g["pos"], pos64, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(g["alpha"], 1 / (1 + np.exp(-s.alpha_logits.astype(np.float64))), atol=1e-6)
+    # raster: bit-exact with the oracle on the step's own clip positions
+    H, W = s.resolutions[0]
+    rast = O.rasterize((H, W), g["pos"], s.tri, s.alpha_level, g["alpha"], True, 5).reshape(-1, 4)
+    np.testing.assert_array_equal(g["rast"].view(np.uint32), rast.view(np.uint32))
+    assert g["stats"][0] == O.covered_fragments(H, W, g["pos"], s.tri)
+    ref = O.reference_step(s, 5, pos=g["pos"], alpha=g["alpha"], color_in=g["color"],
+                           grad_color_in=g["grad_color"])
+    tol_c = 1e-3 if bf16 else 1e-5
+    err = np.abs(g["color"] - ref["color"]).max()
+    print(f"  colour max abs err vs fp64: {err:.3e}; loss {g['loss']:.6f} vs {ref['loss']:.6f}", flush=True)
+    assert err <= tol_c
+    assert abs(g["loss"] - ref["loss"]) <= (1e-3 if bf16 else 1e-5) * ref["loss"]
+    np.testing.assert_allclose(g["pix_loss"], np.abs(g["color"] - s.target).sum(1), rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(g["grad_color"], 2 * (g["color"] - s.target) / g["color"].size, rtol=1e-5,
+                               atol=1e-12)
+    rel = 2e-2 if bf16 else 1e-4
+    _close(g["grad_mlp"][:-1], ref["grad_mlp"], rel, "MLP grads")
+    assert abs(g["grad_mlp"][-1] - ((g["color"][ref["covered"]] - s.target[ref["covered"]]) ** 2).sum()) < 1e-3 * max(1.0, g["grad_mlp"][-1])
+    _close(g["grad_feat"], ref["grad_feat"], rel, "feature grads")
+    _close(g["grad_verts"], ref["grad_verts"], 1e-4, "vertex grads")
+    _close(g["grad_alpha_logits"], ref["grad_alpha_logits"], 1e-4, "opacity logit grads")
+
+
+def test_fused_fd_continuous_params(dev, small):
+    """float64 central differences of the loss w.r.t. features and MLP weights."""
+    s = small
+    g = _run(dev, s, False, seed=9)
+    H, W = s.resolutions[0]
+    rast = g["rast"]
+    lo, hi = s.levels
+    P = rast.shape[0]
+    covered = rast[:, 3] > 0
+    tgt = s.target.astype(np.float64)
+
+    def loss_of(feat, mlp):
+        X = O.multires_color(rast, lo, hi, feat, True).reshape(P, -1).astype(np.float64)
+        h = X[covered]
+        for i, (w, b) in enumerate(mlp):
+            h = h @ w.astype(np.float64).T + b
+            h = np.maximum(h, 0) if i < 2 else 1 / (1 + np.exp(-h))
+        res = X[covered, 3:4]
+        col = np.ones((P, 3))
+        col[covered] = (1 - res) * X[covered, :3] + res * h
+        return ((col - tgt) ** 2).mean()
+
+    rng = np.random.default_rng(0)
+    ts = np.unique(rast[covered, 3].astype(int) - 1)
+    eps = 1e-3
+    checked = 0
+    for _ in range(8):
+        t = int(rng.choice(ts))
+        k = int(rng.integers(0, s.feat.shape[1]))
+        c = int(rng.integers(0, s.feat.shape[2]))
+        fp, fm = s.feat.copy(), s.feat.copy()
+        fp[t, k, c] += eps
+        fm[t, k, c] -= eps
+        fd = (loss_of(fp, s.mlp) - loss_of(fm, s.mlp)) / (2 * eps)
+        got = g["grad_feat"][t, k, c]
+        assert abs(got - fd) <= 2e-3 * abs(fd) + 1e-9, (t, k, c, got, fd)
+        checked += 1
+    dW1 = g["grad_mlp"][:16 * 8].reshape(16, 8)
+    for _ in range(6):
+        i, j = int(rng.integers(0, 16)), int(rng.integers(0, 8))
+        mp = [(w.copy(), b.copy()) for w, b in s.mlp]
+        mm = [(w.copy(), b.copy()) for w, b in s.mlp]
+        mp[0][0][i, j] += eps
+        mm[0][0][i, j] -= eps
+        fd = (loss_of(s.feat, mp) - loss_of(s.feat, mm)) / (2 * eps)
+        assert abs(dW1[i, j] - fd) <= 2e-3 * abs(fd) + 1e-8, (i, j, dW1[i, j], fd)
+        checked += 1
+    assert checked == 14
