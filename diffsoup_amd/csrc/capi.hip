@@ -83,6 +83,7 @@ int check_levels(int lo, int hi) {
 namespace ds {
 int step_make_views(const ds_step_desc* d, Views& v) { return make_views(d->B, d->H, d->W, d->tile, v); }
 int step_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+int plain_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 }  // namespace ds
 
 extern "C" {

@@ -1,0 +1,404 @@
+// The plain FP32 path: the reference's rasterization structure implemented
+// directly in HIP, kept as the comparator for the tile-binned path and as a
+// one-to-one stand-in for each compiled function the reference's Python layer
+// binds (src/main.cpp:30-108, 134-210):
+//
+//   rects      per (view, triangle) conservative rectangle and fragment count,
+//              inclusive scan, total returned to the host
+//   fragments  one thread per candidate: binary search for its triangle,
+//              coverage, barycentrics and depth range (invalid -> view = -1)
+//   compact    order-preserving compaction (a scan of the valid flags), so the
+//              fragment list is in (view, triangle, rectangle position) order
+//   alpha      per-fragment lattice opacity; thresholds from the mask hash
+//   depth      global 64-bit atomicMin of bits(depth + 2) << 32 | fragment
+//              index into a per-pixel key buffer, then a gather into rast
+//   aux        per-fragment score-function opacity gradient and its scatter
+//
+// The host-visible counts (candidates, valid fragments) are returned with a
+// blocking copy, as the reference does; everything else stays on the stream.
+#include "../../include/diffsoup_hip.h"
+#include "ds_kernels.h"
+#include "ds_math.h"
+
+namespace ds {
+namespace {
+
+constexpr int kBS = 256;
+inline unsigned nblk(long long n) { return static_cast<unsigned>((n + kBS - 1) / kBS); }
+
+__global__ void k_rects(int H, int W, int B, int V, const float4* __restrict__ pos, int T,
+                        const int* __restrict__ tri, int4* __restrict__ rects, int* __restrict__ counts) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= (long long)B * T) return;
+    const int b = static_cast<int>(i / T), t = static_cast<int>(i - (long long)b * T);
+    const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+    int4 out = make_int4(0, 0, 0, 0);
+    int cnt = 0;
+    if (i0 >= 0 && i0 < V && i1 >= 0 && i1 < V && i2 >= 0 && i2 < V) {
+        const float4* P = pos + (long long)b * V;
+        const Rect r = triangle_rect(P[i0], P[i1], P[i2], H, W);
+        if (r.y1 > r.y0) {
+            out = make_int4(r.y0, r.y1 - r.y0, r.x0, r.x1 - r.x0);   // h0, h_len, w0, w_len
+            cnt = (r.y1 - r.y0) * (r.x1 - r.x0);
+        }
+    }
+    rects[i] = out;
+    counts[i] = cnt;
+}
+
+__global__ void k_fragments(int H, int W, int V, const float4* __restrict__ pos, int T,
+                            const int* __restrict__ tri, int num_tris, long long num_frags,
+                            const long long* __restrict__ prefix, const int4* __restrict__ rects,
+                            int* __restrict__ frag_pix, float4* __restrict__ frag_attrs) {
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= num_frags) return;
+    frag_pix[3 * idx] = -1;
+    int lo = 0, hi = num_tris;   // first i with prefix[i] > idx (inclusive prefix)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= idx) lo = mid + 1; else hi = mid;
+    }
+    const int i = lo;
+    const int b = i / T, t = i - b * T;
+    const int4 r = rects[i];
+    if (r.y <= 0 || r.w <= 0) return;
+    const long long local = idx - (i == 0 ? 0 : prefix[i - 1]);
+    const int y = r.x + static_cast<int>(local / r.w);
+    const int x = r.z + static_cast<int>(local % r.w);
+    if (x < 0 || x >= W || y < 0 || y >= H) return;
+    const float4* P = pos + (long long)b * V;
+    Hit h;
+    if (!cover(pixel_ndc(y, H), pixel_ndc(x, W), P[tri[3 * t]], P[tri[3 * t + 1]], P[tri[3 * t + 2]], h)) return;
+    float depth;
+    if (!fragment_depth(h, depth)) return;
+    frag_pix[3 * idx] = b;
+    frag_pix[3 * idx + 1] = y;
+    frag_pix[3 * idx + 2] = x;
+    frag_attrs[idx] = make_float4(h.b0, h.b1, depth, static_cast<float>(t + 1));
+}
+
+__global__ void k_valid_flags(long long n, const int* __restrict__ frag_pix, int* __restrict__ flags) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = frag_pix[3 * i] >= 0 ? 1 : 0;
+}
+
+__global__ void k_compact(long long n, const int* __restrict__ frag_pix, const float4* __restrict__ frag_attrs,
+                          const int* __restrict__ pos_excl, int* __restrict__ pix_out,
+                          float4* __restrict__ attrs_out) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n || frag_pix[3 * i] < 0) return;
+    const long long o = pos_excl[i];
+    pix_out[3 * o] = frag_pix[3 * i];
+    pix_out[3 * o + 1] = frag_pix[3 * i + 1];
+    pix_out[3 * o + 2] = frag_pix[3 * i + 2];
+    attrs_out[o] = frag_attrs[i];
+}
+
+__global__ void k_alpha(long long n, const float4* __restrict__ attrs, int level, int S,
+                        const float* __restrict__ alpha_src, float* __restrict__ out) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = attrs[i];
+    const int t = static_cast<int>(a.w) - 1;
+    if (t < 0) { out[i] = 0.f; return; }
+    const Corners c = lattice_corners(a.x, a.y, static_cast<uint32_t>(level));
+    const float* src = alpha_src + (long long)t * S;
+    float v = 0.f;
+    v += c.w[0] * src[c.idx[0]];
+    v += c.w[1] * src[c.idx[1]];
+    v += c.w[2] * src[c.idx[2]];
+    out[i] = v;
+}
+
+__global__ void k_alpha_bwd(long long n, const float4* __restrict__ attrs, int level, int S,
+                            float* __restrict__ grad_src, const float* __restrict__ grad) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = attrs[i];
+    const int t = static_cast<int>(a.w) - 1;
+    if (t < 0) return;
+    const Corners c = lattice_corners(a.x, a.y, static_cast<uint32_t>(level));
+    float* d = grad_src + (long long)t * S;
+    const float g = grad[i];
+    for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&d[c.idx[k]], g * c.w[k]);
+}
+
+__global__ void k_thresholds(long long n, const int* __restrict__ frag_pix, const float4* __restrict__ attrs,
+                             int stochastic, unsigned long long seed, int view_base, float* __restrict__ out) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = stochastic ? mask_threshold(seed, static_cast<uint32_t>(frag_pix[3 * i] + view_base),
+                                         static_cast<uint32_t>(static_cast<int>(attrs[i].w) - 1),
+                                         static_cast<uint32_t>(frag_pix[3 * i + 1]),
+                                         static_cast<uint32_t>(frag_pix[3 * i + 2]))
+                        : 0.5f;
+}
+
+__global__ void k_fill_keys(long long n, long long* __restrict__ z) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) z[i] = kKeyEmpty;
+}
+
+__global__ void k_depth_atomic(int H, int W, long long n, const int* __restrict__ frag_pix,
+                               const float4* __restrict__ attrs, const float* __restrict__ alpha,
+                               const float* __restrict__ thresh, long long* __restrict__ z) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = frag_pix[3 * i];
+    if (b < 0) return;
+    if (alpha[i] < thresh[i]) return;
+    const float d = attrs[i].z;
+    if (!depth_testable(d)) return;
+    const long long px = ((long long)b * H + frag_pix[3 * i + 1]) * W + frag_pix[3 * i + 2];
+    atomicMin(&z[px], static_cast<long long>(depth_key(d, static_cast<uint32_t>(i))));
+}
+
+__global__ void k_depth_gather(long long P, const long long* __restrict__ z, const float4* __restrict__ attrs,
+                               float4* __restrict__ rast) {
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const long long k = z[p];
+    rast[p] = k == kKeyEmpty ? make_float4(0.f, 0.f, 0.f, 0.f) : attrs[static_cast<uint32_t>(k & 0xffffffffll)];
+}
+
+__global__ void k_aux(int H, int W, int C, const float* __restrict__ color, const float* __restrict__ target,
+                      const float4* __restrict__ rast, long long n, const int* __restrict__ frag_pix,
+                      const float4* __restrict__ attrs, const float* __restrict__ alpha, float* __restrict__ grad) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = frag_pix[3 * i];
+    if (b < 0) return;
+    const long long p = ((long long)b * H + frag_pix[3 * i + 1]) * W + frag_pix[3 * i + 2];
+    const float4 a = attrs[i];
+    const float4 r = rast[p];
+    const int tp = static_cast<int>(r.w) - 1;
+    if (tp >= 0 && a.z > r.z) return;
+    float l = 0.f;
+    for (int c = 0; c < C; ++c) l += fabsf(color[p * C + c] - target[p * C + c]);
+    constexpr float eps = 1e-5f;
+    const float al = alpha[i];
+    if (tp >= 0 && static_cast<int>(a.w) - 1 == tp) {
+        grad[i] += l / (al > eps ? al : eps);
+    } else {
+        const float om = 1.f - al;
+        grad[i] -= l / (om > eps ? om : eps);
+    }
+}
+
+// ---- multi-block exclusive scan of int32 flags/counts (int64 output) ----
+constexpr int kScanItems = 4 * 1024;
+
+__global__ void __launch_bounds__(1024) k_block_sums(const int* __restrict__ in, long long n,
+                                                     long long* __restrict__ sums) {
+    __shared__ long long red[16];
+    const long long base = blockIdx.x * (long long)kScanItems;
+    long long s = 0;
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + k * 1024 + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int k = 0; k < 16; ++k) t += red[k];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// in-place exclusive scan of the block sums by one workgroup; sums[nb] = total
+__global__ void __launch_bounds__(1024) k_scan_sums(long long* __restrict__ sums, int nb) {
+    __shared__ long long ws[16];
+    __shared__ long long carry;
+    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + tid;
+        const long long v = i < nb ? sums[i] : 0;
+        long long inc = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) ws[wv] = inc;
+        __syncthreads();
+        long long pre = 0, tot = 0;
+        for (int k = 0; k < 16; ++k) { if (k < wv) pre += ws[k]; tot += ws[k]; }
+        const long long c = carry;
+        if (i < nb) sums[i] = c + pre + inc - v;
+        __syncthreads();
+        if (tid == 0) carry = c + tot;
+        __syncthreads();
+    }
+    if (tid == 0) sums[nb] = carry;
+}
+
+template <typename OUT>
+__global__ void __launch_bounds__(1024) k_block_scan(const int* __restrict__ in, long long n,
+                                                     const long long* __restrict__ sums, OUT* __restrict__ out,
+                                                     int inclusive) {
+    __shared__ long long ws[16];
+    __shared__ long long carry;
+    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    if (tid == 0) carry = sums[blockIdx.x];
+    __syncthreads();
+    const long long base = blockIdx.x * (long long)kScanItems;
+    for (int k = 0; k < 4; ++k) {
+        const long long i = base + k * 1024 + tid;
+        const long long v = i < n ? in[i] : 0;
+        long long inc = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) ws[wv] = inc;
+        __syncthreads();
+        long long pre = 0, tot = 0;
+        for (int q = 0; q < 16; ++q) { if (q < wv) pre += ws[q]; tot += ws[q]; }
+        const long long c = carry;
+        if (i < n) out[i] = static_cast<OUT>(c + pre + (inclusive ? inc : inc - v));
+        __syncthreads();
+        if (tid == 0) carry = c + tot;
+        __syncthreads();
+    }
+}
+
+// Scan n ints into out (inclusive or exclusive); tmp holds nb + 1 int64.
+// Returns the total through a blocking copy.
+template <typename OUT>
+hipError_t scan_ints(const int* in, long long n, OUT* out, long long* tmp, int inclusive, long long* total,
+                     hipStream_t s) {
+    const int nb = static_cast<int>((n + kScanItems - 1) / kScanItems);
+    if (nb == 0) {
+        if (total) *total = 0;
+        return hipSuccess;
+    }
+    k_block_sums<<<nb, 1024, 0, s>>>(in, n, tmp);
+    k_scan_sums<<<1, 1024, 0, s>>>(tmp, nb);
+    k_block_scan<OUT><<<nb, 1024, 0, s>>>(in, n, tmp, out, inclusive);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (total) {
+        e = hipMemcpyAsync(total, tmp + nb, sizeof(long long), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+    }
+    return e;
+}
+
+}  // namespace
+
+int plain_fail(int code, const char* msg);
+
+}  // namespace ds
+
+extern "C" {
+
+size_t ds_plain_scan_tmp_bytes(int64_t n) {
+    return sizeof(long long) * (size_t)((n + ds::kScanItems - 1) / ds::kScanItems + 2);
+}
+
+int64_t ds_plain_triangle_rects(int H, int W, int B, int V, const float* pos, int T, const int* tri,
+                                int* rects, int64_t* prefix, int* counts, void* scan_tmp, void* stream) {
+    using namespace ds;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (H < 1 || W < 1 || B < 1 || V < 0 || T < 0) return plain_fail(DS_ERR_INVALID, "bad sizes");
+    const long long n = (long long)B * T;
+    if (n == 0) return 0;
+    if (!pos || !tri || !rects || !prefix || !counts || !scan_tmp) return plain_fail(DS_ERR_INVALID, "null pointer");
+    k_rects<<<nblk(n), kBS, 0, s>>>(H, W, B, V, reinterpret_cast<const float4*>(pos), T, tri,
+                                    reinterpret_cast<int4*>(rects), counts);
+    long long total = 0;
+    if (scan_ints<long long>(counts, n, reinterpret_cast<long long*>(prefix), static_cast<long long*>(scan_tmp), 1,
+                             &total, s) != hipSuccess)
+        return plain_fail(DS_ERR_HIP, "rect scan");
+    return total;
+}
+
+int ds_plain_fragments(int H, int W, int B, int V, const float* pos, int T, const int* tri, const int64_t* prefix,
+                       const int* rects, int64_t num_frags, int* frag_pix, float* frag_attrs, void* stream) {
+    using namespace ds;
+    if (num_frags <= 0) return DS_OK;
+    if (num_frags > INT32_MAX) return plain_fail(DS_ERR_INVALID, "more than 2^31 candidate fragments");
+    k_fragments<<<nblk(num_frags), kBS, 0, static_cast<hipStream_t>(stream)>>>(
+        H, W, V, reinterpret_cast<const float4*>(pos), T, tri, B * T, num_frags,
+        reinterpret_cast<const long long*>(prefix), reinterpret_cast<const int4*>(rects), frag_pix,
+        reinterpret_cast<float4*>(frag_attrs));
+    return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "fragments");
+}
+
+int64_t ds_plain_compact(int64_t num_frags, const int* frag_pix, const float* frag_attrs, int* frag_pix_out,
+                         float* frag_attrs_out, int* flags, void* scan_tmp, void* stream) {
+    using namespace ds;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_frags <= 0) return 0;
+    k_valid_flags<<<nblk(num_frags), kBS, 0, s>>>(num_frags, frag_pix, flags);
+    long long total = 0;
+    // exclusive positions overwrite the flags in place after they are summed per block
+    if (scan_ints<int>(flags, num_frags, flags, static_cast<long long*>(scan_tmp), 0, &total, s) != hipSuccess)
+        return plain_fail(DS_ERR_HIP, "compaction scan");
+    k_compact<<<nblk(num_frags), kBS, 0, s>>>(num_frags, frag_pix, reinterpret_cast<const float4*>(frag_attrs),
+                                              flags, frag_pix_out, reinterpret_cast<float4*>(frag_attrs_out));
+    return hipGetLastError() == hipSuccess ? total : plain_fail(DS_ERR_HIP, "compact");
+}
+
+int ds_plain_alpha(int64_t n, const float* frag_attrs, int level, const float* alpha_src, float* frag_alpha,
+                   void* stream) {
+    using namespace ds;
+    if (n <= 0) return DS_OK;
+    k_alpha<<<nblk(n), kBS, 0, static_cast<hipStream_t>(stream)>>>(
+        n, reinterpret_cast<const float4*>(frag_attrs), level, static_cast<int>(feats_at_level(level)), alpha_src,
+        frag_alpha);
+    return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "alpha");
+}
+
+int ds_plain_alpha_bwd(int64_t n, const float* frag_attrs, int level, float* grad_alpha_src,
+                       const float* grad_frag_alpha, void* stream) {
+    using namespace ds;
+    if (n <= 0) return DS_OK;
+    k_alpha_bwd<<<nblk(n), kBS, 0, static_cast<hipStream_t>(stream)>>>(
+        n, reinterpret_cast<const float4*>(frag_attrs), level, static_cast<int>(feats_at_level(level)),
+        grad_alpha_src, grad_frag_alpha);
+    return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "alpha bwd");
+}
+
+int ds_plain_thresholds(int64_t n, const int* frag_pix, const float* frag_attrs, int stochastic, uint64_t seed,
+                        int view_base, float* thresh, void* stream) {
+    using namespace ds;
+    if (n <= 0) return DS_OK;
+    k_thresholds<<<nblk(n), kBS, 0, static_cast<hipStream_t>(stream)>>>(
+        n, frag_pix, reinterpret_cast<const float4*>(frag_attrs), stochastic, seed, view_base, thresh);
+    return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "thresholds");
+}
+
+int ds_plain_depth_test(int B, int H, int W, int64_t n, const int* frag_pix, const float* frag_attrs,
+                        const float* frag_alpha, const float* thresh, int64_t* zbuf, float* rast, void* stream) {
+    using namespace ds;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long P = (long long)B * H * W;
+    if (P <= 0) return DS_OK;
+    long long* z = reinterpret_cast<long long*>(zbuf);
+    k_fill_keys<<<nblk(P), kBS, 0, s>>>(P, z);
+    if (n > 0)
+        k_depth_atomic<<<nblk(n), kBS, 0, s>>>(H, W, n, frag_pix, reinterpret_cast<const float4*>(frag_attrs),
+                                               frag_alpha, thresh, z);
+    k_depth_gather<<<nblk(P), kBS, 0, s>>>(P, z, reinterpret_cast<const float4*>(frag_attrs),
+                                           reinterpret_cast<float4*>(rast));
+    return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "depth test");
+}
+
+int ds_plain_opacity_aux(int B, int H, int W, int C, const float* color, const float* target, const float* rast,
+                         int64_t n, const int* frag_pix, const float* frag_attrs, const float* frag_alpha,
+                         float* grad_frag_alpha, void* stream) {
+    using namespace ds;
+    (void)B;
+    if (n <= 0) return DS_OK;
+    k_aux<<<nblk(n), kBS, 0, static_cast<hipStream_t>(stream)>>>(
+        H, W, C, color, target, reinterpret_cast<const float4*>(rast), n, frag_pix,
+        reinterpret_cast<const float4*>(frag_attrs), frag_alpha, grad_frag_alpha);
+    return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "opacity aux");
+}
+
+}  // extern "C"

@@ -219,6 +219,63 @@ size_t ds_step_workspace_size(const ds_step_desc* d);
 /* Enqueue one fused forward + backward step (see above). */
 int ds_train_step(const ds_step_desc* d, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Plain path: the reference's fragment-list rasterizer, one function per
+ * compiled function its Python layer binds, for callers that drive the
+ * reference's composition (python/diffsoup/rasterize.py:14-126) and as the
+ * comparator of the tile-binned path.  Fragment buffers are caller-owned;
+ * frag_pix [N,3] int32 (view, y, x; view -1 = invalid), frag_attrs [N,4]
+ * (b0, b1, z/w, triangle id + 1).  Counts come back through a blocking copy,
+ * as in the reference.  scan_tmp holds ds_plain_scan_tmp_bytes(n) bytes.
+ * ------------------------------------------------------------------------ */
+
+/* Bytes of scan scratch for n items. */
+size_t ds_plain_scan_tmp_bytes(int64_t n);
+
+/* rects [B*T,4] (h0, h_len, w0, w_len), counts [B*T], inclusive int64 prefix
+ * [B*T]; returns the number of candidate fragments or a negative DS_ERR_*.
+ * Replaces _core.compute_triangle_rects (src/main.cpp:30-48). */
+int64_t ds_plain_triangle_rects(int H, int W, int B, int V, const float* pos, int T, const int* tri,
+                                int* rects, int64_t* prefix, int* counts, void* scan_tmp, void* stream);
+
+/* Candidate fragments.  Replaces _core.compute_fragments (src/main.cpp:50-76). */
+int ds_plain_fragments(int H, int W, int B, int V, const float* pos, int T, const int* tri,
+                       const int64_t* prefix, const int* rects, int64_t num_frags, int* frag_pix,
+                       float* frag_attrs, void* stream);
+
+/* Order-preserving compaction of valid fragments; flags [num_frags] int32
+ * scratch; returns the valid count.  Replaces _core.filter_valid_fragments
+ * (src/main.cpp:96-108) with a deterministic output order. */
+int64_t ds_plain_compact(int64_t num_frags, const int* frag_pix, const float* frag_attrs, int* frag_pix_out,
+                         float* frag_attrs_out, int* flags, void* scan_tmp, void* stream);
+
+/* Per-fragment opacity at one level.  Replaces _core.multires_triangle_alpha
+ * (src/main.cpp:174-191). */
+int ds_plain_alpha(int64_t n, const float* frag_attrs, int level, const float* alpha_src, float* frag_alpha,
+                   void* stream);
+
+/* Adjoint of ds_plain_alpha (atomic accumulation).  Replaces
+ * _core.backward_multires_triangle_alpha (src/main.cpp:193-210). */
+int ds_plain_alpha_bwd(int64_t n, const float* frag_attrs, int level, float* grad_alpha_src,
+                       const float* grad_frag_alpha, void* stream);
+
+/* Per-fragment mask thresholds: the hash of (seed, view_base + view,
+ * triangle, row, column), or 0.5 when stochastic == 0.  Replaces the
+ * torch.rand / torch.full thresholds of python/diffsoup/multires.py:97-100. */
+int ds_plain_thresholds(int64_t n, const int* frag_pix, const float* frag_attrs, int stochastic, uint64_t seed,
+                        int view_base, float* thresh, void* stream);
+
+/* Depth test with a global 64-bit atomicMin into zbuf [B*H*W] int64 and a
+ * gather into rast [B,H,W,4].  Replaces _core.depth_test (src/main.cpp:78-94). */
+int ds_plain_depth_test(int B, int H, int W, int64_t n, const int* frag_pix, const float* frag_attrs,
+                        const float* frag_alpha, const float* thresh, int64_t* zbuf, float* rast, void* stream);
+
+/* Per-fragment opacity-aux gradient accumulated into grad_frag_alpha.
+ * Replaces _core.backward_opacity_aux_loss (src/main.cpp:134-154). */
+int ds_plain_opacity_aux(int B, int H, int W, int C, const float* color, const float* target, const float* rast,
+                         int64_t n, const int* frag_pix, const float* frag_attrs, const float* frag_alpha,
+                         float* grad_frag_alpha, void* stream);
+
 /* Fill n floats with value (a kernel, so it is safe inside graph capture). */
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 
