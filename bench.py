@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--stage", default="auto", help="kernel stage timed for the roofline")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--profile-driver", action="store_true",
+                    help="run four fused steps and a 256 MiB calibration kernel (for rocprofv3 --pmc), print nothing")
     return ap.parse_args()
 
 
@@ -53,6 +57,9 @@ def main():
 
     dev = torch.device("cuda:0")
     soup = scenes.sphere_torus_soup(0)
+    if args.profile_driver:
+        profile_driver(soup, dev)
+        return
     prm = fused.SoupParams.from_soup(soup, dev)
     mvp = torch.from_numpy(soup.mvp).to(dev)
     target = torch.from_numpy(soup.target).to(dev)
@@ -120,6 +127,12 @@ def main():
         out["bf16_vs_fp32"] = round(ms32 / ms16, 3)
         del st16
 
+    if not args.no_plain:
+        out["plain_fp32_it_s"] = round(plain_its(soup, prm, mvp, target, args.plain_steps), 3)
+        out["speedup_vs_plain_fp32"] = round(out["value"] / out["plain_fp32_it_s"], 2)
+        if "bf16_mlp_it_s" in out:
+            out["bf16_speedup_vs_plain_fp32"] = round(out["bf16_mlp_it_s"] / out["plain_fp32_it_s"], 2)
+
     # roofline of the dominant kernel, timed with events on the launch stream
     stage = "resolve" if args.stage == "auto" else args.stage
     _, _, kms = timed(st32, max(10, args.steps // 2), 2, stage=stage)
@@ -132,6 +145,62 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(soup, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def profile_driver(soup, dev):
+    """Counter-collection driver: four fp32 fused steps, then torch.neg on a
+    256 MiB tensor, a kernel that reads and writes exactly 256 MiB and so
+    calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    from diffsoup_amd import fused
+
+    prm = fused.SoupParams.from_soup(soup, dev)
+    mvp = torch.from_numpy(soup.mvp).to(dev)
+    target = torch.from_numpy(soup.target).to(dev)
+    st = fused.FusedStep(soup.resolutions, soup.T, soup.verts.shape[0], soup.alpha_level, soup.levels,
+                         soup.feat.shape[2], soup.mlp_dims[1], device=dev)
+    for i in range(4):
+        st.run(prm, mvp, target, seed=100 + i)
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+
+
+def plain_its(soup, prm, mvp, target, steps):
+    """The plain per-view FP32 path (fragment lists, global-atomic depth test,
+    torch MLP, one view per launch) on the same workload and inputs."""
+    import torch
+
+    from diffsoup_amd import fused, plain
+
+    leaves = fused.SoupParams(prm.verts.clone().requires_grad_(True), prm.tri,
+                              prm.alpha_logits.clone().requires_grad_(True), prm.feat.clone().requires_grad_(True),
+                              *[t.clone() for t in (prm.w1, prm.b1, prm.w2, prm.b2, prm.w3, prm.b3)])
+    layers = []
+    for w, b in ((leaves.w1, leaves.b1), (leaves.w2, leaves.b2), (leaves.w3, leaves.b3)):
+        lin = torch.nn.Linear(w.shape[1], w.shape[0]).to(w.device)
+        with torch.no_grad():
+            lin.weight.copy_(w)
+            lin.bias.copy_(b)
+        layers.append(lin)
+    ps = plain.PlainStep(leaves, layers, soup.levels, soup.alpha_level)
+
+    def one(i):
+        for t in [leaves.verts, leaves.alpha_logits, leaves.feat] + [p for l in layers for p in l.parameters()]:
+            t.grad = None
+        ps.run(mvp, target, soup.resolutions, seed=3000 + i)
+
+    for i in range(2):
+        one(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        one(10 + i)
+    torch.cuda.synchronize()
+    return steps / (time.perf_counter() - t0)
 
 
 def cpu_baseline(soup, budget_s):

@@ -14,7 +14,7 @@
 
 #include <string>
 
-#include "../../include/diffsoup_hip.h"
+#include "diffsoup_hip.h"
 #include "ds_kernels.h"
 #include "ds_math.h"
 

@@ -46,6 +46,7 @@ struct RasterWorkspace {
     TriRecord* recs;     // [B * T]
     int* tile_count;     // [n_tiles]
     int* tile_off;       // [n_tiles + 1]
+    int* tile_cov;       // [n_tiles] covered fragments per tile (summed into stats)
     int* bins;           // [capacity]
     long long* stats;    // [kStatCount]
     int64_t capacity;
@@ -76,8 +77,9 @@ struct BackgroundShade {
     float* grad_color;     // [P, 3]
     float* pix_loss;       // [P]
     float* loss_part;      // [n_tiles] per-tile sum of squared background error
-    int* covered;          // [P] covered pixel indices (unordered)
-    int* n_covered;        // device counter, zeroed before the launch
+    int2* tile_list;       // [n_tiles * tile^2] (pixel, rank within its triangle)
+    int* tile_ncov;        // [n_tiles] covered pixels per tile
+    int* tri_count;        // [T] covered pixels per triangle over all views (zeroed before)
     float grad_scale;      // dL/dcolor = grad_scale * (color - target)
     float background;      // constant background colour
 };
@@ -107,6 +109,23 @@ hipError_t launch_sh2(const Views& v, const float* rast, const float* inv_mvp, f
 hipError_t launch_edge_grad(const Views& v, int C, const float* color, const float* grad_color,
                             const float* rast, int V, const float* pos, float* grad_pos,
                             int T, const int* tri, hipStream_t s);
+
+// Covered pixels grouped by triangle: list[tri_off[t] + rank] =
+// (pixel, bits(b0), bits(b1), t).
+hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
+                                      const float* rast, const int* tri_off, int4* list, hipStream_t s);
+
+hipError_t launch_edge_grad_list(const Views& v, int C, const float* color, const float* grad_color,
+                                 const float* rast, int V, const float* pos, float* grad_pos, int T,
+                                 const int* tri, const int* list, const int* n_list, long long max_n,
+                                 hipStream_t s);
+
+// Same gradients over the per-tile covered lists of the training-step
+// resolve (raster order within a tile keeps neighbours in cache).
+hipError_t launch_edge_grad_tiles(const Views& v, int C, const float* color, const float* grad_color,
+                                  const float* rast, int V, const float* pos, float* grad_pos, int T,
+                                  const int* tri, const TriRecord* recs, const int2* tile_list,
+                                  const int* tile_ncov, hipStream_t s);
 
 hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long long n, float value, hipStream_t s);

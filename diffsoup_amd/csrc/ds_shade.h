@@ -10,10 +10,11 @@ namespace ds {
 
 struct ShadeArgs {
     const float4* rast;     // [P] (b0, b1, depth, id + 1)
-    const int* covered;     // [P] covered pixel indices
-    const int* n_covered;   // device count of covered pixels
+    const int* tri_off;     // [T + 1] start of each triangle's covered pixels in list
+    const int4* list;       // covered pixels grouped by triangle: (pixel, bits b0, bits b1, t)
+    int T;
     const float* feat;      // [T, S_tot, C]
-    float* grad_feat;       // [T, S_tot, C] accumulated
+    float* grad_feat;       // [T, S_tot, C] written (every row)
     int S_tot;
     int lo;                 // first feature level
     const float* blobs;     // per-lane weight operands (k_blobs)

@@ -53,11 +53,11 @@ __device__ __forceinline__ bool crossed_edge(const float4 P[3], float x0, float 
     float rx[3], ry[3];
     for (int k = 0; k < 3; ++k) { rx[k] = P[k].x / P[k].w; ry[k] = P[k].y / P[k].w; }
     float ix, iy;
-    if (segments_meet(x0, y0, x1, y1, rx[0], ry[0], rx[1], ry[1], ix, iy)) { i0 = 0; i1 = 1; }
-    else if (segments_meet(x0, y0, x1, y1, rx[1], ry[1], rx[2], ry[2], ix, iy)) { i0 = 1; i1 = 2; }
-    else if (segments_meet(x0, y0, x1, y1, rx[2], ry[2], rx[0], ry[0], ix, iy)) { i0 = 2; i1 = 0; }
+    float4 A, B;
+    if (segments_meet(x0, y0, x1, y1, rx[0], ry[0], rx[1], ry[1], ix, iy)) { i0 = 0; i1 = 1; A = P[0]; B = P[1]; }
+    else if (segments_meet(x0, y0, x1, y1, rx[1], ry[1], rx[2], ry[2], ix, iy)) { i0 = 1; i1 = 2; A = P[1]; B = P[2]; }
+    else if (segments_meet(x0, y0, x1, y1, rx[2], ry[2], rx[0], ry[0], ix, iy)) { i0 = 2; i1 = 0; A = P[2]; B = P[0]; }
     else return false;
-    const float4 A = P[i0], B = P[i1];
     const float dx = B.x - A.x, dy = B.y - A.y, dw = B.w - A.w;
     const float ax = A.x - ix * A.w, ay = A.y - iy * A.w;
     const float bx = dx - ix * dw, by = dy - iy * dw;
@@ -122,41 +122,67 @@ __device__ __forceinline__ bool vertices_ok(const int vi[3], int V) {
     return vi[0] >= 0 && vi[0] < V && vi[1] >= 0 && vi[1] < V && vi[2] >= 0 && vi[2] < V;
 }
 
-__global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ color,
-                            const float* __restrict__ grad_color, const float4* __restrict__ rast,
-                            int V, const float4* __restrict__ pos, float* __restrict__ grad_pos,
-                            const int* __restrict__ tri) {
-    const long long p0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (p0 >= v.pix_base[v.n]) return;
+// Per-pixel edge gradient.  With REC the triangle corners come from the
+// raster's per-(view, triangle) clip records (one 48-byte read instead of an
+// index read followed by three vertex reads); b >= 0 gives the pixel's view.
+// All five raster texels and colour / gradient values are loaded before any
+// of them is used, so the dependent-load chain is two levels deep.
+template <bool REC>
+__device__ __forceinline__ bool edge_eval(const Views& v, int T, int C, const float* __restrict__ color,
+                                          const float* __restrict__ grad_color, const float4* __restrict__ rast,
+                                          int V, const float4* __restrict__ pos, const int* __restrict__ tri,
+                                          const TriRecord* __restrict__ recs, long long p0, int& b, float g[12],
+                                          int v0[3], int& it0) {
     const float4 r0 = rast[p0];
-    const int it0 = static_cast<int>(r0.w) - 1;
-    if (it0 < 0 || it0 >= T) return;
-    int b = 0;
-    while (b + 1 < v.n && v.pix_base[b + 1] <= p0) ++b;
+    it0 = static_cast<int>(r0.w) - 1;
+    if (it0 < 0 || it0 >= T) return false;
+    if (b < 0) {
+        b = 0;
+        while (b + 1 < v.n && v.pix_base[b + 1] <= p0) ++b;
+    }
     const int W = v.W[b], H = v.H[b];
     const int local = static_cast<int>(p0 - v.pix_base[b]);
     const int y0 = local / W, x0 = local - y0 * W;
     const float nx0 = pixel_ndc(x0, W), ny0 = pixel_ndc(y0, H);
     const float depth0 = r0.z;
 
+    const int DX[4] = {1, -1, 0, 0};
+    const int DY[4] = {0, 0, -1, 1};
+    // neighbour raster texels first: independent loads in flight together
+    bool inb[4];
+    long long pn[4];
+    float4 rn[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x1 = x0 + DX[e], y1 = y0 + DY[e];
+        inb[e] = !(y1 < 0 || y1 >= H || x1 < 0 || x1 >= W);
+        pn[e] = inb[e] ? v.pix_base[b] + (long long)y1 * W + x1 : p0;
+        rn[e] = rast[pn[e]];
+    }
+
     const float4* bp = pos + (long long)b * V;
-    const int v0[3] = {tri[3 * it0], tri[3 * it0 + 1], tri[3 * it0 + 2]};
-    if (!vertices_ok(v0, V)) return;
-    const float4 P0[3] = {bp[v0[0]], bp[v0[1]], bp[v0[2]]};
+    float4 P0[3];
+    if constexpr (REC) {
+        const TriRecord& rc = recs[(long long)b * T + it0];
+        P0[0] = rc.p0; P0[1] = rc.p1; P0[2] = rc.p2;
+        v0[0] = tri[3 * it0]; v0[1] = tri[3 * it0 + 1]; v0[2] = tri[3 * it0 + 2];
+    } else {
+        v0[0] = tri[3 * it0]; v0[1] = tri[3 * it0 + 1]; v0[2] = tri[3 * it0 + 2];
+        if (!vertices_ok(v0, V)) return false;
+        P0[0] = bp[v0[0]]; P0[1] = bp[v0[1]]; P0[2] = bp[v0[2]];
+    }
     const float* c0 = color + p0 * C;
     const float* g0 = grad_color + p0 * C;
 
-    float g[12];
     for (int k = 0; k < 12; ++k) g[k] = 0.f;
     bool touched = false;
 
-    const int DX[4] = {1, -1, 0, 0};
-    const int DY[4] = {0, 0, -1, 1};
+#pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int x1 = x0 + DX[e], y1 = y0 + DY[e];
-        if (y1 < 0 || y1 >= H || x1 < 0 || x1 >= W) continue;
-        const long long p1 = v.pix_base[b] + (long long)y1 * W + x1;
-        const float4 r1 = rast[p1];
+        if (!inb[e]) continue;
+        const long long p1 = pn[e];
+        const float4 r1 = rn[e];
         const float depth1 = r1.z;
         int it1 = static_cast<int>(r1.w) - 1;
         if (it1 >= T) it1 = -1;
@@ -186,9 +212,15 @@ __global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ col
             // implicit edge: triangle intersection or opacity edge
             const float depth1_own = h1.z / h1.w;
             if (it1 >= 0 && depth1 < depth1_own) {
-                const int v1[3] = {tri[3 * it1], tri[3 * it1 + 1], tri[3 * it1 + 2]};
-                if (!vertices_ok(v1, V)) continue;
-                const float4 P1[3] = {bp[v1[0]], bp[v1[1]], bp[v1[2]]};
+                float4 P1[3];
+                if constexpr (REC) {
+                    const TriRecord& rc = recs[(long long)b * T + it1];
+                    P1[0] = rc.p0; P1[1] = rc.p1; P1[2] = rc.p2;
+                } else {
+                    const int v1[3] = {tri[3 * it1], tri[3 * it1 + 1], tri[3 * it1 + 2]};
+                    if (!vertices_ok(v1, V)) continue;
+                    P1[0] = bp[v1[0]]; P1[1] = bp[v1[1]]; P1[2] = bp[v1[2]];
+                }
                 Hit h0;
                 if (cover(ny0, nx0, P1[0], P1[1], P1[2], h0)) {
                     const float depth0_other = h0.z / h0.w;
@@ -226,30 +258,168 @@ __global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ col
             int i0, i1;
             float s0, s1;
             if (crossed_edge(P0, nx0, ny0, nx1, ny1, i0, i1, s0, s1)) {
-                const float cx = s0 * P0[i0].x + s1 * P0[i1].x;
-                const float cy = s0 * P0[i0].y + s1 * P0[i1].y;
-                const float cw = s0 * P0[i0].w + s1 * P0[i1].w;
+                // the crossed edge's end points, selected without dynamic
+                // register indexing (which would live in scratch memory)
+                const float4 A = i0 == 0 ? P0[0] : (i0 == 1 ? P0[1] : P0[2]);
+                const float4 Bv = i1 == 0 ? P0[0] : (i1 == 1 ? P0[1] : P0[2]);
+                const float cx = s0 * A.x + s1 * Bv.x;
+                const float cy = s0 * A.y + s1 * Bv.y;
+                const float cw = s0 * A.w + s1 * Bv.w;
                 const float iw = 1.f / cw;
                 const float iw2 = iw * iw;
                 const float gs = gx * cx + gy * cy;
-                g[i0 * 4 + 0] += s0 * gx * iw;
-                g[i1 * 4 + 0] += s1 * gx * iw;
-                g[i0 * 4 + 1] += s0 * gy * iw;
-                g[i1 * 4 + 1] += s1 * gy * iw;
-                g[i0 * 4 + 3] += -s0 * gs * iw2;
-                g[i1 * 4 + 3] += -s1 * gs * iw2;
+                const float ax = s0 * gx * iw, bx = s1 * gx * iw;
+                const float ay = s0 * gy * iw, by = s1 * gy * iw;
+                const float aw = -s0 * gs * iw2, bw = -s1 * gs * iw2;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (k == i0) { g[k * 4 + 0] += ax; g[k * 4 + 1] += ay; g[k * 4 + 3] += aw; }
+                    if (k == i1) { g[k * 4 + 0] += bx; g[k * 4 + 1] += by; g[k * 4 + 3] += bw; }
+                }
                 touched = true;
             }
         }
     }
-    if (!touched) return;
+    return touched;
+}
+
+// Per-lane evaluation followed by per-lane atomics (generic entry points).
+template <bool REC>
+__device__ __forceinline__ void edge_at(const Views& v, int T, int C, const float* __restrict__ color,
+                                        const float* __restrict__ grad_color, const float4* __restrict__ rast,
+                                        int V, const float4* __restrict__ pos, float* __restrict__ grad_pos,
+                                        const int* __restrict__ tri, const TriRecord* __restrict__ recs,
+                                        long long p0, int b) {
+    float g[12];
+    int v0[3] = {0, 0, 0};
+    int it0 = -1;
+    if (!edge_eval<REC>(v, T, C, color, grad_color, rast, V, pos, tri, recs, p0, b, g, v0, it0)) return;
     float* gp = grad_pos + (long long)b * V * 4;
+#pragma unroll
     for (int k = 0; k < 3; ++k)
+#pragma unroll
         for (int j = 0; j < 4; ++j)
             if (g[k * 4 + j] != 0.f) unsafeAtomicAdd(&gp[v0[k] * 4 + j], g[k * 4 + j]);
 }
 
+__global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ color,
+                            const float* __restrict__ grad_color, const float4* __restrict__ rast,
+                            int V, const float4* __restrict__ pos, float* __restrict__ grad_pos,
+                            const int* __restrict__ tri) {
+    const long long p0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p0 >= v.pix_base[v.n]) return;
+    edge_at<false>(v, T, C, color, grad_color, rast, V, pos, grad_pos, tri, nullptr, p0, -1);
+}
+
+// Same gradients over a list of covered pixels (n_list read on the device).
+__global__ void k_edge_grad_list(Views v, int T, int C, const float* __restrict__ color,
+                                 const float* __restrict__ grad_color, const float4* __restrict__ rast,
+                                 int V, const float4* __restrict__ pos, float* __restrict__ grad_pos,
+                                 const int* __restrict__ tri, const int* __restrict__ list,
+                                 const int* __restrict__ n_list) {
+    const int n = *n_list;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        edge_at<false>(v, T, C, color, grad_color, rast, V, pos, grad_pos, tri, nullptr, list[i], -1);
+}
+
+// One workgroup per screen tile, one lane per covered pixel of the tile.
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_edge_grad_tiles(Views v, int T, int C, const float* __restrict__ color, const float* __restrict__ grad_color,
+                  const float4* __restrict__ rast, int V, const float4* __restrict__ pos,
+                  float* __restrict__ grad_pos, const int* __restrict__ tri, const TriRecord* __restrict__ recs,
+                  const int2* __restrict__ tile_list, const int* __restrict__ tile_ncov) {
+    const int n = tile_ncov[blockIdx.x];
+    const int wave_first = (threadIdx.x / 64) * 64;
+    if (wave_first >= n) return;          // whole wave idle (wave-uniform exit)
+    int b = 0;
+    while (b + 1 < v.n && v.tile_base[b + 1] <= static_cast<int>(blockIdx.x)) ++b;
+    const int lane = threadIdx.x % 64;
+    float g[12];
+    int v0[3] = {0, 0, 0};
+    int it0 = -1;
+    bool act = false;
+    if (static_cast<int>(threadIdx.x) < n) {
+        const long long p0 = tile_list[(long long)blockIdx.x * NT + threadIdx.x].x;
+        act = edge_eval<true>(v, T, C, color, grad_color, rast, V, pos, tri, recs, p0, b, g, v0, it0);
+    }
+    // Pixels of one triangle add into the same 12 clip-space components.
+    // Group the wave's lanes by triangle, add each lane's components into
+    // its group's LDS slot, then flush: one global atomic per (group,
+    // component), spread over distinct lanes.
+    __shared__ float slot[NT / 64][64][13];
+    __shared__ int sv[NT / 64][64][3];
+    const int w = threadIdx.x / 64;
+    int grp = -1, ng = 0;
+    unsigned long long todo = __ballot(act);
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int key = __shfl(it0, leader, 64);
+        const unsigned long long same = __ballot(act && it0 == key);
+        if (act && it0 == key) grp = ng;
+        if (lane == leader) { sv[w][ng][0] = v0[0]; sv[w][ng][1] = v0[1]; sv[w][ng][2] = v0[2]; }
+        ++ng;
+        todo &= ~same;
+    }
+    for (int i = lane; i < ng * 13; i += 64) (&slot[w][0][0])[i] = 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (act)
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            if (g[k] != 0.f) atomicAdd(&slot[w][grp][k], g[k]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    float* gp = grad_pos + (long long)b * V * 4;
+    for (int i = lane; i < ng * 12; i += 64) {
+        const int q = i / 12, k = i - q * 12;
+        const float x = slot[w][q][k];
+        if (x != 0.f) unsafeAtomicAdd(&gp[sv[w][q][k / 4] * 4 + (k % 4)], x);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_edge_grad_tiles(const Views& v, int C, const float* color, const float* grad_color,
+                                  const float* rast, int V, const float* pos, float* grad_pos, int T,
+                                  const int* tri, const TriRecord* recs, const int2* tile_list,
+                                  const int* tile_ncov, hipStream_t s) {
+    const int n_tiles = v.tile_base[v.n];
+    if (n_tiles == 0) return hipSuccess;
+    const float4* r4 = reinterpret_cast<const float4*>(rast);
+    const float4* p4 = reinterpret_cast<const float4*>(pos);
+    switch (v.tile) {
+        case 8:
+            k_edge_grad_tiles<64><<<n_tiles, 64, 0, s>>>(v, T, C, color, grad_color, r4, V, p4, grad_pos, tri,
+                                                         recs, tile_list, tile_ncov);
+            break;
+        case 16:
+            k_edge_grad_tiles<256><<<n_tiles, 256, 0, s>>>(v, T, C, color, grad_color, r4, V, p4, grad_pos, tri,
+                                                           recs, tile_list, tile_ncov);
+            break;
+        case 32:
+            k_edge_grad_tiles<1024><<<n_tiles, 1024, 0, s>>>(v, T, C, color, grad_color, r4, V, p4, grad_pos, tri,
+                                                             recs, tile_list, tile_ncov);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_grad_list(const Views& v, int C, const float* color, const float* grad_color,
+                                 const float* rast, int V, const float* pos, float* grad_pos, int T,
+                                 const int* tri, const int* list, const int* n_list, long long max_n,
+                                 hipStream_t s) {
+    if (max_n <= 0) return hipSuccess;
+    long long blocks = (max_n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    k_edge_grad_list<<<static_cast<unsigned>(blocks), 256, 0, s>>>(
+        v, T, C, color, grad_color, reinterpret_cast<const float4*>(rast), V,
+        reinterpret_cast<const float4*>(pos), grad_pos, tri, list, n_list);
+    return hipGetLastError();
+}
 
 hipError_t launch_edge_grad(const Views& v, int C, const float* color, const float* grad_color,
                             const float* rast, int V, const float* pos, float* grad_pos, int T,

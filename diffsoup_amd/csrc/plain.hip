@@ -16,9 +16,10 @@
 //
 // The host-visible counts (candidates, valid fragments) are returned with a
 // blocking copy, as the reference does; everything else stays on the stream.
-#include "../../include/diffsoup_hip.h"
+#include "diffsoup_hip.h"
 #include "ds_kernels.h"
 #include "ds_math.h"
+#include "ds_scan.h"
 
 namespace ds {
 namespace {
@@ -185,109 +186,6 @@ __global__ void k_aux(int H, int W, int C, const float* __restrict__ color, cons
     }
 }
 
-// ---- multi-block exclusive scan of int32 flags/counts (int64 output) ----
-constexpr int kScanItems = 4 * 1024;
-
-__global__ void __launch_bounds__(1024) k_block_sums(const int* __restrict__ in, long long n,
-                                                     long long* __restrict__ sums) {
-    __shared__ long long red[16];
-    const long long base = blockIdx.x * (long long)kScanItems;
-    long long s = 0;
-    for (int k = 0; k < 4; ++k) {
-        const long long i = base + k * 1024 + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long t = 0;
-        for (int k = 0; k < 16; ++k) t += red[k];
-        sums[blockIdx.x] = t;
-    }
-}
-
-// in-place exclusive scan of the block sums by one workgroup; sums[nb] = total
-__global__ void __launch_bounds__(1024) k_scan_sums(long long* __restrict__ sums, int nb) {
-    __shared__ long long ws[16];
-    __shared__ long long carry;
-    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        const int i = base + tid;
-        const long long v = i < nb ? sums[i] : 0;
-        long long inc = v;
-        for (int d = 1; d < 64; d <<= 1) {
-            const long long o = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += o;
-        }
-        if (lane == 63) ws[wv] = inc;
-        __syncthreads();
-        long long pre = 0, tot = 0;
-        for (int k = 0; k < 16; ++k) { if (k < wv) pre += ws[k]; tot += ws[k]; }
-        const long long c = carry;
-        if (i < nb) sums[i] = c + pre + inc - v;
-        __syncthreads();
-        if (tid == 0) carry = c + tot;
-        __syncthreads();
-    }
-    if (tid == 0) sums[nb] = carry;
-}
-
-template <typename OUT>
-__global__ void __launch_bounds__(1024) k_block_scan(const int* __restrict__ in, long long n,
-                                                     const long long* __restrict__ sums, OUT* __restrict__ out,
-                                                     int inclusive) {
-    __shared__ long long ws[16];
-    __shared__ long long carry;
-    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
-    if (tid == 0) carry = sums[blockIdx.x];
-    __syncthreads();
-    const long long base = blockIdx.x * (long long)kScanItems;
-    for (int k = 0; k < 4; ++k) {
-        const long long i = base + k * 1024 + tid;
-        const long long v = i < n ? in[i] : 0;
-        long long inc = v;
-        for (int d = 1; d < 64; d <<= 1) {
-            const long long o = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += o;
-        }
-        if (lane == 63) ws[wv] = inc;
-        __syncthreads();
-        long long pre = 0, tot = 0;
-        for (int q = 0; q < 16; ++q) { if (q < wv) pre += ws[q]; tot += ws[q]; }
-        const long long c = carry;
-        if (i < n) out[i] = static_cast<OUT>(c + pre + (inclusive ? inc : inc - v));
-        __syncthreads();
-        if (tid == 0) carry = c + tot;
-        __syncthreads();
-    }
-}
-
-// Scan n ints into out (inclusive or exclusive); tmp holds nb + 1 int64.
-// Returns the total through a blocking copy.
-template <typename OUT>
-hipError_t scan_ints(const int* in, long long n, OUT* out, long long* tmp, int inclusive, long long* total,
-                     hipStream_t s) {
-    const int nb = static_cast<int>((n + kScanItems - 1) / kScanItems);
-    if (nb == 0) {
-        if (total) *total = 0;
-        return hipSuccess;
-    }
-    k_block_sums<<<nb, 1024, 0, s>>>(in, n, tmp);
-    k_scan_sums<<<1, 1024, 0, s>>>(tmp, nb);
-    k_block_scan<OUT><<<nb, 1024, 0, s>>>(in, n, tmp, out, inclusive);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (total) {
-        e = hipMemcpyAsync(total, tmp + nb, sizeof(long long), hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(s);
-    }
-    return e;
-}
-
 }  // namespace
 
 int plain_fail(int code, const char* msg);
@@ -296,9 +194,7 @@ int plain_fail(int code, const char* msg);
 
 extern "C" {
 
-size_t ds_plain_scan_tmp_bytes(int64_t n) {
-    return sizeof(long long) * (size_t)((n + ds::kScanItems - 1) / ds::kScanItems + 2);
-}
+size_t ds_plain_scan_tmp_bytes(int64_t n) { return ds::scan_tmp_bytes(n); }
 
 int64_t ds_plain_triangle_rects(int H, int W, int B, int V, const float* pos, int T, const int* tri,
                                 int* rects, int64_t* prefix, int* counts, void* scan_tmp, void* stream) {
@@ -311,8 +207,10 @@ int64_t ds_plain_triangle_rects(int H, int W, int B, int V, const float* pos, in
     k_rects<<<nblk(n), kBS, 0, s>>>(H, W, B, V, reinterpret_cast<const float4*>(pos), T, tri,
                                     reinterpret_cast<int4*>(rects), counts);
     long long total = 0;
-    if (scan_ints<long long>(counts, n, reinterpret_cast<long long*>(prefix), static_cast<long long*>(scan_tmp), 1,
-                             &total, s) != hipSuccess)
+    if (launch_inclusive_scan_i64(counts, n, reinterpret_cast<long long*>(prefix), scan_tmp, s) != hipSuccess ||
+        hipMemcpyAsync(&total, scan_total_slot(scan_tmp, n), sizeof(long long), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return plain_fail(DS_ERR_HIP, "rect scan");
     return total;
 }
@@ -336,8 +234,11 @@ int64_t ds_plain_compact(int64_t num_frags, const int* frag_pix, const float* fr
     if (num_frags <= 0) return 0;
     k_valid_flags<<<nblk(num_frags), kBS, 0, s>>>(num_frags, frag_pix, flags);
     long long total = 0;
-    // exclusive positions overwrite the flags in place after they are summed per block
-    if (scan_ints<int>(flags, num_frags, flags, static_cast<long long*>(scan_tmp), 0, &total, s) != hipSuccess)
+    // exclusive positions overwrite the flags in place
+    if (launch_exclusive_scan_inplace(flags, num_frags, scan_tmp, s) != hipSuccess ||
+        hipMemcpyAsync(&total, scan_total_slot(scan_tmp, num_frags), sizeof(long long), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return plain_fail(DS_ERR_HIP, "compaction scan");
     k_compact<<<nblk(num_frags), kBS, 0, s>>>(num_frags, frag_pix, reinterpret_cast<const float4*>(frag_attrs),
                                               flags, frag_pix_out, reinterpret_cast<float4*>(frag_attrs_out));

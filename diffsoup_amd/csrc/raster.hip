@@ -250,10 +250,13 @@ __global__ void __launch_bounds__(TS * TS)
 k_resolve(Views v, int T, AlphaSource a, int mask_mode, unsigned long long seed,
           const TriRecord* __restrict__ recs, const int* __restrict__ tile_off,
           const int* __restrict__ bins, long long* __restrict__ stats, float4* __restrict__ rast,
-          BackgroundShade bg) {
+          int* __restrict__ tile_cov, BackgroundShade bg) {
     constexpr int NT = TS * TS;
+    constexpr int NW = NT / kWave;
     __shared__ TileStage<TS> st;
-    __shared__ float wave_loss[NT / kWave];
+    __shared__ float wave_loss[NW];
+    __shared__ int wave_cnt[NW];
+    __shared__ int wave_cov[NW];
     const TileWalk w = tile_walk<TS>(v, T, tile_off, stats);
 
     long long best = kKeyEmpty;
@@ -294,10 +297,12 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, unsigned long long seed,
         rast[p] = bt >= 0 ? make_float4(bb0, bb1, bd, static_cast<float>(bt + 1))
                           : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
     if constexpr (SHADE) {
         // Background pixels are shaded here (constant colour, MSE term and its
-        // gradient, per-pixel L1 for the opacity gradient); covered pixels are
-        // appended to the list the MLP kernel consumes.
+        // gradient, per-pixel L1 for the opacity gradient).  Covered pixels are
+        // listed per tile in lane order, each with its rank among the covered
+        // pixels of its triangle (one atomic per distinct triangle per wave).
         float l = 0.f;
         if (inside && bt < 0) {
             const float* tg = bg.target + p * 3;
@@ -315,27 +320,83 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, unsigned long long seed,
         }
         const bool cov = inside && bt >= 0;
         const unsigned long long m = __ballot(cov);
-        const int lane = threadIdx.x % kWave;
-        int base = 0;
-        if (lane == 0 && m) base = atomicAdd(bg.n_covered, __popcll(m));
-        base = __shfl(base, 0, kWave);
-        if (cov) {
-            const int rank = __popcll(m & ((1ull << lane) - 1ull));
-            bg.covered[base + rank] = static_cast<int>(p);
+        const unsigned long long below = m & ((1ull << lane) - 1ull);
+        // rank within the triangle: group equal triangle ids of the wave, then
+        // every group's first lane claims the group's slots with one atomic
+        // (all groups in the same instruction)
+        int leader = -1, local = 0, group = 0;
+        unsigned long long todo = m;
+        while (todo) {
+            const int l0 = __ffsll(static_cast<long long>(todo)) - 1;
+            const int tl = __shfl(bt, l0, kWave);
+            const unsigned long long same = __ballot(cov && bt == tl);
+            if (cov && bt == tl) {
+                leader = l0;
+                local = __popcll(same & ((1ull << lane) - 1ull));
+                group = __popcll(same);
+            }
+            todo &= ~same;
         }
+        int base = 0;
+        if (cov && lane == leader) base = atomicAdd(&bg.tri_count[bt], group);
+        base = __shfl(base, leader < 0 ? lane : leader, kWave);
+        const int rank = base + local;
         for (int d = kWave / 2; d > 0; d >>= 1) l += __shfl_xor(l, d, kWave);
-        if (lane == 0) wave_loss[threadIdx.x / kWave] = l;
+        if (lane == 0) {
+            wave_loss[wv] = l;
+            wave_cnt[wv] = __popcll(m);
+        }
         __syncthreads();
+        int off = 0;
+        for (int k = 0; k < wv; ++k) off += wave_cnt[k];
+        if (cov) bg.tile_list[(long long)blockIdx.x * NT + off + __popcll(below)] = make_int2(static_cast<int>(p), rank);
         if (threadIdx.x == 0) {
             float sum = 0.f;
-            for (int k = 0; k < NT / kWave; ++k) sum += wave_loss[k];
+            int n = 0;
+            for (int k = 0; k < NW; ++k) { sum += wave_loss[k]; n += wave_cnt[k]; }
             bg.loss_part[blockIdx.x] = sum;
+            bg.tile_ncov[blockIdx.x] = n;
         }
     }
-    covered = wave_sum_ll(covered);
-    if (threadIdx.x % kWave == 0 && covered)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&stats[kStatCovered]),
-                  static_cast<unsigned long long>(covered));
+    int cv = static_cast<int>(covered);
+    for (int d = kWave / 2; d > 0; d >>= 1) cv += __shfl_xor(cv, d, kWave);
+    if (lane == 0) wave_cov[wv] = cv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int k = 0; k < NW; ++k) n += wave_cov[k];
+        tile_cov[blockIdx.x] = n;
+    }
+}
+
+// Sum of the per-tile covered-fragment counts into the stats block.
+__global__ void __launch_bounds__(1024) k_sum_covered(const int* __restrict__ tile_cov, int n,
+                                                      long long* __restrict__ stats) {
+    __shared__ long long red[16];
+    long long s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += tile_cov[i];
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int k = 0; k < 16; ++k) t += red[k];
+        stats[kStatCovered] = t;
+    }
+}
+
+// list[tri_off[t] + rank] = pixel for every covered pixel of every tile.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fill_lists(const int2* __restrict__ tile_list,
+                                                   const int* __restrict__ tile_ncov,
+                                                   const float4* __restrict__ rast,
+                                                   const int* __restrict__ tri_off, int4* __restrict__ list) {
+    const int k = threadIdx.x;
+    if (k >= tile_ncov[blockIdx.x]) return;
+    const int2 e = tile_list[(long long)blockIdx.x * NT + k];
+    const float4 r = rast[e.x];
+    const int t = static_cast<int>(r.w) - 1;
+    list[tri_off[t] + e.y] = make_int4(e.x, __float_as_int(r.x), __float_as_int(r.y), t);
 }
 
 // Score-function gradient of the per-pixel L1 loss w.r.t. fragment opacity,
@@ -412,7 +473,7 @@ size_t raster_workspace_bytes(int B, int T, int n_tiles, int64_t capacity) {
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
     return up(sizeof(long long) * kStatCount) + up(sizeof(TriRecord) * (size_t)B * T) +
            up(sizeof(int) * (size_t)n_tiles) + up(sizeof(int) * ((size_t)n_tiles + 1)) +
-           up(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
+           up(sizeof(int) * (size_t)n_tiles) + up(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
 }
 
 RasterWorkspace raster_workspace_carve(void* ws, int B, int T, int n_tiles, int64_t capacity) {
@@ -423,6 +484,7 @@ RasterWorkspace raster_workspace_carve(void* ws, int B, int T, int n_tiles, int6
     w.recs = reinterpret_cast<TriRecord*>(p);            p += up(sizeof(TriRecord) * (size_t)B * T);
     w.tile_count = reinterpret_cast<int*>(p);            p += up(sizeof(int) * (size_t)n_tiles);
     w.tile_off = reinterpret_cast<int*>(p);              p += up(sizeof(int) * ((size_t)n_tiles + 1));
+    w.tile_cov = reinterpret_cast<int*>(p);              p += up(sizeof(int) * (size_t)n_tiles);
     w.bins = reinterpret_cast<int*>(p);
     w.capacity = capacity;
     return w;
@@ -470,19 +532,20 @@ static hipError_t resolve_dispatch(const Views& v, int T, const AlphaSource& a, 
     switch (v.tile) {
         case 8:
             k_resolve<8, SHADE><<<n_tiles, 64, 0, s>>>(v, T, a, mask_mode, seed, w.recs, w.tile_off,
-                                                       w.bins, w.stats, out, bg);
+                                                       w.bins, w.stats, out, w.tile_cov, bg);
             break;
         case 16:
             k_resolve<16, SHADE><<<n_tiles, 256, 0, s>>>(v, T, a, mask_mode, seed, w.recs, w.tile_off,
-                                                         w.bins, w.stats, out, bg);
+                                                         w.bins, w.stats, out, w.tile_cov, bg);
             break;
         case 32:
             k_resolve<32, SHADE><<<n_tiles, 1024, 0, s>>>(v, T, a, mask_mode, seed, w.recs, w.tile_off,
-                                                          w.bins, w.stats, out, bg);
+                                                          w.bins, w.stats, out, w.tile_cov, bg);
             break;
         default:
             return hipErrorInvalidValue;
     }
+    k_sum_covered<<<1, 1024, 0, s>>>(w.tile_cov, n_tiles, w.stats);
     return hipGetLastError();
 }
 
@@ -518,6 +581,20 @@ hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const
             break;
         default:
             return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
+                                      const float* rast, const int* tri_off, int4* list, hipStream_t s) {
+    const int n_tiles = v.tile_base[v.n];
+    if (n_tiles == 0) return hipSuccess;
+    const float4* r4 = reinterpret_cast<const float4*>(rast);
+    switch (v.tile) {
+        case 8: k_fill_lists<64><<<n_tiles, 64, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list); break;
+        case 16: k_fill_lists<256><<<n_tiles, 256, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list); break;
+        case 32: k_fill_lists<1024><<<n_tiles, 1024, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -3,8 +3,14 @@
 // MSE against the target, and the whole adjoint (MLP input and weight
 // gradients, feature-lattice scatter) in the same pass.
 //
-// Work unit: one wave shades a tile of 16 covered pixels (the list the
-// training-step resolve kernel appends).  Every layer is an MFMA product with
+// Work unit: the covered pixels, grouped by triangle, are split into equal
+// contiguous ranges, one per wave.  For each triangle segment in its range a
+// wave stages the triangle's feature row (all levels, all channels) in LDS,
+// shades the segment's pixels (any view) in tiles of 16, accumulates the input
+// gradient into an LDS gradient row with LDS atomics, and writes that row
+// once - with plain coalesced stores when the triangle lies wholly in the
+// range, with global atomics only for the few triangles split between two
+// ranges.  Every layer is an MFMA product with
 // the layer's output units on the M axis and the 16 pixels on the N axis.  In
 // the accumulator layout of v_mfma_f32_16x16x{4f32,16bf16} lane l holds rows
 // 4(l/16) + r, r = 0..3, of pixel l % 16, which is exactly the B-operand
@@ -62,9 +68,13 @@ struct Cfg {
     static constexpr int TILE_ELEMS = 4 * kWave;
     static constexpr int BLOB_ELEMS = NTILES * TILE_ELEMS;
     static constexpr int NPARAM = DH * DIN + DH * DH + 3 * DH + DH + DH + 3 + 1;
-    // per-wave LDS staging for the weight-gradient transposes (floats)
+    // per-wave LDS staging for the weight-gradient transposes (floats): one
+    // (delta, activation) pair at a time - (D1, X), (D2, H1), (D3, H2)
     static constexpr int SX = DINP + 4, SH = DH + 4, SD3 = 16 + 4;
-    static constexpr int STAGE = 16 * (SX + 4 * SH + SD3);
+    static constexpr int STAGE = 16 * ((SH + SX > 2 * SH ? SH + SX : 2 * SH) > SD3 + SH
+                                           ? (SH + SX > 2 * SH ? SH + SX : 2 * SH)
+                                           : SD3 + SH);
+    static constexpr int CORN = 16 * 8 * NL;
     static_assert(C % 4 == 0, "channels per level must be a multiple of 4");
     static_assert(DH % 16 == 0, "hidden width must be a multiple of 16");
 };
@@ -72,6 +82,13 @@ struct Cfg {
 // One MFMA step: acc += A * B on a 16x16 tile.  For f32 an operand unit is a
 // single float (K = 4 per step over 4 lane groups), for bf16 four values
 // (K = 16 per step).
+// Weight of lattice vertex vtx in a pixel's cell (0 when it is not a corner):
+// corners staged as (idx0, idx1, idx2, w0, w1, w2) per pixel.
+__device__ __forceinline__ float corner_weight(const float* sc, int vtx) {
+    const int i0 = __float_as_int(sc[0]), i1 = __float_as_int(sc[1]), i2 = __float_as_int(sc[2]);
+    return i0 == vtx ? sc[3] : (i1 == vtx ? sc[4] : (i2 == vtx ? sc[5] : 0.f));
+}
+
 template <bool BF16>
 struct Mma;
 
@@ -93,6 +110,18 @@ struct Mma<false> {
         for (int s = 0; s < 4; ++s) {
             const int px = 4 * s + g;
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[px * sa + i], Bm[px * sb + i], acc, 0, 0, 0);
+        }
+        return acc;
+    }
+    // lattice scatter over 16 pixels: A[vertex][px] = corner weight, B[px][f] = dX
+    __device__ static __forceinline__ f32x4 scatter(const float* corners, const float* bx, int sb, int lane,
+                                                    int vtx) {
+        const int g = lane / 16, j = lane % 16;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < 4; ++s) {
+            const int px = 4 * s + g;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(corner_weight(corners + px * 8, vtx), bx[px * sb + j], acc,
+                                                       0, 0, 0);
         }
         return acc;
     }
@@ -122,6 +151,17 @@ struct Mma<true> {
         }
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pack(a), pack(b), acc, 0, 0, 0);
     }
+    __device__ static __forceinline__ f32x4 scatter(const float* corners, const float* bx, int sb, int lane,
+                                                    int vtx) {
+        const int g = lane / 16, j = lane % 16;
+        f32x4 a, b;
+        for (int e = 0; e < 4; ++e) {
+            const int px = 4 * g + e;
+            a[e] = corner_weight(corners + px * 8, vtx);
+            b[e] = bx[px * sb + j];
+        }
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pack(a), pack(b), f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
 };
 
 // Offsets (in floats) of a layer tile pair inside the LDS blob copy.  A bf16
@@ -143,19 +183,17 @@ __device__ __forceinline__ void dense(const float* blobs, int base, int lane, co
 
 template <class K>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
-k_shade(ShadeArgs a) {
+k_shade(ShadeArgs a, int rowp) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
     extern __shared__ float smem[];
     float* blobs = smem;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
-    float* stage = smem + K::BLOB_ELEMS + wave * K::STAGE;
-    float* sX = stage;
-    float* sH1 = sX + 16 * K::SX;
-    float* sH2 = sH1 + 16 * K::SH;
-    float* sD1 = sH2 + 16 * K::SH;
-    float* sD2 = sD1 + 16 * K::SH;
-    float* sD3 = sD2 + 16 * K::SH;
+    float* frow = smem + K::BLOB_ELEMS + wave * (2 * rowp + K::STAGE + K::CORN);
+    float* grow = frow + rowp;
+    float* stage = grow + rowp;
+    float* scr = stage + K::STAGE;   // per (level, pixel): 3 lattice indices, 3 weights
+    const int row = a.S_tot * C;
 
     // stage the weight blobs
     {
@@ -181,134 +219,225 @@ k_shade(ShadeArgs a) {
         for (int k = 0; k < K::NL; ++k) { loff[k] = o; o += feats_at_level(a.lo + k); }
     }
 
-    const int n = *a.n_covered;
+    // Each wave owns an equal, contiguous range of the triangle-grouped pixel
+    // list and walks the triangle segments inside it.  A triangle whose
+    // pixels all fall in one wave's range gets its gradient row stored; a
+    // triangle split between waves is added with atomics into the zeroed table.
     const int n_waves = gridDim.x * kWavesPerBlock;
-    for (int tile = blockIdx.x * kWavesPerBlock + wave; tile * 16 < n; tile += n_waves) {
-        const int idx = tile * 16 + j;
-        const bool valid = idx < n;
-        int p = 0, t = 0;
-        float b0 = 0.f, b1 = 0.f;
-        if (valid) {
-            p = a.covered[idx];
-            const float4 r4 = a.rast[p];
-            b0 = r4.x; b1 = r4.y;
-            t = static_cast<int>(r4.w) - 1;
+    const int wid = blockIdx.x * kWavesPerBlock + wave;
+    const int total = a.tri_off[a.T];
+    const int per = (total + n_waves - 1) / n_waves;
+    const int r0 = min(total, wid * per), r1 = min(total, r0 + per);
+    int t = 0;
+    {   // first triangle whose segment ends after r0
+        int lo = 0, hi = a.T;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.tri_off[mid + 1] <= r0) lo = mid + 1; else hi = mid;
         }
-        const float* frow = a.feat + (long long)t * a.S_tot * C;
-
-        // ---- gather: lane (g, j) holds features 16 tt + 4 g + r of pixel j
-        f32x4 x[NI];
-        for (int tt = 0; tt < NI; ++tt) {
-            x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
-            const int f0 = 16 * tt + 4 * g;
-            if (valid && f0 < DIN) {
-                const int lv = f0 / C, c0 = f0 % C;
-                const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
-                const float* fl = frow + (long long)loff[lv] * C + c0;
-                const float4 v0 = *reinterpret_cast<const float4*>(fl + cr.idx[0] * C);
-                const float4 v1 = *reinterpret_cast<const float4*>(fl + cr.idx[1] * C);
-                const float4 v2 = *reinterpret_cast<const float4*>(fl + cr.idx[2] * C);
-                float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
-                e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
-                e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;
-                e2 += cr.w[0] * v0.z; e2 += cr.w[1] * v1.z; e2 += cr.w[2] * v2.z;
-                e3 += cr.w[0] * v0.w; e3 += cr.w[1] * v1.w; e3 += cr.w[2] * v2.w;
-                x[tt] = f32x4{e0, e1, e2, e3};
-            }
+        t = lo;
+    }
+    for (int pos = r0; pos < r1; ++t) {
+        const int tb = a.tri_off[t], te = a.tri_off[t + 1];
+        if (te <= pos) continue;
+        const int beg = pos, end = min(te, r1);
+        const int npx = end - beg;
+        const bool owned = tb >= r0 && te <= r1;
+        pos = end;
+        float4* g4 = reinterpret_cast<float4*>(grow);
+        float4* f4 = reinterpret_cast<float4*>(frow);
+        for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        {
+            const float4* src = reinterpret_cast<const float4*>(a.feat + (long long)t * row);
+            for (int i = lane; i < row / 4; i += kWave) f4[i] = src[i];
         }
-
-        // ---- forward
-        f32x4 h1[NH], h2[NH], y[1];
-        for (int m = 0; m < NH; ++m)
-            for (int r = 0; r < 4; ++r) { h1[m][r] = a.b1[16 * m + 4 * g + r]; h2[m][r] = a.b2[16 * m + 4 * g + r]; }
-        for (int r = 0; r < 4; ++r) y[0][r] = (g == 0 && r < 3) ? a.b3[r] : 0.f;
-        dense<K, NH, NI>(blobs, K::L1, lane, x, h1);
-        for (int m = 0; m < NH; ++m)
-            for (int r = 0; r < 4; ++r) h1[m][r] = fmaxf(h1[m][r], 0.f);
-        dense<K, NH, NH>(blobs, K::L2, lane, h1, h2);
-        for (int m = 0; m < NH; ++m)
-            for (int r = 0; r < 4; ++r) h2[m][r] = fmaxf(h2[m][r], 0.f);
-        dense<K, 1, NH>(blobs, K::L3, lane, h2, y);
-
-        // ---- blend, loss and output gradient (lanes of group 0 own pixel j)
-        f32x4 d3[1] = {f32x4{0.f, 0.f, 0.f, 0.f}};
-        float dxr[4] = {0.f, 0.f, 0.f, 0.f};
-        if (g == 0 && valid) {
-            const float res = x[0][3];
-            const float* tg = a.target + (long long)p * 3;
-            float* col = a.color + (long long)p * 3;
-            float* gc = a.grad_color + (long long)p * 3;
-            float l1 = 0.f, dres = 0.f;
-            for (int c = 0; c < 3; ++c) {
-                const float s = sigmoidf(y[0][c]);
-                const float rgb = x[0][c];
-                const float out = (1.0f - res) * rgb + res * s;
-                const float d = out - tg[c];
-                loss += d * d;
-                l1 += fabsf(d);
-                const float go = a.grad_scale * d;
-                col[c] = out;
-                gc[c] = go;
-                dxr[c] = go * (1.0f - res);
-                dres += go * (s - rgb);
-                d3[0][c] = go * res * s * (1.0f - s);
-            }
-            dxr[3] = dres;
-            a.pix_loss[p] = l1;
-        }
-
-        // ---- backward through the MLP
-        f32x4 dh2[NH], dh1[NH], dx[NI];
-        for (int m = 0; m < NH; ++m) dh2[m] = dh1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int m = 0; m < NI; ++m) dx[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        dense<K, NH, 1>(blobs, K::L3T, lane, d3, dh2);
-        for (int m = 0; m < NH; ++m)
-            for (int r = 0; r < 4; ++r) dh2[m][r] = h2[m][r] > 0.f ? dh2[m][r] : 0.f;
-        dense<K, NH, NH>(blobs, K::L2T, lane, dh2, dh1);
-        for (int m = 0; m < NH; ++m)
-            for (int r = 0; r < 4; ++r) dh1[m][r] = h1[m][r] > 0.f ? dh1[m][r] : 0.f;
-        dense<K, NI, NH>(blobs, K::L1T, lane, dh1, dx);
-        if (g == 0)
-            for (int r = 0; r < 4; ++r) dx[0][r] += dxr[r];
-
-        // ---- scatter the input gradient onto the feature lattice
-        if (valid) {
-            float* grow = a.grad_feat + (long long)t * a.S_tot * C;
-            for (int tt = 0; tt < NI; ++tt) {
-                const int f0 = 16 * tt + 4 * g;
-                if (f0 >= DIN) continue;
-                const int lv = f0 / C, c0 = f0 % C;
-                const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
-                float* gl = grow + (long long)loff[lv] * C + c0;
-                for (int i = 0; i < 3; ++i)
-                    for (int r = 0; r < 4; ++r)
-                        unsafeAtomicAdd(&gl[cr.idx[i] * C + r], dx[tt][r] * cr.w[i]);
-            }
-        }
-
-        // ---- bias gradients stay per lane until the end
-        for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
-        gb3 += d3[0];
-
-        // ---- weight gradients: transpose through LDS, contract over the 16 pixels
-        for (int tt = 0; tt < NI; ++tt)
-            for (int r = 0; r < 4; ++r) sX[j * K::SX + 16 * tt + 4 * g + r] = x[tt][r];
-        for (int m = 0; m < NH; ++m)
-            for (int r = 0; r < 4; ++r) {
-                const int u = 16 * m + 4 * g + r;
-                sH1[j * K::SH + u] = h1[m][r];
-                sH2[j * K::SH + u] = h2[m][r];
-                sD1[j * K::SH + u] = dh1[m][r];
-                sD2[j * K::SH + u] = dh2[m][r];
-            }
-        for (int r = 0; r < 4; ++r) sD3[j * K::SD3 + 4 * g + r] = d3[0][r];
         wave_lds_sync();
-        for (int m = 0; m < NH; ++m) {
-            for (int nn = 0; nn < NI; ++nn)
-                gW1[m][nn] = Mma<K::BF16>::outer(sD1 + 16 * m, K::SH, sX + 16 * nn, K::SX, lane, gW1[m][nn]);
-            for (int nn = 0; nn < NH; ++nn)
-                gW2[m][nn] = Mma<K::BF16>::outer(sD2 + 16 * m, K::SH, sH1 + 16 * nn, K::SH, lane, gW2[m][nn]);
-            gW3[m] = Mma<K::BF16>::outer(sD3, K::SD3, sH2 + 16 * m, K::SH, lane, gW3[m]);
+
+        int4 rec = j < npx ? a.list[beg + j] : make_int4(0, 0, 0, 0);
+        for (int c0 = 0; c0 < npx; c0 += 16) {
+            const int idx = c0 + j;
+            const bool valid = idx < npx;
+            const int p = rec.x;
+            const float b0 = __int_as_float(rec.y), b1 = __int_as_float(rec.z);
+            // next tile's records and this tile's target are in flight while it shades
+            const int4 next = idx + 16 < npx ? a.list[beg + idx + 16] : make_int4(0, 0, 0, 0);
+            float tg0 = 0.f, tg1 = 0.f, tg2 = 0.f;
+            if (g == 0 && valid) {
+                const float* tg = a.target + (long long)p * 3;
+                tg0 = tg[0]; tg1 = tg[1]; tg2 = tg[2];
+            }
+
+            // ---- gather from the LDS row: lane (g, j) holds features 16 tt + 4 g + r of pixel j
+            f32x4 x[NI];
+            for (int tt = 0; tt < NI; ++tt) {
+                x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                const int f0 = 16 * tt + 4 * g;
+                if (!valid && f0 < DIN && f0 % C == 0) {
+                    float* sc = scr + ((f0 / C) * 16 + j) * 8;
+                    sc[0] = sc[1] = sc[2] = __int_as_float(-1);
+                }
+                if (valid && f0 < DIN) {
+                    const int lv = f0 / C, c = f0 % C;
+                    const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
+                    if (c == 0) {
+                        float* sc = scr + (lv * 16 + j) * 8;
+                        sc[0] = __int_as_float(static_cast<int>(cr.idx[0]));
+                        sc[1] = __int_as_float(static_cast<int>(cr.idx[1]));
+                        sc[2] = __int_as_float(static_cast<int>(cr.idx[2]));
+                        sc[3] = cr.w[0]; sc[4] = cr.w[1]; sc[5] = cr.w[2];
+                    }
+                    const float* fl = frow + loff[lv] * C + c;
+                    const float4 v0 = *reinterpret_cast<const float4*>(fl + cr.idx[0] * C);
+                    const float4 v1 = *reinterpret_cast<const float4*>(fl + cr.idx[1] * C);
+                    const float4 v2 = *reinterpret_cast<const float4*>(fl + cr.idx[2] * C);
+                    float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+                    e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
+                    e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;
+                    e2 += cr.w[0] * v0.z; e2 += cr.w[1] * v1.z; e2 += cr.w[2] * v2.z;
+                    e3 += cr.w[0] * v0.w; e3 += cr.w[1] * v1.w; e3 += cr.w[2] * v2.w;
+                    x[tt] = f32x4{e0, e1, e2, e3};
+                }
+            }
+
+            // ---- forward
+            f32x4 h1[NH], h2[NH], y[1];
+            for (int m = 0; m < NH; ++m)
+                for (int r = 0; r < 4; ++r) { h1[m][r] = a.b1[16 * m + 4 * g + r]; h2[m][r] = a.b2[16 * m + 4 * g + r]; }
+            for (int r = 0; r < 4; ++r) y[0][r] = (g == 0 && r < 3) ? a.b3[r] : 0.f;
+            dense<K, NH, NI>(blobs, K::L1, lane, x, h1);
+            for (int m = 0; m < NH; ++m)
+                for (int r = 0; r < 4; ++r) h1[m][r] = fmaxf(h1[m][r], 0.f);
+            dense<K, NH, NH>(blobs, K::L2, lane, h1, h2);
+            for (int m = 0; m < NH; ++m)
+                for (int r = 0; r < 4; ++r) h2[m][r] = fmaxf(h2[m][r], 0.f);
+            dense<K, 1, NH>(blobs, K::L3, lane, h2, y);
+
+            // ---- blend, loss and output gradient (lanes of group 0 own pixel j)
+            f32x4 d3[1] = {f32x4{0.f, 0.f, 0.f, 0.f}};
+            float dxr[4] = {0.f, 0.f, 0.f, 0.f};
+            if (g == 0 && valid) {
+                const float res = x[0][3];
+                const float tg[3] = {tg0, tg1, tg2};
+                float* col = a.color + (long long)p * 3;
+                float* gc = a.grad_color + (long long)p * 3;
+                float l1 = 0.f, dres = 0.f;
+                for (int c = 0; c < 3; ++c) {
+                    const float sg = sigmoidf(y[0][c]);
+                    const float rgb = x[0][c];
+                    const float out = (1.0f - res) * rgb + res * sg;
+                    const float d = out - tg[c];
+                    loss += d * d;
+                    l1 += fabsf(d);
+                    const float go = a.grad_scale * d;
+                    col[c] = out;
+                    gc[c] = go;
+                    dxr[c] = go * (1.0f - res);
+                    dres += go * (sg - rgb);
+                    d3[0][c] = go * res * sg * (1.0f - sg);
+                }
+                dxr[3] = dres;
+                a.pix_loss[p] = l1;
+            }
+
+            // ---- backward through the MLP
+            f32x4 dh2[NH], dh1[NH], dx[NI];
+            for (int m = 0; m < NH; ++m) dh2[m] = dh1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int m = 0; m < NI; ++m) dx[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+            dense<K, NH, 1>(blobs, K::L3T, lane, d3, dh2);
+            for (int m = 0; m < NH; ++m)
+                for (int r = 0; r < 4; ++r) dh2[m][r] = h2[m][r] > 0.f ? dh2[m][r] : 0.f;
+            dense<K, NH, NH>(blobs, K::L2T, lane, dh2, dh1);
+            for (int m = 0; m < NH; ++m)
+                for (int r = 0; r < 4; ++r) dh1[m][r] = h1[m][r] > 0.f ? dh1[m][r] : 0.f;
+            dense<K, NI, NH>(blobs, K::L1T, lane, dh1, dx);
+            if (g == 0)
+                for (int r = 0; r < 4; ++r) dx[0][r] += dxr[r];
+
+            // ---- scatter the input gradient onto the LDS gradient row.  Per
+            // level, G[v][f] = sum_px W[px][v] dX[px][f] is a 16-pixel contraction:
+            // A = barycentric weight of lattice vertex v in pixel px (rebuilt
+            // from the staged corners), B = dX transposed through LDS.  Every
+            // (vertex, channel) of the result belongs to one lane, so the row
+            // update has no conflicting lanes.
+            {
+                float* sdx = stage;
+                for (int tt = 0; tt < NI; ++tt)
+                    for (int r = 0; r < 4; ++r) sdx[j * K::SX + 16 * tt + 4 * g + r] = dx[tt][r];
+                wave_lds_sync();
+                for (int tt = 0; tt < NI; ++tt) {
+                    if (16 * tt >= DIN) break;
+                    const int lv_lo = (16 * tt) / C;
+                    const int lv_hi = min(K::NL - 1, (16 * tt + 15) / C);
+                    for (int lv = lv_lo; lv <= lv_hi; ++lv) {
+                        const int S = static_cast<int>(feats_at_level(a.lo + lv));
+                        for (int m = 0; m * 16 < S; ++m) {
+                            const f32x4 acc = Mma<K::BF16>::scatter(scr + lv * 16 * 8, sdx + 16 * tt, K::SX, lane,
+                                                                    16 * m + j);
+                            const int f = 16 * tt + j;
+                            if (f < DIN && f / C == lv) {
+                                float* gcol = grow + loff[lv] * C + (f % C);
+                                for (int r = 0; r < 4; ++r) {
+                                    const int vtx = 16 * m + 4 * g + r;
+                                    if (vtx < S) gcol[vtx * C] += acc[r];
+                                }
+                            }
+                        }
+                    }
+                }
+                wave_lds_sync();
+            }
+
+            // ---- bias gradients stay per lane until the end
+            for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
+            gb3 += d3[0];
+
+            // ---- weight gradients: transpose (delta, activation) pairs through
+            // LDS one at a time and contract over the 16 pixels
+            {
+                float* sD = stage;
+                float* sA = stage + 16 * K::SH;
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) sD[j * K::SH + 16 * m + 4 * g + r] = dh1[m][r];
+                for (int tt = 0; tt < NI; ++tt)
+                    for (int r = 0; r < 4; ++r) sA[j * K::SX + 16 * tt + 4 * g + r] = x[tt][r];
+                wave_lds_sync();
+                for (int m = 0; m < NH; ++m)
+                    for (int nn = 0; nn < NI; ++nn)
+                        gW1[m][nn] = Mma<K::BF16>::outer(sD + 16 * m, K::SH, sA + 16 * nn, K::SX, lane, gW1[m][nn]);
+                wave_lds_sync();
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) {
+                        sD[j * K::SH + 16 * m + 4 * g + r] = dh2[m][r];
+                        sA[j * K::SH + 16 * m + 4 * g + r] = h1[m][r];
+                    }
+                wave_lds_sync();
+                for (int m = 0; m < NH; ++m)
+                    for (int nn = 0; nn < NH; ++nn)
+                        gW2[m][nn] = Mma<K::BF16>::outer(sD + 16 * m, K::SH, sA + 16 * nn, K::SH, lane, gW2[m][nn]);
+                wave_lds_sync();
+                float* sD3 = stage;
+                float* sH2 = stage + 16 * K::SD3;
+                for (int r = 0; r < 4; ++r) sD3[j * K::SD3 + 4 * g + r] = d3[0][r];
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) sH2[j * K::SH + 16 * m + 4 * g + r] = h2[m][r];
+                wave_lds_sync();
+                for (int m = 0; m < NH; ++m)
+                    gW3[m] = Mma<K::BF16>::outer(sD3, K::SD3, sH2 + 16 * m, K::SH, lane, gW3[m]);
+                wave_lds_sync();
+            }
+            rec = next;
+        }
+
+        // ---- the triangle's feature gradient row
+        if (owned) {
+            float4* dst = reinterpret_cast<float4*>(a.grad_feat + (long long)t * row);
+            for (int i = lane; i < row / 4; i += kWave) dst[i] = g4[i];
+        } else {
+            float* dst = a.grad_feat + (long long)t * row;
+            for (int i = lane; i < row; i += kWave) {
+                const float gv = grow[i];
+                if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
+            }
         }
         wave_lds_sync();
     }
@@ -422,18 +551,20 @@ __global__ void k_reduce_partials(const float* __restrict__ partials, int nblock
 
 template <class K>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
-    const size_t lds = sizeof(float) * (K::BLOB_ELEMS + kWavesPerBlock * K::STAGE);
+    const int rowp = (a.S_tot * K::C + 3) & ~3;
+    const size_t lds = sizeof(float) * (K::BLOB_ELEMS + kWavesPerBlock * (2 * rowp + K::STAGE + K::CORN));
     const size_t red = sizeof(float) * (K::BLOB_ELEMS + K::NPARAM);
     const size_t bytes = lds > red ? lds : red;
-    static bool opted_in = false;
-    if (bytes > 65536 && !opted_in) {
+    if (bytes > 160 * 1024) return hipErrorInvalidValue;
+    static size_t opted_in = 65536;
+    if (bytes > opted_in) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  static_cast<int>(bytes));
         if (e != hipSuccess) return e;
-        opted_in = true;
+        opted_in = bytes;
     }
-    k_shade<K><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a);
+    k_shade<K><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
     return hipGetLastError();
 }
 

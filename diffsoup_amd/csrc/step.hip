@@ -16,17 +16,18 @@
 // workspace and output buffers, so the sequence can be captured in a graph.
 // Composition follows examples/02_synthetic.py:354-388 with MSE in place of
 // L1 + SSIM and the features concatenated per level.
-#include "../../include/diffsoup_hip.h"
+#include "diffsoup_hip.h"
 #include "ds_kernels.h"
 #include "ds_math.h"
+#include "ds_scan.h"
 #include "ds_shade.h"
 
 namespace ds {
 namespace {
 
 __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict__ alpha, long long n_alpha,
-                            float* __restrict__ gfeat, long long n_feat, float* __restrict__ galpha,
-                            float* __restrict__ gpos, long long n_pos, int* __restrict__ n_covered) {
+                            float* __restrict__ galpha, float* __restrict__ gpos, long long n_pos,
+                            int* __restrict__ tri_count, int T, float* __restrict__ gfeat, long long n_feat) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     for (long long i = i0; i < n_alpha; i += stride) {
@@ -35,10 +36,9 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
     }
     float4* gf4 = reinterpret_cast<float4*>(gfeat);
     for (long long i = i0; i < n_feat / 4; i += stride) gf4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (long long i = (n_feat / 4) * 4 + i0; i < n_feat; i += stride) gfeat[i] = 0.f;
     float4* gp4 = reinterpret_cast<float4*>(gpos);
     for (long long i = i0; i < n_pos / 4; i += stride) gp4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (i0 == 0) *n_covered = 0;
+    for (long long i = i0; i < T; i += stride) tri_count[i] = 0;
 }
 
 // clip[b, v] = M_b [x, y, z, 1], row-major M (examples/utils.py:412-426)
@@ -100,7 +100,8 @@ inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((
 inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct StepLayout {
-    size_t raster, gpos, galpha, covered, ncov, bgloss, blobs, partials, total;
+    size_t raster, gpos, galpha, tile_list, tile_ncov, tri_count, tri_off, list, scan, bgloss, blobs, partials,
+        total;
     int n_tiles, nblocks, nparam;
     long long P;
 };
@@ -124,8 +125,13 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.raster = o; o += up256(raster_workspace_bytes(d->B, d->T, L.n_tiles, d->capacity));
     L.gpos = o; o += up256(sizeof(float) * 4 * (size_t)d->B * d->V);
     L.galpha = o; o += up256(sizeof(float) * (size_t)d->T * Sa);
-    L.covered = o; o += up256(sizeof(int) * (size_t)L.P);
-    L.ncov = o; o += 256;
+    const size_t nt2 = (size_t)d->tile * d->tile;
+    L.tile_list = o; o += up256(sizeof(int) * 2 * (size_t)L.n_tiles * nt2);
+    L.tile_ncov = o; o += up256(sizeof(int) * (size_t)L.n_tiles);
+    L.tri_count = o; o += up256(sizeof(int) * (size_t)(d->T > 0 ? d->T : 1));
+    L.tri_off = o; o += up256(sizeof(int) * ((size_t)d->T + 1));
+    L.list = o; o += up256(sizeof(int) * 4 * (size_t)L.P);
+    L.scan = o; o += up256(scan_tmp_bytes(d->T));
     L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
     L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(nl, d->channels, d->hidden));
     L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
@@ -184,8 +190,12 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     RasterWorkspace rw = raster_workspace_carve(ws + L.raster, d->B, d->T, L.n_tiles, d->capacity);
     float* gpos = reinterpret_cast<float*>(ws + L.gpos);
     float* galpha = reinterpret_cast<float*>(ws + L.galpha);
-    int* covered = reinterpret_cast<int*>(ws + L.covered);
-    int* ncov = reinterpret_cast<int*>(ws + L.ncov);
+    int2* tile_list = reinterpret_cast<int2*>(ws + L.tile_list);
+    int* tile_ncov = reinterpret_cast<int*>(ws + L.tile_ncov);
+    int* tri_count = reinterpret_cast<int*>(ws + L.tri_count);
+    int* tri_off = reinterpret_cast<int*>(ws + L.tri_off);
+    int4* list = reinterpret_cast<int4*>(ws + L.list);
+    void* scan_tmp = ws + L.scan;
     float* bgloss = reinterpret_cast<float*>(ws + L.bgloss);
     float* blobs = reinterpret_cast<float*>(ws + L.blobs);
     float* partials = reinterpret_cast<float*>(ws + L.partials);
@@ -206,10 +216,11 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
         return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
     };
 
-    const long long prep_n = std::max(std::max(n_alpha, n_feat / 4), std::max(n_pos / 4, 1LL));
+    const long long prep_n = std::max(std::max(n_alpha, (long long)d->T),
+                                      std::max(std::max(n_pos / 4, n_feat / 4), 1LL));
     DS_TRY(mark(DS_STAGE_PREP, true), "event");
     k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
-        d->alpha_logits, d->alpha, n_alpha, d->grad_feat, n_feat, galpha, gpos, n_pos, ncov);
+        d->alpha_logits, d->alpha, n_alpha, galpha, gpos, n_pos, tri_count, d->T, d->grad_feat, n_feat);
     DS_TRY(hipGetLastError(), "step prep");
     DS_TRY(mark(DS_STAGE_PREP, false), "event");
     DS_TRY(mark(DS_STAGE_BLOBS, true), "event");
@@ -226,21 +237,24 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     DS_TRY(launch_raster_bin(v, d->V, d->pos, d->T, d->tri, rw, s), "raster bin");
     DS_TRY(mark(DS_STAGE_BIN, false), "event");
     AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(Sa)};
-    BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, bgloss, covered, ncov,
+    BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, bgloss, tile_list, tile_ncov, tri_count,
                        2.0f * inv_n, d->background};
     DS_TRY(mark(DS_STAGE_RESOLVE, true), "event");
     DS_TRY(launch_raster_resolve_shade(v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, d->seed,
                                        d->rast, rw, bg, s),
            "resolve");
     DS_TRY(mark(DS_STAGE_RESOLVE, false), "event");
-    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), covered, ncov, d->feat, d->grad_feat,
+    DS_TRY(launch_exclusive_scan(tri_count, d->T, tri_off, scan_tmp, s), "triangle scan");
+    DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, tri_off, list, s), "triangle lists");
+    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), tri_off, list, d->T, d->feat, d->grad_feat,
                  static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->target, d->color,
                  d->grad_color, d->pix_loss, 2.0f * inv_n, partials};
     DS_TRY(mark(DS_STAGE_SHADE, true), "event");
     DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade");
     DS_TRY(mark(DS_STAGE_SHADE, false), "event");
     DS_TRY(mark(DS_STAGE_EDGE, true), "event");
-    DS_TRY(launch_edge_grad(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri, s),
+    DS_TRY(launch_edge_grad_tiles(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri,
+                                  rw.recs, tile_list, tile_ncov, s),
            "edge grad");
     DS_TRY(mark(DS_STAGE_EDGE, false), "event");
     DS_TRY(mark(DS_STAGE_AUX, true), "event");

@@ -130,6 +130,16 @@ class _PlainOpacityAuxFn(torch.autograd.Function):
         return None, None, None, None, None, None, (ctx.weight * grad_loss * ga).view(ctx.shape)
 
 
+def project_ordered(verts, m):
+    """clip = m @ [v, 1] for one row-major 4x4 m, summed as
+    ((m0 x + m1 y) + m2 z) + m3 with every product and sum rounded to fp32 -
+    the same order as the fused step's projection kernel, so both paths see
+    bit-identical clip positions.  Differentiable in verts."""
+    x, y, z = verts[:, 0], verts[:, 1], verts[:, 2]
+    cols = [m[i, 0] * x + m[i, 1] * y + m[i, 2] * z + m[i, 3] for i in range(4)]
+    return torch.stack(cols, dim=-1)
+
+
 class PlainStep:
     """The training step of the fused path, composed per view from the
     reference's building blocks; gradients land in ``.grad`` of the leaves."""
@@ -155,11 +165,10 @@ class PlainStep:
         tri = p.tri
         alpha = torch.sigmoid(p.alpha_logits)
         P = sum(h * w for h, w in resolutions)
-        vh = torch.cat([p.verts, torch.ones_like(p.verts[:, :1])], dim=-1)
         total = torch.zeros((), device=p.verts.device)
         off = 0
         for b, (H, W) in enumerate(resolutions):
-            pos = (vh @ mvp[b].T).unsqueeze(0).contiguous()
+            pos = project_ordered(p.verts, mvp[b]).unsqueeze(0).contiguous()
             rast = rasterize((H, W), pos.detach(), tri, self.alpha_level, alpha.detach(), True, seed, view_base=b)
             x = multires_feature_gather(rast, self.lo, self.hi, p.feat, "concat")
             mask = rast[..., 3] > 0
@@ -180,7 +189,7 @@ class PlainStep:
             aux = aux * (H * W / P)
             color = edge_grad(color.contiguous(), rast, pos, tri)
             loss = ((color - tgt) ** 2).sum() / (3 * P)
-            (loss + aux).backward()
-            total = total + loss.detach()
+            total = total + loss + aux
             off += H * W
-        return total
+        total.backward()
+        return total.detach()
