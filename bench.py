@@ -26,7 +26,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 METRIC = "fwd+bwd iters/s @15K tris, 8x800^2 views"
+PROFILE_MD = os.path.join(ROOT, "profiles", "fused_step_sphere_torus_15k_8x800.md")
+# kernel behind each timed stage of ds_train_step, as named in the rocprofv3 summary
+STAGE_KERNEL = {"resolve": "k_resolve<16,true>", "shade": "k_shade<4,8,32,false>",
+                "edge": "k_edge_grad_tiles<256>", "aux": "k_opacity_aux<16>"}
 
 
 def parse():
@@ -37,16 +43,22 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--stage", default="auto", help="kernel stage timed for the roofline")
+    ap.add_argument("--stage", default="shade", choices=["resolve", "shade", "edge", "aux"],
+                    help="kernel stage reported as the roofline (the dominant one by default)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--profile-driver", action="store_true",
                     help="run four fused steps and a 256 MiB calibration kernel (for rocprofv3 --pmc), print nothing")
+    ap.add_argument("--summarize-rocprof", nargs=4, metavar=("KERNEL_STATS_CSV", "FETCH_CSV", "WRITE_CSV", "OUT_MD"),
+                    help="write the markdown summary of a kernel trace and two counter runs, then exit")
     return ap.parse_args()
 
 
 def main():
     args = parse()
+    if args.summarize_rocprof:
+        summarize_rocprof(*args.summarize_rocprof)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -133,18 +145,134 @@ def main():
         if "bf16_mlp_it_s" in out:
             out["bf16_speedup_vs_plain_fp32"] = round(out["bf16_mlp_it_s"] / out["plain_fp32_it_s"], 2)
 
-    # roofline of the dominant kernel, timed with events on the launch stream
-    stage = "resolve" if args.stage == "auto" else args.stage
-    _, _, kms = timed(st32, max(10, args.steps // 2), 2, stage=stage)
-    alg = st32.algorithmic_bytes(stage)
-    achieved = alg / (kms * 1e-3) / 1e9
-    out["roofline"] = {"kernel": stage, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                       "avg_launch_ms": round(kms, 5), "algorithmic_bytes": int(alg)}
+    # per-stage kernel time with HIP events recorded on the launch stream,
+    # and the roofline of each stage's kernel
+    traffic = profiled_traffic()
+    stages = {}
+    for stage in ("resolve", "shade", "edge", "aux"):
+        _, _, kms = timed(st32, max(10, args.steps // 2), 2, stage=stage)
+        stages[stage] = roofline(st32, stage, kms, traffic)
+    out["roofline"] = stages[args.stage]
+    out["stages"] = {k: {"avg_launch_us": round(v["avg_launch_ms"] * 1e3, 1), "bound": v["bound"],
+                         "frac": v["frac"]} for k, v in stages.items()}
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(soup, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def roofline(step, stage, kms, traffic):
+    """Roofline entry of one stage's kernel: shading is compared with the fp32
+    MFMA peak (its arithmetic intensity is above the fp32 ridge of 19.7
+    flop/B), the raster kernels with HBM bandwidth."""
+    kernel = STAGE_KERNEL[stage]
+    tr = traffic.get(kernel)
+    if stage == "shade":
+        flops = step.algorithmic_flops(stage)
+        achieved = flops / (kms * 1e-3) / 1e12
+        peak = MFMA_BF16_PEAK_TFLOPS if step.bf16 else MFMA_FP32_PEAK_TFLOPS
+        r = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+             "algorithmic_flops": int(flops)}
+    else:
+        nbytes = step.algorithmic_bytes(stage)
+        achieved = nbytes / (kms * 1e-3) / 1e9
+        peak = HBM_PEAK_GBS
+        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": peak, "unit": "GB/s",
+             "algorithmic_bytes": int(nbytes)}
+    r["frac"] = round(achieved / peak, 4)
+    r["traffic"] = None if tr is None else int(tr)
+    r["kernel"] = kernel
+    r["avg_launch_ms"] = round(kms, 5)
+    if tr is not None:
+        r["traffic_source"] = os.path.relpath(PROFILE_MD, ROOT) + " (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated)"
+        r["traffic_gbs"] = round(tr / (kms * 1e-3) / 1e9, 1)
+    return r
+
+
+def profiled_traffic():
+    """HBM bytes per launch of each kernel from the committed rocprofv3
+    counter summary (its 'traffic MiB' column); empty if it is absent."""
+    out = {}
+    if not os.path.exists(PROFILE_MD):
+        return out
+    in_table = False
+    for line in open(PROFILE_MD):
+        if line.startswith("| kernel | fetch MiB"):
+            in_table = True
+            continue
+        if in_table:
+            cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+            if len(cells) != 4 or cells[0].startswith("---"):
+                if not line.strip():
+                    break
+                continue
+            out[cells[0]] = float(cells[3]) * 1024 * 1024
+    return out
+
+
+def _kernel_key(name):
+    """Short kernel name: the k_* function plus its template arguments."""
+    import re
+    m = re.search(r"::(k_\w+)", name)
+    if m:
+        k = m.group(1)
+        t = re.search(r"Cfg<([^>]*)>", name) or re.search(r"k_\w+<([^>(]*)>", name)
+        return k + ("<" + t.group(1).replace(" ", "") + ">" if t else "")
+    if "neg" in name:
+        return "calibration torch.neg 256 MiB"
+    return re.sub(r"\(.*", "", name)[:50]
+
+
+def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
+    """Markdown summary of `rocprofv3 --kernel-trace --stats` (per-kernel
+    average duration) and two `--pmc` runs of --profile-driver (FETCH_SIZE,
+    WRITE_SIZE in KiB per dispatch), scaled by what each counter reads for
+    torch.neg over 256 MiB, which moves exactly 256 MiB each way."""
+    import collections
+    import csv
+
+    rows = list(csv.DictReader(open(stats_csv)))
+    res = collections.defaultdict(dict)
+    for ctr, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        agg = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            agg[_kernel_key(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        for k, v in agg.items():
+            res[k][ctr] = sum(v) / len(v)
+    cal = res["calibration torch.neg 256 MiB"]
+    fs, ws = 262144.0 / cal["FETCH_SIZE"], 262144.0 / cal["WRITE_SIZE"]
+    cmd_stats = ("cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+                 "-d $REPO/gpurun_out/prof_bench -o bench -- python3 $REPO/bench.py --steps 20 --warmup 3 "
+                 "--no-cpu-baseline --no-plain")
+    cmd_pmc = "\n".join(
+        f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc {c} --output-format csv -d $REPO/gpurun_out/pmc_{c} "
+        f"-o run -- python3 $REPO/bench.py --profile-driver" for c in ("FETCH_SIZE", "WRITE_SIZE"))
+    cmd_sum = ("python3 bench.py --summarize-rocprof gpurun_out/prof_bench/bench_kernel_stats.csv "
+               "gpurun_out/pmc_FETCH_SIZE/run_counter_collection.csv "
+               "gpurun_out/pmc_WRITE_SIZE/run_counter_collection.csv " + os.path.relpath(PROFILE_MD, ROOT))
+    out = ["# Fused training step: sphere+torus soup, 15K triangles, 8 views 800x800\n",
+           "bench.py runs the fp32-MLP step (`k_shade<4,8,32,false>`) and the bf16-MLP step "
+           "(`k_shade<4,8,32,true>`); every other kernel is shared.  `$REPO` is the repository root.\n",
+           "Kernel trace:\n", "```\n" + cmd_stats + "\n```\n",
+           "| kernel | calls | avg us | min us | max us | % time |", "|---|---:|---:|---:|---:|---:|"]
+    for r in rows[:24]:
+        out.append(f"| `{_kernel_key(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    out += ["\nHBM traffic per launch (two counter runs, one counter each):\n", "```\n" + cmd_pmc + "\n```\n",
+            f"Calibration (torch.neg over 256 MiB reads and writes 262144 KiB): FETCH_SIZE "
+            f"{cal['FETCH_SIZE']:.0f} KiB, WRITE_SIZE {cal['WRITE_SIZE']:.0f} KiB; scale factors fetch "
+            f"x{fs:.3f}, write x{ws:.3f}.\n",
+            "| kernel | fetch MiB | write MiB | traffic MiB |", "|---|---:|---:|---:|"]
+    order = sorted(res, key=lambda k: -(res[k].get("FETCH_SIZE", 0) * fs + res[k].get("WRITE_SIZE", 0) * ws))
+    for k in order[:14]:
+        if k.startswith("calibration") or not k.startswith("k_"):
+            continue
+        f = res[k].get("FETCH_SIZE", 0) * fs / 1024
+        w = res[k].get("WRITE_SIZE", 0) * ws / 1024
+        out.append(f"| `{k}` | {f:.1f} | {w:.1f} | {f + w:.1f} |")
+    out += ["", "Summary written by:", "```\n" + cmd_sum + "\n```"]
+    with open(out_md, "w") as fh:
+        fh.write("\n".join(out) + "\n")
 
 
 def profile_driver(soup, dev):

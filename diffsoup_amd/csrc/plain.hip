@@ -95,33 +95,42 @@ __global__ void k_compact(long long n, const int* __restrict__ frag_pix, const f
     attrs_out[o] = frag_attrs[i];
 }
 
-__global__ void k_alpha(long long n, const float4* __restrict__ attrs, int level, int S,
+// Per-fragment opacity summed over levels lo..hi (feature_dim 1), levels
+// ascending and corners 0, 1, 2 from +0 as in the reference's interpolation
+// (src/cuda/multires.cu:10-68, :130-156).
+__global__ void k_alpha(long long n, const float4* __restrict__ attrs, int lo, int hi, int S,
                         const float* __restrict__ alpha_src, float* __restrict__ out) {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 a = attrs[i];
     const int t = static_cast<int>(a.w) - 1;
     if (t < 0) { out[i] = 0.f; return; }
-    const Corners c = lattice_corners(a.x, a.y, static_cast<uint32_t>(level));
     const float* src = alpha_src + (long long)t * S;
     float v = 0.f;
-    v += c.w[0] * src[c.idx[0]];
-    v += c.w[1] * src[c.idx[1]];
-    v += c.w[2] * src[c.idx[2]];
+    for (int l = lo; l <= hi; ++l) {
+        const Corners c = lattice_corners(a.x, a.y, static_cast<uint32_t>(l));
+        v += c.w[0] * src[c.idx[0]];
+        v += c.w[1] * src[c.idx[1]];
+        v += c.w[2] * src[c.idx[2]];
+        src += feats_at_level(static_cast<uint32_t>(l));
+    }
     out[i] = v;
 }
 
-__global__ void k_alpha_bwd(long long n, const float4* __restrict__ attrs, int level, int S,
+__global__ void k_alpha_bwd(long long n, const float4* __restrict__ attrs, int lo, int hi, int S,
                             float* __restrict__ grad_src, const float* __restrict__ grad) {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 a = attrs[i];
     const int t = static_cast<int>(a.w) - 1;
     if (t < 0) return;
-    const Corners c = lattice_corners(a.x, a.y, static_cast<uint32_t>(level));
     float* d = grad_src + (long long)t * S;
     const float g = grad[i];
-    for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&d[c.idx[k]], g * c.w[k]);
+    for (int l = lo; l <= hi; ++l) {
+        const Corners c = lattice_corners(a.x, a.y, static_cast<uint32_t>(l));
+        for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&d[c.idx[k]], g * c.w[k]);
+        d += feats_at_level(static_cast<uint32_t>(l));
+    }
 }
 
 __global__ void k_thresholds(long long n, const int* __restrict__ frag_pix, const float4* __restrict__ attrs,
@@ -245,23 +254,27 @@ int64_t ds_plain_compact(int64_t num_frags, const int* frag_pix, const float* fr
     return hipGetLastError() == hipSuccess ? total : plain_fail(DS_ERR_HIP, "compact");
 }
 
-int ds_plain_alpha(int64_t n, const float* frag_attrs, int level, const float* alpha_src, float* frag_alpha,
-                   void* stream) {
+int ds_plain_alpha(int64_t n, const float* frag_attrs, int min_level, int max_level, const float* alpha_src,
+                   float* frag_alpha, void* stream) {
     using namespace ds;
+    if (min_level < 0 || max_level < min_level || max_level > 10)
+        return plain_fail(DS_ERR_INVALID, "alpha: levels must satisfy 0 <= min_level <= max_level <= 10");
     if (n <= 0) return DS_OK;
     k_alpha<<<nblk(n), kBS, 0, static_cast<hipStream_t>(stream)>>>(
-        n, reinterpret_cast<const float4*>(frag_attrs), level, static_cast<int>(feats_at_level(level)), alpha_src,
-        frag_alpha);
+        n, reinterpret_cast<const float4*>(frag_attrs), min_level, max_level,
+        static_cast<int>(feats_in_levels(min_level, max_level)), alpha_src, frag_alpha);
     return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "alpha");
 }
 
-int ds_plain_alpha_bwd(int64_t n, const float* frag_attrs, int level, float* grad_alpha_src,
+int ds_plain_alpha_bwd(int64_t n, const float* frag_attrs, int min_level, int max_level, float* grad_alpha_src,
                        const float* grad_frag_alpha, void* stream) {
     using namespace ds;
+    if (min_level < 0 || max_level < min_level || max_level > 10)
+        return plain_fail(DS_ERR_INVALID, "alpha bwd: levels must satisfy 0 <= min_level <= max_level <= 10");
     if (n <= 0) return DS_OK;
     k_alpha_bwd<<<nblk(n), kBS, 0, static_cast<hipStream_t>(stream)>>>(
-        n, reinterpret_cast<const float4*>(frag_attrs), level, static_cast<int>(feats_at_level(level)),
-        grad_alpha_src, grad_frag_alpha);
+        n, reinterpret_cast<const float4*>(frag_attrs), min_level, max_level,
+        static_cast<int>(feats_in_levels(min_level, max_level)), grad_alpha_src, grad_frag_alpha);
     return hipGetLastError() == hipSuccess ? DS_OK : plain_fail(DS_ERR_HIP, "alpha bwd");
 }
 

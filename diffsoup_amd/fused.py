@@ -209,6 +209,21 @@ class FusedStep:
             return 16.0 * P + 4.0 * P + 64.0 * self.B * self.T + 4.0 * pairs + 8.0 * self.T * self.Sa
         raise ValueError(stage)
 
+    def algorithmic_flops(self, stage: str) -> float:
+        """Compulsory floating-point work of one launch of the shading stage.
+
+        Per covered pixel: the MLP forward (din*h + h*h + 3*h multiply-adds),
+        the input adjoint (the same count again) and the weight gradients
+        (the same count a third time), 2 flops per multiply-add; plus the
+        feature gather and its scatter (3 corners x channels per level, each
+        way).  With din = 32, h = 32, 4 levels x 8 channels: 13,248 flops.
+        """
+        if stage != "shade":
+            raise ValueError(stage)
+        din, h = self.nl * self.C, self.hidden
+        per_px = 6.0 * (din * h + h * h + 3 * h) + 4.0 * 3 * din
+        return per_px * self.covered_pixels()
+
     def stats(self) -> torch.Tensor:
         """int64 raster stats of the last step (DS_STAT_* order)."""
         base = self.desc.workspace - self._ws.data_ptr()

@@ -29,8 +29,8 @@ _lib.register("ds_plain_scan_tmp_bytes", [_i64], _sz)
 _lib.register("ds_plain_triangle_rects", [_i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i64)
 _lib.register("ds_plain_fragments", [_i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp])
 _lib.register("ds_plain_compact", [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i64)
-_lib.register("ds_plain_alpha", [_i64, _vp, _i, _vp, _vp, _vp])
-_lib.register("ds_plain_alpha_bwd", [_i64, _vp, _i, _vp, _vp, _vp])
+_lib.register("ds_plain_alpha", [_i64, _vp, _i, _i, _vp, _vp, _vp])
+_lib.register("ds_plain_alpha_bwd", [_i64, _vp, _i, _i, _vp, _vp, _vp])
 _lib.register("ds_plain_thresholds", [_i64, _vp, _vp, _i, _u64, _i, _vp, _vp])
 _lib.register("ds_plain_depth_test", [_i, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
 _lib.register("ds_plain_opacity_aux", [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp])
@@ -72,7 +72,7 @@ def compute_fragments(resolution, pos, tri):
 def _fragment_alpha(frag_attrs, level, alpha_src):
     n = frag_attrs.shape[0]
     out = torch.empty(n, dtype=torch.float32, device=frag_attrs.device)
-    _lib.check(_lib.lib().ds_plain_alpha(n, _lib.ptr(frag_attrs), level, _lib.ptr(alpha_src), _lib.ptr(out),
+    _lib.check(_lib.lib().ds_plain_alpha(n, _lib.ptr(frag_attrs), level, level, _lib.ptr(alpha_src), _lib.ptr(out),
                                          _lib.stream_ptr(frag_attrs.device)), "ds_plain_alpha")
     return out
 
@@ -118,7 +118,7 @@ class _PlainOpacityAuxFn(torch.autograd.Function):
                                           _lib.ptr(fp), _lib.ptr(fa), _lib.ptr(alpha), _lib.ptr(g), s),
                    "ds_plain_opacity_aux")
         ga = torch.zeros_like(a2)
-        _lib.check(L.ds_plain_alpha_bwd(n, _lib.ptr(fa), level, _lib.ptr(ga), _lib.ptr(g), s), "ds_plain_alpha_bwd")
+        _lib.check(L.ds_plain_alpha_bwd(n, _lib.ptr(fa), level, level, _lib.ptr(ga), _lib.ptr(g), s), "ds_plain_alpha_bwd")
         ctx.save_for_backward(ga)
         ctx.weight = 1.0 / color.numel()
         ctx.shape = alpha_src.shape

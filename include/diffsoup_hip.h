@@ -249,14 +249,16 @@ int ds_plain_fragments(int H, int W, int B, int V, const float* pos, int T, cons
 int64_t ds_plain_compact(int64_t num_frags, const int* frag_pix, const float* frag_attrs, int* frag_pix_out,
                          float* frag_attrs_out, int* flags, void* scan_tmp, void* stream);
 
-/* Per-fragment opacity at one level.  Replaces _core.multires_triangle_alpha
+/* Per-fragment opacity, summed over levels min_level..max_level of
+ * alpha_src [T, S] (S = sum of ds_feats_at_level over the range); fragments
+ * with id 0 get 0.  Replaces _core.multires_triangle_alpha
  * (src/main.cpp:174-191). */
-int ds_plain_alpha(int64_t n, const float* frag_attrs, int level, const float* alpha_src, float* frag_alpha,
-                   void* stream);
+int ds_plain_alpha(int64_t n, const float* frag_attrs, int min_level, int max_level, const float* alpha_src,
+                   float* frag_alpha, void* stream);
 
 /* Adjoint of ds_plain_alpha (atomic accumulation).  Replaces
  * _core.backward_multires_triangle_alpha (src/main.cpp:193-210). */
-int ds_plain_alpha_bwd(int64_t n, const float* frag_attrs, int level, float* grad_alpha_src,
+int ds_plain_alpha_bwd(int64_t n, const float* frag_attrs, int min_level, int max_level, float* grad_alpha_src,
                        const float* grad_frag_alpha, void* stream);
 
 /* Per-fragment mask thresholds: the hash of (seed, view_base + view,

@@ -53,3 +53,31 @@ def test_bad_arguments_fail_without_a_gpu(L):
     assert rc == -1 and b"level range" in L.ds_last_error()
     Hbig = _lib.int_array([70000])
     assert L.ds_raster_tile_count(1, Hbig, W, 16) == -1
+
+
+def test_core_dropin_surface_and_type_checks(L):
+    """The drop-in _core module offers every compute function the reference's
+    Python layer binds (src/main.cpp:30-304, remeshing aside) and, like
+    nanobind, raises TypeError on a wrong dtype, rank or layout before any
+    GPU work."""
+    import torch
+
+    from diffsoup_amd import _core
+    ref_names = ["compute_triangle_rects", "compute_fragments", "depth_test", "filter_valid_fragments",
+                 "backward_edge_grad", "backward_opacity_aux_loss", "encode_view_dir_sh2",
+                 "multires_triangle_alpha", "backward_multires_triangle_alpha", "multires_triangle_color",
+                 "backward_multires_triangle_color", "accumulate_to_level_forward", "accumulate_to_level_backward"]
+    assert all(callable(getattr(_core, n)) for n in ref_names)
+    assert _core.__version__ == "0.1.0"
+    pos = torch.zeros(1, 3, 4, dtype=torch.float64)
+    tri = torch.zeros(1, 3, dtype=torch.int32)
+    with pytest.raises(TypeError, match="pos"):
+        _core.compute_triangle_rects(8, 8, pos, tri, torch.zeros(1, 4, dtype=torch.int32),
+                                     torch.zeros(1, dtype=torch.int32))
+    with pytest.raises(TypeError, match="frag_attrs"):
+        _core.multires_triangle_alpha(torch.zeros(4, 4, dtype=torch.float32).t(), 0, 0,
+                                      torch.zeros(1, 3), torch.zeros(4))
+    # a CPU tensor of the right type is refused loudly: there is no CPU path
+    from diffsoup_amd._lib import DiffSoupError
+    with pytest.raises(DiffSoupError, match="GPU tensor"):
+        _core.encode_view_dir_sh2(torch.zeros(1, 2, 2, 4), torch.zeros(1, 4, 4), torch.zeros(1, 2, 2, 9))
