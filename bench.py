@@ -105,10 +105,35 @@ def main():
         step.time_stage(None)
         return ms, wall / steps * 1e3, kms
 
-    st32 = make(False)
-    ms32, wall32, _ = timed(st32, args.steps, args.warmup)
-    stats = st32.stats().cpu().numpy()
-    covered = int(stats[0])
+    def timed_graph(bf16, steps, warmup):
+        """The captured training iteration: forward + backward + Adam/VectorAdam,
+        one graph replay per step.  The optimiser reads and writes every
+        parameter and moment each iteration but stores the updated parameters
+        beside the inputs, so each iteration sees the configured scene (trained
+        in place on a random target, the soup's triangles grow and the workload
+        would drift with the step count)."""
+        step = make(bf16)
+        params = fused.SoupParams.from_soup(soup, dev)
+        updated = fused.SoupParams.from_soup(soup, dev)
+        opt = fused.soup_optimizer(step, params, out=updated)
+        cap = fused.CapturedStep(step, params, mvp, target, opt, seed=1000)
+        for _ in range(warmup):
+            cap.replay()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            cap.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return e0.elapsed_time(e1) / steps, wall / steps * 1e3, step
+
+    ms32, wall32, gstep = timed_graph(False, args.steps, args.warmup)
+    covered = int(gstep.stats().cpu().numpy()[0])
+    del gstep
     out = {
         "metric": METRIC,
         "value": round(1000.0 / ms32, 2),
@@ -124,20 +149,26 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "sphere+torus triangle soup, 15K tris, 8 views 800x800, levels 0-3 x 8ch concat, "
-                        "MLP 32-32-32-3, stochastic mask, MSE, edge + opacity grads",
+                        "MLP 32-32-32-3, stochastic mask with a new seed per iteration, MSE, edge + opacity "
+                        "grads, Adam (features, opacities, MLP) + VectorAdam (vertices) written out of place, "
+                        "one HIP graph per iteration",
             "triangles": soup.T, "views": soup.B, "resolution": list(soup.resolutions[0]),
-            "pixels_per_step": soup.pixels, "parallelism": "single GPU",
+            "pixels_per_step": soup.pixels, "tile": 16, "parallelism": "single GPU",
         },
         "wall_ms_per_step": round(wall32, 4),
         "covered_fragments_per_step": covered,
         "gfrag_per_s": round(covered * 1000.0 / ms32 / 1e9, 4),
     }
     if not args.no_bf16:
-        st16 = make(True)
-        ms16, _, _ = timed(st16, args.steps, args.warmup)
+        ms16, _, g16 = timed_graph(True, args.steps, args.warmup)
+        del g16
         out["bf16_mlp_it_s"] = round(1000.0 / ms16, 2)
         out["bf16_vs_fp32"] = round(ms32 / ms16, 3)
-        del st16
+
+    # forward + backward only (no optimiser, fixed parameters, eager launches)
+    st32 = make(False)
+    msfb, _, _ = timed(st32, args.steps, args.warmup)
+    out["fwd_bwd_only_eager_it_s"] = round(1000.0 / msfb, 2)
 
     if not args.no_plain:
         out["plain_fp32_it_s"] = round(plain_its(soup, prm, mvp, target, args.plain_steps), 3)
@@ -299,10 +330,12 @@ def profile_driver(soup, dev):
 
 def plain_its(soup, prm, mvp, target, steps):
     """The plain per-view FP32 path (fragment lists, global-atomic depth test,
-    torch MLP, one view per launch) on the same workload and inputs."""
+    torch MLP, one view per launch) on the same workload and inputs, with the
+    reference's optimisers (torch Adam, VectorAdam) after each backward."""
     import torch
 
     from diffsoup_amd import fused, plain
+    from diffsoup_amd.optimize import VectorAdam
 
     leaves = fused.SoupParams(prm.verts.clone().requires_grad_(True), prm.tri,
                               prm.alpha_logits.clone().requires_grad_(True), prm.feat.clone().requires_grad_(True),
@@ -315,11 +348,16 @@ def plain_its(soup, prm, mvp, target, steps):
             lin.bias.copy_(b)
         layers.append(lin)
     ps = plain.PlainStep(leaves, layers, soup.levels, soup.alpha_level)
+    opt_soup = torch.optim.Adam([{"params": [leaves.feat], "lr": 5e-2}, {"params": [leaves.alpha_logits], "lr": 5e-2}])
+    opt_vert = VectorAdam([leaves.verts], lr=1e-3)
+    opt_mlp = torch.optim.Adam([p for l in layers for p in l.parameters()], lr=1e-2)
 
     def one(i):
-        for t in [leaves.verts, leaves.alpha_logits, leaves.feat] + [p for l in layers for p in l.parameters()]:
-            t.grad = None
+        for o in (opt_soup, opt_vert, opt_mlp):
+            o.zero_grad(set_to_none=True)
         ps.run(mvp, target, soup.resolutions, seed=3000 + i)
+        for o in (opt_soup, opt_vert, opt_mlp):
+            o.step()
 
     for i in range(2):
         one(i)
@@ -332,20 +370,43 @@ def plain_its(soup, prm, mvp, target, steps):
 
 
 def cpu_baseline(soup, budget_s):
-    """The CPU oracle's full step (C raster / gather / edge / opacity, torch
-    CPU MLP in fp32) on a bounded sample of iterations, one thread."""
+    """The CPU oracle's full iteration on a bounded sample, one thread: C
+    raster / gather / edge / opacity, torch CPU fp32 MLP, then torch Adam and
+    VectorAdam updates of every parameter from the gradients it produced."""
+    import numpy as np
     import torch
 
+    from diffsoup_amd.optimize import VectorAdam
     from oracle import oracle
 
     torch.set_num_threads(1)
+    t = lambda a: torch.tensor(np.asarray(a, np.float32))
+    feat, alog, verts = t(soup.feat), t(soup.alpha_logits), t(soup.verts)
+    mlp = [t(x) for wb in soup.mlp for x in wb]
+    opts = [torch.optim.Adam([{"params": [feat], "lr": 5e-2}, {"params": [alog], "lr": 5e-2}], foreach=False),
+            VectorAdam([verts], lr=1e-3), torch.optim.Adam(mlp, lr=1e-2, foreach=False)]
     n, t0 = 0, time.perf_counter()
     while n < 1 or (time.perf_counter() - t0 < budget_s and n < 20):
-        oracle.reference_step(soup, 2000 + n, dtype=torch.float32)
+        r = oracle.reference_step(soup, 2000 + n, dtype=torch.float32)
+        feat.grad, alog.grad, verts.grad = t(r["grad_feat"]), t(r["grad_alpha_logits"]), t(r["grad_verts"])
+        g, o = r["grad_mlp"], 0
+        flat = {}
+        for i, (w, b) in enumerate(soup.mlp):
+            flat[2 * i] = (o, w.shape)
+            o += w.size
+        for i, (w, b) in enumerate(soup.mlp):
+            flat[2 * i + 1] = (o, b.shape)
+            o += b.size
+        for k, p in enumerate(mlp):
+            off, shape = flat[k]
+            p.grad = t(g[off:off + int(np.prod(shape))].reshape(shape))
+        for opt in opts:
+            opt.step()
         n += 1
     dt = (time.perf_counter() - t0) / n
     return {"value": round(1.0 / dt, 4), "unit": "it/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full iterations of the same workload (single-threaded C oracle + torch CPU fp32 MLP)"}
+            "sample": f"{n} full iterations of the same workload (single-threaded C oracle forward + backward, "
+                      f"torch CPU fp32 MLP, torch Adam + VectorAdam updates)"}
 
 
 if __name__ == "__main__":

@@ -84,6 +84,7 @@ namespace ds {
 int step_make_views(const ds_step_desc* d, Views& v) { return make_views(d->B, d->H, d->W, d->tile, v); }
 int step_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 int plain_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+int optim_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 }  // namespace ds
 
 extern "C" {
@@ -150,7 +151,7 @@ int ds_rasterize_fwd(int B, const int* H, const int* W, int V, const float* pos,
     if (rc != DS_OK) return rc;
     ds::AlphaSource a{alpha_src, level, static_cast<int>(ds::feats_at_level(level))};
     return hip_status(ds::launch_raster_resolve(v, T, a, stochastic ? ds::kMaskHash : ds::kMaskDeterministic,
-                                                seed, rast, w, s),
+                                                ds::MaskSeed{seed, nullptr}, rast, w, s),
                       "raster resolve");
 }
 

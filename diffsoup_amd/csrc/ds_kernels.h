@@ -64,10 +64,18 @@ struct AlphaSource {
     int S;
 };
 
+// Seed of the stochastic mask: a value, or (when device is set) the uint64
+// in device memory that the kernel reads when it runs, so that a captured
+// graph picks up a new seed on every replay.
+struct MaskSeed {
+    unsigned long long value;
+    const unsigned long long* device;
+};
+
 hipError_t launch_raster_bin(const Views& v, int V, const float* pos, int T, const int* tri,
                              const RasterWorkspace& w, hipStream_t s);
 hipError_t launch_raster_resolve(const Views& v, int T, const AlphaSource& a, int mask_mode,
-                                 uint64_t seed, float* rast, const RasterWorkspace& w, hipStream_t s);
+                                 MaskSeed seed, float* rast, const RasterWorkspace& w, hipStream_t s);
 
 // Training-step variant of the resolve: background pixels get their colour,
 // MSE gradient and L1 written here, covered pixels are appended to a list.
@@ -84,7 +92,7 @@ struct BackgroundShade {
     float background;      // constant background colour
 };
 hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource& a, int mask_mode,
-                                       uint64_t seed, float* rast, const RasterWorkspace& w,
+                                       MaskSeed seed, float* rast, const RasterWorkspace& w,
                                        const BackgroundShade& bg, hipStream_t s);
 hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const float* rast,
                               const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,

@@ -1,0 +1,140 @@
+// Fused optimiser step over every parameter tensor of a training step:
+// torch.optim.Adam's update for element-wise groups and VectorAdam's
+// (second moment pooled over the last dimension) for vector groups, in one
+// launch, with the step counter kept in device memory so that a captured
+// graph advances it on every replay.
+//
+// Reference: python/diffsoup/optimize.py:33-68 (VectorAdam), and the Adam
+// groups of examples/02_synthetic.py:310-315 (torch.optim.Adam defaults:
+// betas (0.9, 0.999), eps 1e-8, no weight decay).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "diffsoup_hip.h"
+
+namespace ds {
+namespace {
+
+constexpr int kMaxGroups = 16;
+constexpr int kThreads = 256;
+constexpr int kUnitsPerBlock = 4096;
+
+struct Group {
+    const float* p;
+    float* out;       // updated parameter (== p for an in-place update)
+    const float* g;
+    float* m;
+    float* v;
+    long long units;  // elements (Adam) or vectors (VectorAdam)
+    int dim;          // 0: Adam; k > 0: VectorAdam over vectors of k floats
+    double lr, b1, b2;     // bias corrections are formed in double
+    float b1f, b2f, w1, w2, eps, lrf;  // fp32 forms: beta, 1 - beta (rounded from double), eps, lr
+    long long block_base;  // first block of this group
+};
+
+struct Table {
+    Group grp[kMaxGroups];
+    int n;
+};
+
+__global__ void __launch_bounds__(kThreads) k_adam(Table tab, const long long* __restrict__ step_before) {
+    int gi = 0;
+    while (gi + 1 < tab.n && (long long)blockIdx.x >= tab.grp[gi + 1].block_base) ++gi;
+    const Group& G = tab.grp[gi];
+    const double t = static_cast<double>(*step_before + 1);
+    // bias corrections in double, as torch evaluates beta ** step on the host
+    const double bc1 = 1.0 - pow(static_cast<double>(G.b1), t);
+    const double bc2 = 1.0 - pow(static_cast<double>(G.b2), t);
+    const long long u0 = ((long long)blockIdx.x - G.block_base) * kUnitsPerBlock;
+    const long long u1 = min(u0 + kUnitsPerBlock, G.units);
+    if (G.dim == 0) {
+        // torch.optim.Adam (single-tensor form): m.lerp_(g, 1 - b1);
+        // v = v * b2 + (1 - b2) g g; p -= (lr / bc1) * m / (sqrt(v) / sqrt(bc2) + eps)
+        const float step_size = static_cast<float>(G.lr / bc1);
+        const float bc2_sqrt = static_cast<float>(sqrt(bc2));
+        for (long long i = u0 + threadIdx.x; i < u1; i += kThreads) {
+            const float g = G.g[i];
+            float m = G.m[i];
+            m = m + G.w1 * (g - m);
+            float v = G.v[i] * G.b2f;
+            v = v + G.w2 * (g * g);
+            G.m[i] = m;
+            G.v[i] = v;
+            const float denom = sqrtf(v) / bc2_sqrt + G.eps;
+            G.out[i] = G.p[i] - step_size * (m / denom);
+        }
+    } else {
+        // VectorAdam: g1 = b1 g1 + (1 - b1) g; g2 = b2 g2 + (1 - b2) |g|^2;
+        // p -= lr * (g1 / bc1) / (sqrt(g2 / bc2) + 1e-8)
+        const float ibc1 = static_cast<float>(bc1), ibc2 = static_cast<float>(bc2);
+        const int k = G.dim;
+        for (long long u = u0 + threadIdx.x; u < u1; u += kThreads) {
+            const long long base = u * k;
+            float sq = 0.f;
+            for (int j = 0; j < k; ++j) {
+                const float g = G.g[base + j];
+                sq += g * g;
+            }
+            const float g2 = G.v[u] * G.b2f + G.w2 * sq;
+            G.v[u] = g2;
+            const float den = sqrtf(g2 / ibc2) + G.eps;
+            for (int j = 0; j < k; ++j) {
+                const float g = G.g[base + j];
+                const float g1 = G.m[base + j] * G.b1f + G.w1 * g;
+                G.m[base + j] = g1;
+                G.out[base + j] = G.p[base + j] - G.lrf * ((g1 / ibc1) / den);
+            }
+        }
+    }
+}
+
+__global__ void k_tick(long long* step) {
+    if (threadIdx.x == 0) *step += 1;
+}
+
+}  // namespace
+
+int optim_fail(int code, const char* msg);
+
+}  // namespace ds
+
+extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream) {
+    using namespace ds;
+    if (n_groups < 1 || n_groups > kMaxGroups)
+        return optim_fail(DS_ERR_INVALID, "adam: 1..16 parameter groups");
+    if (!step) return optim_fail(DS_ERR_INVALID, "adam: step counter is null");
+    Table tab{};
+    tab.n = n_groups;
+    long long blocks = 0;
+    for (int i = 0; i < n_groups; ++i) {
+        const ds_adam_group& s = groups[i];
+        if (s.n < 0 || s.vector_dim < 0 || (s.vector_dim > 0 && s.n % s.vector_dim) ||
+            (s.n > 0 && (!s.param || !s.grad || !s.exp_avg || !s.exp_avg_sq)))
+            return optim_fail(DS_ERR_INVALID, "adam: bad group (size, vector width or null pointer)");
+        Group& G = tab.grp[i];
+        G.p = s.param;
+        G.out = s.param_out ? s.param_out : s.param;
+        G.g = s.grad;
+        G.m = s.exp_avg;
+        G.v = s.exp_avg_sq;
+        G.dim = s.vector_dim;
+        G.units = s.vector_dim ? s.n / s.vector_dim : s.n;
+        G.lr = s.lr;
+        G.b1 = s.beta1;
+        G.b2 = s.beta2;
+        G.b1f = static_cast<float>(s.beta1);
+        G.b2f = static_cast<float>(s.beta2);
+        G.w1 = static_cast<float>(1.0 - s.beta1);
+        G.w2 = static_cast<float>(1.0 - s.beta2);
+        G.eps = static_cast<float>(s.eps);
+        G.lrf = static_cast<float>(s.lr);
+        G.block_base = blocks;
+        blocks += (G.units + kUnitsPerBlock - 1) / kUnitsPerBlock;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    long long* t = reinterpret_cast<long long*>(step);
+    if (blocks > 0) k_adam<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(tab, t);
+    k_tick<<<1, 64, 0, st>>>(t);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DS_OK : optim_fail(DS_ERR_HIP, hipGetErrorString(e));
+}

@@ -247,7 +247,7 @@ __device__ __forceinline__ long long wave_sum_ll(long long x) {
 
 template <int TS, bool SHADE>
 __global__ void __launch_bounds__(TS * TS)
-k_resolve(Views v, int T, AlphaSource a, int mask_mode, unsigned long long seed,
+k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
           const TriRecord* __restrict__ recs, const int* __restrict__ tile_off,
           const int* __restrict__ bins, long long* __restrict__ stats, float4* __restrict__ rast,
           int* __restrict__ tile_cov, BackgroundShade bg) {
@@ -258,6 +258,7 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, unsigned long long seed,
     __shared__ int wave_cnt[NW];
     __shared__ int wave_cov[NW];
     const TileWalk w = tile_walk<TS>(v, T, tile_off, stats);
+    const unsigned long long seed = mseed.device ? *mseed.device : mseed.value;
 
     long long best = kKeyEmpty;
     float bb0 = 0.f, bb1 = 0.f, bd = 0.f;
@@ -524,7 +525,7 @@ hipError_t launch_raster_bin(const Views& v, int V, const float* pos, int T, con
 
 template <bool SHADE>
 static hipError_t resolve_dispatch(const Views& v, int T, const AlphaSource& a, int mask_mode,
-                                   uint64_t seed, float* rast, const RasterWorkspace& w,
+                                   MaskSeed seed, float* rast, const RasterWorkspace& w,
                                    const BackgroundShade& bg, hipStream_t s) {
     const int n_tiles = v.tile_base[v.n];
     if (n_tiles == 0) return hipSuccess;
@@ -550,12 +551,12 @@ static hipError_t resolve_dispatch(const Views& v, int T, const AlphaSource& a, 
 }
 
 hipError_t launch_raster_resolve(const Views& v, int T, const AlphaSource& a, int mask_mode,
-                                 uint64_t seed, float* rast, const RasterWorkspace& w, hipStream_t s) {
+                                 MaskSeed seed, float* rast, const RasterWorkspace& w, hipStream_t s) {
     return resolve_dispatch<false>(v, T, a, mask_mode, seed, rast, w, BackgroundShade{}, s);
 }
 
 hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource& a, int mask_mode,
-                                       uint64_t seed, float* rast, const RasterWorkspace& w,
+                                       MaskSeed seed, float* rast, const RasterWorkspace& w,
                                        const BackgroundShade& bg, hipStream_t s) {
     return resolve_dispatch<true>(v, T, a, mask_mode, seed, rast, w, bg, s);
 }

@@ -240,7 +240,8 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, bgloss, tile_list, tile_ncov, tri_count,
                        2.0f * inv_n, d->background};
     DS_TRY(mark(DS_STAGE_RESOLVE, true), "event");
-    DS_TRY(launch_raster_resolve_shade(v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, d->seed,
+    const MaskSeed seed{d->seed, reinterpret_cast<const unsigned long long*>(d->seed_device)};
+    DS_TRY(launch_raster_resolve_shade(v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, seed,
                                        d->rast, rw, bg, s),
            "resolve");
     DS_TRY(mark(DS_STAGE_RESOLVE, false), "event");

@@ -10,7 +10,7 @@ and the world-space vertex gradient.  Views may differ in resolution.
 
 All buffers are allocated once by ``FusedStep`` and reused, so a step
 allocates nothing and never synchronises the host: it can be replayed from a
-captured HIP graph (``FusedStep.capture``).
+captured HIP graph (``CapturedStep``).
 """
 from __future__ import annotations
 
@@ -44,6 +44,7 @@ class StepDesc(ctypes.Structure):
         ("grad_feat", ctypes.c_void_p), ("grad_mlp", ctypes.c_void_p), ("loss", ctypes.c_void_p),
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("capacity", ctypes.c_int64),
         ("timed_stage", ctypes.c_int), ("ev_begin", ctypes.c_void_p), ("ev_end", ctypes.c_void_p),
+        ("seed_device", ctypes.c_void_p),
     ]
 
 
@@ -257,3 +258,64 @@ def smoke_step(dev) -> None:
     if err > 1e-5 or abs(float(loss.item()) - loss64) > 1e-5 * max(1.0, loss64):
         raise AssertionError(f"smoke: fused colour/loss mismatch (max err {err})")
     print(f"smoke: fused step ok (loss {loss.item():.6f}, colour max err vs fp64 {err:.2e})", flush=True)
+
+
+def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5e-2, lr_verts=1e-3,
+                   lr_mlp=1e-2, out: SoupParams | None = None):
+    """The reference's optimisers for a soup (examples/02_synthetic.py:310-315):
+    Adam on features and opacity logits (lr 5e-2) and on the MLP (1e-2),
+    VectorAdam on vertex positions (1e-3), reading the gradients that
+    ``step`` writes.  One FusedAdam, so one launch per iteration.  With
+    ``out`` the updated parameters are written there and ``params`` stays
+    unchanged (every iteration then sees the same scene, as a throughput
+    measurement of a fixed configuration needs)."""
+    from .optimize import FusedAdam
+
+    o = out
+    groups = [dict(param=params.feat, grad=step.grad_feat, lr=lr_feat, param_out=o and o.feat),
+              dict(param=params.alpha_logits, grad=step.grad_alpha_logits, lr=lr_alpha,
+                   param_out=o and o.alpha_logits),
+              dict(param=params.verts, grad=step.grad_verts, lr=lr_verts, vector_dim=3, param_out=o and o.verts)]
+    dW1, db1, dW2, db2, dW3, db3 = step.mlp_grads()
+    for name, g in (("w1", dW1), ("b1", db1), ("w2", dW2), ("b2", db2), ("w3", dW3), ("b3", db3)):
+        groups.append(dict(param=getattr(params, name), grad=g, lr=lr_mlp, param_out=o and getattr(o, name)))
+    return FusedAdam(groups)
+
+
+class CapturedStep:
+    """One training iteration captured in a HIP graph: ds_train_step with the
+    mask seed read from device memory, the optimiser step, and a seed
+    increment.  Replay i after ``set_seed(s)`` uses seed s + i.
+
+    The graph owns the kernel arguments of the capture: the tensors in
+    ``params``, ``mvp``, ``target`` and the step's buffers must stay alive
+    and in place (update them with copy_, not by rebinding).
+    """
+
+    def __init__(self, step: FusedStep, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor,
+                 optimizer=None, seed: int = 0):
+        self.step = step
+        self.seed_dev = torch.zeros(1, dtype=torch.int64, device=step.dev)
+        # one eager step first: it loads the code objects and sets kernel
+        # attributes, which must not happen during capture; it writes only
+        # the step's own outputs, not the parameters
+        step.run(params, mvp, target, seed)
+        torch.cuda.synchronize(step.dev)
+        step.desc.seed_device = self.seed_dev.data_ptr()
+        self.graph = torch.cuda.CUDAGraph()
+        try:
+            with torch.cuda.graph(self.graph):
+                step.run(params, mvp, target, 0)
+                if optimizer is not None:
+                    optimizer.step()
+                self.seed_dev.add_(1)
+        finally:
+            step.desc.seed_device = None
+        self.set_seed(seed)
+
+    def set_seed(self, seed: int) -> None:
+        self.seed_dev.fill_(int(seed) & (2**63 - 1))
+
+    def replay(self) -> torch.Tensor:
+        self.graph.replay()
+        return self.step.loss

@@ -1,14 +1,22 @@
-"""VectorAdam: Adam with the second moment pooled over the last dimension
-(python/diffsoup/optimize.py:7-68; Ling, Sharp & Jacobson, NeurIPS 2022).
+"""Optimisers.
 
-Same update as the reference:
+VectorAdam is the reference's torch optimiser (python/diffsoup/optimize.py:7-68;
+Ling, Sharp & Jacobson, NeurIPS 2022): Adam with the second moment pooled
+over the last dimension.  FusedAdam applies torch.optim.Adam's update and
+VectorAdam's to every parameter group of a training step in one HIP launch.
+
+VectorAdam's update, as in the reference:
     g1 <- b1 g1 + (1 - b1) g
     g2 <- b2 g2 + (1 - b2) sum_last(g^2)
     p  <- p - lr * (g1 / (1 - b1^t)) / (sqrt(g2 / (1 - b2^t)) + 1e-8)
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
+
+from . import _lib
 
 
 class VectorAdam(torch.optim.Optimizer):
@@ -37,3 +45,80 @@ class VectorAdam(torch.optim.Optimizer):
                 m1 = g1 / (1 - b1**t)
                 m2 = g2 / (1 - b2**t)
                 p.sub_(m1 / (m2.sqrt() + 1e-8), alpha=lr)
+
+
+class AdamGroup(ctypes.Structure):
+    """ds_adam_group of include/diffsoup_hip.h."""
+    _fields_ = [("param", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
+                ("exp_avg_sq", ctypes.c_void_p), ("n", ctypes.c_int64), ("vector_dim", ctypes.c_int),
+                ("lr", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
+                ("eps", ctypes.c_double), ("param_out", ctypes.c_void_p)]
+
+
+_lib.register("ds_adam_step", [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+
+
+class FusedAdam:
+    """Adam and VectorAdam over several (parameter, gradient) pairs in one
+    HIP launch (ds_adam_step).
+
+    ``groups`` is a list of dicts with keys ``param`` and ``grad`` (fp32 GPU
+    tensors of the same shape, contiguous), and optionally ``lr`` (1e-3),
+    ``betas`` ((0.9, 0.999)), ``eps`` (1e-8) and ``vector_dim`` (0 for Adam;
+    k > 0 pools the second moment over the last dimension of width k, as
+    VectorAdam does) and ``param_out`` (None: update in place; else a tensor
+    of the parameter's shape that receives the updated values while
+    ``param`` stays as it is).  Gradients are read from ``grad`` (for example
+    the buffers a fused training step writes), not from ``param.grad``.  The step
+    counter lives in device memory, so a captured graph advances it.
+    Hyper-parameters are baked into each launch: change them with
+    ``set_lr`` and re-capture any graph that contains the step.
+    """
+
+    def __init__(self, groups):
+        if not 1 <= len(groups) <= 16:
+            raise ValueError("FusedAdam: 1..16 parameter groups")
+        self.groups = []
+        for g in groups:
+            p, gr = g["param"], g["grad"]
+            k = int(g.get("vector_dim", 0))
+            for t, name in ((p, "param"), (gr, "grad")):
+                _lib.require_device(t, name)
+                if t.dtype != torch.float32 or not t.is_contiguous():
+                    raise TypeError(f"FusedAdam: {name} must be a contiguous fp32 tensor")
+            if p.shape != gr.shape:
+                raise ValueError("FusedAdam: param and grad shapes differ")
+            if k and p.shape[-1] != k:
+                raise ValueError("FusedAdam: vector_dim must equal the last dimension")
+            po = g.get("param_out")
+            if po is not None:
+                _lib.require_device(po, "param_out")
+                if po.dtype != torch.float32 or not po.is_contiguous() or po.shape != p.shape:
+                    raise TypeError("FusedAdam: param_out must be a contiguous fp32 tensor of the parameter's shape")
+            m = torch.zeros_like(p)
+            v = torch.zeros(p.numel() // k if k else p.numel(), dtype=torch.float32, device=p.device)
+            b1, b2 = g.get("betas", (0.9, 0.999))
+            self.groups.append(dict(param=p, grad=gr, param_out=po, exp_avg=m, exp_avg_sq=v, vector_dim=k,
+                                    lr=float(g.get("lr", 1e-3)), betas=(float(b1), float(b2)),
+                                    eps=float(g.get("eps", 1e-8))))
+        self.device = self.groups[0]["param"].device
+        self.step_count = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self._build()
+
+    def _build(self):
+        arr = (AdamGroup * len(self.groups))()
+        for a, g in zip(arr, self.groups):
+            a.param, a.grad = g["param"].data_ptr(), g["grad"].data_ptr()
+            a.exp_avg, a.exp_avg_sq = g["exp_avg"].data_ptr(), g["exp_avg_sq"].data_ptr()
+            a.n, a.vector_dim = g["param"].numel(), g["vector_dim"]
+            a.lr, (a.beta1, a.beta2), a.eps = g["lr"], g["betas"], g["eps"]
+            a.param_out = None if g["param_out"] is None else g["param_out"].data_ptr()
+        self._arr = arr
+
+    def set_lr(self, index, lr):
+        self.groups[index]["lr"] = float(lr)
+        self._build()
+
+    def step(self):
+        _lib.check(_lib.lib().ds_adam_step(self._arr, len(self.groups), _lib.ptr(self.step_count),
+                                           _lib.stream_ptr(self.device)), "ds_adam_step")

@@ -196,6 +196,10 @@ typedef struct ds_step_desc {
     int timed_stage;          /* DS_STAGE_* bracketed by the events, or 0    */
     void* ev_begin;           /* hipEvent_t recorded before that stage       */
     void* ev_end;             /* hipEvent_t recorded after it                */
+    const uint64_t* seed_device; /* if non-null, the mask seed is read from this
+                                    device uint64 when the step runs (seed is
+                                    then ignored): a captured graph replays with
+                                    whatever seed the caller last stored there */
 } ds_step_desc;
 
 /* Stages of ds_train_step that timed_stage can select. */
@@ -277,6 +281,34 @@ int ds_plain_depth_test(int B, int H, int W, int64_t n, const int* frag_pix, con
 int ds_plain_opacity_aux(int B, int H, int W, int C, const float* color, const float* target, const float* rast,
                          int64_t n, const int* frag_pix, const float* frag_attrs, const float* frag_alpha,
                          float* grad_frag_alpha, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Fused optimiser step.  One launch updates every parameter group: groups
+ * with vector_dim == 0 get torch.optim.Adam's update (element-wise second
+ * moment; the optimiser of the features, opacities and MLP in
+ * examples/02_synthetic.py:310-315), groups with vector_dim == k > 0 get
+ * VectorAdam's (python/diffsoup/optimize.py:33-68: second moment pooled over
+ * the last dimension of width k; eps is the reference's 1e-8).  *step is a
+ * device int64 holding the number of steps taken so far; the call uses
+ * *step + 1 for the bias corrections and then increments it, so a captured
+ * graph advances it on every replay.  exp_avg has the parameter's shape,
+ * exp_avg_sq has n / k entries for vector groups.  Replaces
+ * torch.optim.Adam.step and VectorAdam.step.
+ * ------------------------------------------------------------------------ */
+typedef struct ds_adam_group {
+    float* param;
+    const float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    int64_t n;        /* floats in param                                    */
+    int vector_dim;   /* 0: Adam; k > 0: VectorAdam over vectors of k floats */
+    double lr, beta1, beta2, eps;
+    float* param_out; /* NULL: update param in place; else write the updated
+                         values here and leave param unchanged            */
+} ds_adam_group;
+
+/* At most 16 groups. */
+int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream);
 
 /* Fill n floats with value (a kernel, so it is safe inside graph capture). */
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);

@@ -338,6 +338,6 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
     gmlp = np.concatenate([Ws[0][0].grad.numpy().ravel(), Ws[1][0].grad.numpy().ravel(),
                            Ws[2][0].grad.numpy().ravel(), Ws[0][1].grad.numpy(), Ws[1][1].grad.numpy(),
                            Ws[2][1].grad.numpy()])
-    return dict(rast=rast, color=color.detach().numpy(), grad_color=grad_color32, loss=float(loss),
+    return dict(rast=rast, color=color.detach().numpy(), grad_color=grad_color32, loss=float(loss.detach()),
                 grad_feat=grad_feat, grad_verts=gverts, grad_alpha_logits=galog, grad_mlp=gmlp,
                 covered=covered, pos=pos, alpha=alpha)
