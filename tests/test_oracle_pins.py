@@ -291,3 +291,61 @@ def test_sh2_orthonormal_basis_and_axis_direction():
     X, Yv, Z = eye / np.linalg.norm(eye)
     np.testing.assert_allclose(enc[[1, 2, 3]], [-c1 * Yv, c1 * Z, -c1 * X], atol=1e-5)
     assert O.sh2(np.zeros((1, 1, 1, 4), np.float32), inv).sum() == 0
+
+
+# --------------------------------------------------------------------------
+# Opacity auxiliary gradient: unbiasedness by enumerating the masks
+# --------------------------------------------------------------------------
+
+def _depth_test_with(fp, fa, alpha, keep_by_tri, B, H, W):
+    """Depth test with thresholds forcing each triangle's fragments kept
+    (threshold 0) or dropped (threshold 2 > any opacity)."""
+    thr = np.where(keep_by_tri[fa[:, 3].astype(int) - 1], 0.0, 2.0).astype(np.float32)
+    rast = np.zeros((B, H, W, 4), np.float32)
+    O.lib().dso_depth_test(B, H, W, fp.shape[0], np.ascontiguousarray(fp, np.int32),
+                           np.ascontiguousarray(fa, np.float32), np.ascontiguousarray(alpha, np.float32), thr, rast)
+    return rast
+
+
+def test_opacity_aux_estimator_is_unbiased():
+    """Two overlapping triangles with constant opacities a0 (front) and a1
+    (back), flat colours and a white background.  Over the four keep/drop
+    outcomes, E[L] = sum_o P(o) L(o) with L the per-pixel L1 summed over
+    pixels is multilinear in (a0, a1), and the score-function estimator of
+    python/diffsoup/rasterize.py:139-196 must average to dE[L]/da exactly.
+    A per-triangle constant opacity spreads over its 3 level-0 lattice values
+    with weights summing to 1, so the estimator summed over a triangle's
+    lattice values is compared."""
+    H, W = 12, 16
+    pos = np.array([[[-0.8, -0.7, -0.2, 1.0], [0.6, -0.5, -0.2, 1.0], [-0.2, 0.8, -0.2, 1.0],
+                     [-0.4, -0.9, 0.3, 1.0], [0.9, 0.2, 0.3, 1.0], [-0.7, 0.6, 0.3, 1.0]]], np.float32)
+    tri = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    rng = np.random.default_rng(5)
+    target = rng.uniform(0, 1, size=(1, H, W, 3)).astype(np.float32)
+    flat = np.array([[0.9, 0.1, 0.2], [0.1, 0.7, 0.3]], np.float32)
+    fp, fa = O.fragments(H, W, pos, tri)
+    assert fp.shape[0] > 50
+    for a0, a1 in ((0.3, 0.6), (0.8, 0.25)):
+        src = np.array([[a0] * 3, [a1] * 3], np.float32)
+        alpha = O.multires_alpha(fa, 0, src)
+
+        def outcome(k0, k1):
+            rast = _depth_test_with(fp, fa, alpha, np.array([k0, k1], bool), 1, H, W)
+            ids = rast[..., 3].astype(int)
+            color = np.where(ids[..., None] > 0, flat[np.maximum(ids - 1, 0)], 1.0).astype(np.float32)
+            L = float(np.abs(color.astype(np.float64) - target).sum())
+            est = O.opacity_aux_grad(color, target, rast, pos, tri, 0, src).astype(np.float64).sum(axis=1)
+            return L, est
+
+        outs = {(k0, k1): outcome(k0, k1) for k0 in (0, 1) for k1 in (0, 1)}
+
+        def expected_loss(b0, b1):
+            p = lambda k, a: a if k else 1.0 - a
+            return sum(p(k0, b0) * p(k1, b1) * outs[(k0, k1)][0] for (k0, k1) in outs)
+
+        mean_est = sum((a0 if k0 else 1 - a0) * (a1 if k1 else 1 - a1) * outs[(k0, k1)][1] for (k0, k1) in outs)
+        h = 1e-3   # E[L] is linear in each opacity, so central differences are exact up to rounding
+        d0 = (expected_loss(a0 + h, a1) - expected_loss(a0 - h, a1)) / (2 * h)
+        d1 = (expected_loss(a0, a1 + h) - expected_loss(a0, a1 - h)) / (2 * h)
+        np.testing.assert_allclose(mean_est, [d0, d1], rtol=1e-4)
+        assert abs(d0) > 1.0 and abs(d1) > 1.0
