@@ -30,9 +30,13 @@ MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 METRIC = "fwd+bwd iters/s @15K tris, 8x800^2 views"
 PROFILE_MD = os.path.join(ROOT, "profiles", "fused_step_sphere_torus_15k_8x800.md")
-# kernel behind each timed stage of ds_train_step, as named in the rocprofv3 summary
-STAGE_KERNEL = {"resolve": "k_resolve<16,true>", "shade": "k_shade<4,8,32,false>",
-                "edge": "k_edge_grad_tiles<256>", "aux": "k_opacity_aux<16>"}
+
+
+def stage_kernel(stage, tile):
+    """Kernel behind a timed stage of ds_train_step, as named in the rocprofv3
+    summary (template arguments: tile edge or threads per tile)."""
+    return {"resolve": f"k_resolve<{tile},true>", "shade": "k_shade<4,8,32,false>",
+            "edge": f"k_edge_grad_tiles<{tile * tile}>", "aux": f"k_opacity_aux<{tile}>"}[stage]
 
 
 def parse():
@@ -115,6 +119,7 @@ def main():
         step = make(bf16)
         params = fused.SoupParams.from_soup(soup, dev)
         updated = fused.SoupParams.from_soup(soup, dev)
+        tune = step.autotune(params, mvp, target, tiles=(8, 16, 32), reps=5, cache=None)
         opt = fused.soup_optimizer(step, params, out=updated)
         cap = fused.CapturedStep(step, params, mvp, target, opt, seed=1000)
         for _ in range(warmup):
@@ -129,10 +134,11 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        return e0.elapsed_time(e1) / steps, wall / steps * 1e3, step
+        return e0.elapsed_time(e1) / steps, wall / steps * 1e3, step, tune
 
-    ms32, wall32, gstep = timed_graph(False, args.steps, args.warmup)
+    ms32, wall32, gstep, tune32 = timed_graph(False, args.steps, args.warmup)
     covered = int(gstep.stats().cpu().numpy()[0])
+    tile32 = gstep.tile
     del gstep
     out = {
         "metric": METRIC,
@@ -153,20 +159,22 @@ def main():
                         "grads, Adam (features, opacities, MLP) + VectorAdam (vertices) written out of place, "
                         "one HIP graph per iteration",
             "triangles": soup.T, "views": soup.B, "resolution": list(soup.resolutions[0]),
-            "pixels_per_step": soup.pixels, "tile": 16, "parallelism": "single GPU",
+            "pixels_per_step": soup.pixels, "tile": tile32, "parallelism": "single GPU",
         },
+        "autotune_ms_per_step": {str(k): (None if v is None else round(v, 4)) for k, v in tune32.items()},
         "wall_ms_per_step": round(wall32, 4),
         "covered_fragments_per_step": covered,
         "gfrag_per_s": round(covered * 1000.0 / ms32 / 1e9, 4),
     }
     if not args.no_bf16:
-        ms16, _, g16 = timed_graph(True, args.steps, args.warmup)
+        ms16, _, g16, _ = timed_graph(True, args.steps, args.warmup)
         del g16
         out["bf16_mlp_it_s"] = round(1000.0 / ms16, 2)
         out["bf16_vs_fp32"] = round(ms32 / ms16, 3)
 
     # forward + backward only (no optimiser, fixed parameters, eager launches)
     st32 = make(False)
+    st32.set_tile(tile32)
     msfb, _, _ = timed(st32, args.steps, args.warmup)
     out["fwd_bwd_only_eager_it_s"] = round(1000.0 / msfb, 2)
 
@@ -196,7 +204,7 @@ def roofline(step, stage, kms, traffic):
     """Roofline entry of one stage's kernel: shading is compared with the fp32
     MFMA peak (its arithmetic intensity is above the fp32 ridge of 19.7
     flop/B), the raster kernels with HBM bandwidth."""
-    kernel = STAGE_KERNEL[stage]
+    kernel = stage_kernel(stage, step.tile)
     tr = traffic.get(kernel)
     if stage == "shade":
         flops = step.algorithmic_flops(stage)

@@ -72,13 +72,16 @@ struct MaskSeed {
     const unsigned long long* device;
 };
 
+// clear = false when the caller has already zeroed w.tile_count and w.stats.
 hipError_t launch_raster_bin(const Views& v, int V, const float* pos, int T, const int* tri,
-                             const RasterWorkspace& w, hipStream_t s);
+                             const RasterWorkspace& w, hipStream_t s, bool clear = true);
 hipError_t launch_raster_resolve(const Views& v, int T, const AlphaSource& a, int mask_mode,
                                  MaskSeed seed, float* rast, const RasterWorkspace& w, hipStream_t s);
 
 // Training-step variant of the resolve: background pixels get their colour,
 // MSE gradient and L1 written here, covered pixels are appended to a list.
+// It leaves the covered-fragment count per tile in w.tile_cov and does not
+// sum it into the stats block (the caller does).
 struct BackgroundShade {
     const float* target;   // [P, 3]
     float* color;          // [P, 3]

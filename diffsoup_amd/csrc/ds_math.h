@@ -98,6 +98,16 @@ __host__ __device__ __forceinline__ bool cover(float ndc_y, float ndc_x,
     const float A2 = q0x * q1y - q0y * q1x;
     const float A = A0 + A1 + A2;
     if (A == 0.f) return false;
+    // Early rejection that changes no decision: when A_i and A differ in sign
+    // and |A_i| >= |A| 2^-60, b_i = A_i * (1/A) rounds to a non-zero negative
+    // value (it cannot underflow to -0), so the inclusive test below fails;
+    // the division is skipped for those pixels only.
+    {
+        const bool neg = A < 0.f;
+        const float lim = fabsf(A) * 0x1p-60f;
+        if ((A0 < 0.f) != neg && A0 != 0.f && fabsf(A0) >= lim) return false;
+        if ((A1 < 0.f) != neg && A1 != 0.f && fabsf(A1) >= lim) return false;
+    }
     const float invA = 1.f / A;
     h.b0 = A0 * invA;
     h.b1 = A1 * invA;

@@ -35,6 +35,5 @@ int shade_nparam(int nl, int c, int dh);
 hipError_t launch_shade_blobs(int nl, int c, int dh, bool bf16, const float* w1, const float* w2,
                               const float* w3, float* blobs, hipStream_t s);
 hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s);
-hipError_t launch_reduce_partials(const float* partials, int nblocks, int nparam, float* out, hipStream_t s);
 
 }  // namespace ds

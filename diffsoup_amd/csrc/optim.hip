@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "diffsoup_hip.h"
 
 namespace ds {
@@ -17,7 +19,8 @@ namespace {
 
 constexpr int kMaxGroups = 16;
 constexpr int kThreads = 256;
-constexpr int kUnitsPerBlock = 4096;
+constexpr int kChunk = 4 * kThreads;   // units per chunk: one float4 per thread
+constexpr int kResidentBlocks = 256 * 8;  // 256 CUs x 8 blocks of 4 waves
 
 struct Group {
     const float* p;
@@ -29,7 +32,7 @@ struct Group {
     int dim;          // 0: Adam; k > 0: VectorAdam over vectors of k floats
     double lr, b1, b2;     // bias corrections are formed in double
     float b1f, b2f, w1, w2, eps, lrf;  // fp32 forms: beta, 1 - beta (rounded from double), eps, lr
-    long long block_base;  // first block of this group
+    long long block_base;  // first chunk of this group
 };
 
 struct Table {
@@ -37,52 +40,96 @@ struct Table {
     int n;
 };
 
-__global__ void __launch_bounds__(kThreads) k_adam(Table tab, const long long* __restrict__ step_before) {
-    int gi = 0;
-    while (gi + 1 < tab.n && (long long)blockIdx.x >= tab.grp[gi + 1].block_base) ++gi;
-    const Group& G = tab.grp[gi];
-    const double t = static_cast<double>(*step_before + 1);
-    // bias corrections in double, as torch evaluates beta ** step on the host
-    const double bc1 = 1.0 - pow(static_cast<double>(G.b1), t);
-    const double bc2 = 1.0 - pow(static_cast<double>(G.b2), t);
-    const long long u0 = ((long long)blockIdx.x - G.block_base) * kUnitsPerBlock;
-    const long long u1 = min(u0 + kUnitsPerBlock, G.units);
-    if (G.dim == 0) {
-        // torch.optim.Adam (single-tensor form): m.lerp_(g, 1 - b1);
-        // v = v * b2 + (1 - b2) g g; p -= (lr / bc1) * m / (sqrt(v) / sqrt(bc2) + eps)
-        const float step_size = static_cast<float>(G.lr / bc1);
-        const float bc2_sqrt = static_cast<float>(sqrt(bc2));
-        for (long long i = u0 + threadIdx.x; i < u1; i += kThreads) {
-            const float g = G.g[i];
-            float m = G.m[i];
-            m = m + G.w1 * (g - m);
-            float v = G.v[i] * G.b2f;
-            v = v + G.w2 * (g * g);
-            G.m[i] = m;
-            G.v[i] = v;
-            const float denom = sqrtf(v) / bc2_sqrt + G.eps;
-            G.out[i] = G.p[i] - step_size * (m / denom);
-        }
-    } else {
-        // VectorAdam: g1 = b1 g1 + (1 - b1) g; g2 = b2 g2 + (1 - b2) |g|^2;
-        // p -= lr * (g1 / bc1) / (sqrt(g2 / bc2) + 1e-8)
-        const float ibc1 = static_cast<float>(bc1), ibc2 = static_cast<float>(bc2);
-        const int k = G.dim;
-        for (long long u = u0 + threadIdx.x; u < u1; u += kThreads) {
-            const long long base = u * k;
-            float sq = 0.f;
-            for (int j = 0; j < k; ++j) {
-                const float g = G.g[base + j];
-                sq += g * g;
+__device__ __forceinline__ float adam_update(const Group& G, float step_size, float bc2_sqrt, float p, float g,
+                                             float& m, float& v) {
+    // torch.optim.Adam (single-tensor form): m.lerp_(g, 1 - b1);
+    // v = v * b2 + (1 - b2) g g; p -= (lr / bc1) * m / (sqrt(v) / sqrt(bc2) + eps)
+    m = m + G.w1 * (g - m);
+    v = v * G.b2f;
+    v = v + G.w2 * (g * g);
+    const float denom = sqrtf(v) / bc2_sqrt + G.eps;
+    return p - step_size * (m / denom);
+}
+
+// Persistent over chunks of kChunk units: chunk c belongs to the group
+// whose [chunk_base, next chunk_base) holds it.  A block forms the bias
+// corrections of every group once (in double, as torch evaluates beta ** step
+// on the host), then walks the chunks with a grid stride, so the grid is one
+// resident wave of blocks and no tail of stragglers is left.
+__global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks,
+                                                   const long long* __restrict__ step_before) {
+    __shared__ double bc[kMaxGroups][2];
+    if (threadIdx.x < (unsigned)tab.n) {
+        const Group& G = tab.grp[threadIdx.x];
+        const double t = static_cast<double>(*step_before + 1);
+        bc[threadIdx.x][0] = 1.0 - pow(G.b1, t);
+        bc[threadIdx.x][1] = 1.0 - pow(G.b2, t);
+    }
+    __syncthreads();
+    for (long long c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        int gi = 0;
+        while (gi + 1 < tab.n && c >= tab.grp[gi + 1].block_base) ++gi;
+        const Group& G = tab.grp[gi];
+        const double bc1 = bc[gi][0], bc2 = bc[gi][1];
+        const long long u0 = (c - G.block_base) * kChunk;
+        const long long u1 = min(u0 + kChunk, G.units);
+        if (G.dim == 0) {
+            const float step_size = static_cast<float>(G.lr / bc1);
+            const float bc2_sqrt = static_cast<float>(sqrt(bc2));
+            const bool vec = ((reinterpret_cast<uintptr_t>(G.p) | reinterpret_cast<uintptr_t>(G.out) |
+                               reinterpret_cast<uintptr_t>(G.g) | reinterpret_cast<uintptr_t>(G.m) |
+                               reinterpret_cast<uintptr_t>(G.v)) & 15u) == 0;
+            long long i = u0;
+            if (vec) {
+                // one float4 per thread: u0 is a multiple of kChunk, so the
+                // accesses stay 16-byte aligned
+                const long long n4 = (u1 - u0) / 4;
+                const long long q = threadIdx.x;
+                if (q < n4) {
+                    const long long e = u0 + 4 * q;
+                    const float4 p = *reinterpret_cast<const float4*>(G.p + e);
+                    const float4 g = *reinterpret_cast<const float4*>(G.g + e);
+                    float4 m = *reinterpret_cast<const float4*>(G.m + e);
+                    float4 v = *reinterpret_cast<const float4*>(G.v + e);
+                    float4 o;
+                    o.x = adam_update(G, step_size, bc2_sqrt, p.x, g.x, m.x, v.x);
+                    o.y = adam_update(G, step_size, bc2_sqrt, p.y, g.y, m.y, v.y);
+                    o.z = adam_update(G, step_size, bc2_sqrt, p.z, g.z, m.z, v.z);
+                    o.w = adam_update(G, step_size, bc2_sqrt, p.w, g.w, m.w, v.w);
+                    *reinterpret_cast<float4*>(G.m + e) = m;
+                    *reinterpret_cast<float4*>(G.v + e) = v;
+                    *reinterpret_cast<float4*>(G.out + e) = o;
+                }
+                i = u0 + 4 * n4;
             }
-            const float g2 = G.v[u] * G.b2f + G.w2 * sq;
-            G.v[u] = g2;
-            const float den = sqrtf(g2 / ibc2) + G.eps;
-            for (int j = 0; j < k; ++j) {
-                const float g = G.g[base + j];
-                const float g1 = G.m[base + j] * G.b1f + G.w1 * g;
-                G.m[base + j] = g1;
-                G.out[base + j] = G.p[base + j] - G.lrf * ((g1 / ibc1) / den);
+            for (i += threadIdx.x; i < u1; i += kThreads) {
+                float m = G.m[i], v = G.v[i];
+                const float o = adam_update(G, step_size, bc2_sqrt, G.p[i], G.g[i], m, v);
+                G.m[i] = m;
+                G.v[i] = v;
+                G.out[i] = o;
+            }
+        } else {
+            // VectorAdam: g1 = b1 g1 + (1 - b1) g; g2 = b2 g2 + (1 - b2) |g|^2;
+            // p -= lr * (g1 / bc1) / (sqrt(g2 / bc2) + 1e-8)
+            const float ibc1 = static_cast<float>(bc1), ibc2 = static_cast<float>(bc2);
+            const int k = G.dim;
+            for (long long u = u0 + threadIdx.x; u < u1; u += kThreads) {
+                const long long base = u * k;
+                float sq = 0.f;
+                for (int j = 0; j < k; ++j) {
+                    const float g = G.g[base + j];
+                    sq += g * g;
+                }
+                const float g2 = G.v[u] * G.b2f + G.w2 * sq;
+                G.v[u] = g2;
+                const float den = sqrtf(g2 / ibc2) + G.eps;
+                for (int j = 0; j < k; ++j) {
+                    const float g = G.g[base + j];
+                    const float g1 = G.m[base + j] * G.b1f + G.w1 * g;
+                    G.m[base + j] = g1;
+                    G.out[base + j] = G.p[base + j] - G.lrf * ((g1 / ibc1) / den);
+                }
             }
         }
     }
@@ -129,11 +176,13 @@ extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* 
         G.eps = static_cast<float>(s.eps);
         G.lrf = static_cast<float>(s.lr);
         G.block_base = blocks;
-        blocks += (G.units + kUnitsPerBlock - 1) / kUnitsPerBlock;
+        blocks += (G.units + kChunk - 1) / kChunk;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     long long* t = reinterpret_cast<long long*>(step);
-    if (blocks > 0) k_adam<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(tab, t);
+    if (blocks > 0)
+        k_adam<<<static_cast<unsigned>(std::min<long long>(blocks, kResidentBlocks)), kThreads, 0, st>>>(tab, blocks,
+                                                                                                          t);
     k_tick<<<1, 64, 0, st>>>(t);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? DS_OK : optim_fail(DS_ERR_HIP, hipGetErrorString(e));

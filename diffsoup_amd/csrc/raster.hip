@@ -23,6 +23,7 @@
 // python/diffsoup/multires.py:52-102 (the composition).
 #include "ds_kernels.h"
 #include "ds_math.h"
+#include "ds_scan.h"
 
 namespace ds {
 namespace {
@@ -91,48 +92,13 @@ __global__ void k_setup(Views v, int V, const float4* __restrict__ pos, int T,
 // Exclusive scan of n ints by one 1024-thread workgroup, 4 items per thread
 // per round.  off[n] receives the total; totals above `capacity` set the
 // overflow stat so that k_emit writes nothing and k_resolve falls back.
+// Exclusive scan of the per-tile counts into bin offsets, one workgroup
+// (block_scan_1024).  off[n] is the total clamped to the bin capacity; the
+// total and the overflow flag go to stats.
 __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int* __restrict__ off,
                                                int n, long long capacity, long long* stats) {
-    __shared__ long long wave_sum[1024 / kWave];
-    __shared__ long long carry_s;
-    const int tid = threadIdx.x, lane = tid % kWave, wv = tid / kWave;
-    if (tid == 0) carry_s = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 4 * 1024) {
-        const int i = base + 4 * tid;
-        int v0 = i + 0 < n ? cnt[i + 0] : 0;
-        int v1 = i + 1 < n ? cnt[i + 1] : 0;
-        int v2 = i + 2 < n ? cnt[i + 2] : 0;
-        int v3 = i + 3 < n ? cnt[i + 3] : 0;
-        const long long s = (long long)v0 + v1 + v2 + v3;
-        long long inc = s;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const long long o = __shfl_up(inc, d, kWave);
-            if (lane >= d) inc += o;
-        }
-        if (lane == kWave - 1) wave_sum[wv] = inc;
-        __syncthreads();
-        long long wpre = 0, total = 0;
-        for (int k = 0; k < 1024 / kWave; ++k) {
-            const long long ws = wave_sum[k];
-            if (k < wv) wpre += ws;
-            total += ws;
-        }
-        const long long carry = carry_s;
-        long long e = carry + wpre + inc - s;
-        if (i + 0 < n) off[i + 0] = static_cast<int>(e);
-        e += v0;
-        if (i + 1 < n) off[i + 1] = static_cast<int>(e);
-        e += v1;
-        if (i + 2 < n) off[i + 2] = static_cast<int>(e);
-        e += v2;
-        if (i + 3 < n) off[i + 3] = static_cast<int>(e);
-        __syncthreads();
-        if (tid == 0) carry_s = carry + total;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        const long long total = carry_s;
+    const long long total = block_scan_1024<int>(cnt, n, off, false);
+    if (threadIdx.x == 0) {
         off[n] = total > capacity ? static_cast<int>(capacity) : static_cast<int>(total);
         stats[kStatPairs] = total;
         if (total > capacity) stats[kStatOverflow] = 1;
@@ -371,11 +337,22 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
 }
 
 // Sum of the per-tile covered-fragment counts into the stats block.
+// stats[covered] = sum of the per-tile covered-fragment counts: each thread
+// issues all its loads (a strided run of up to 32) before adding.
 __global__ void __launch_bounds__(1024) k_sum_covered(const int* __restrict__ tile_cov, int n,
                                                       long long* __restrict__ stats) {
     __shared__ long long red[16];
     long long s = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += tile_cov[i];
+    for (int base = 0; base < n; base += 32 * 1024) {
+        int v[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int i = base + k * 1024 + static_cast<int>(threadIdx.x);
+            v[k] = i < n ? tile_cov[i] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 32; ++k) s += v[k];
+    }
     for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
     if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
     __syncthreads();
@@ -506,10 +483,11 @@ hipError_t launch_fill_f32(float* p, long long n, float value, hipStream_t s) {
 }
 
 hipError_t launch_raster_bin(const Views& v, int V, const float* pos, int T, const int* tri,
-                             const RasterWorkspace& w, hipStream_t s) {
+                             const RasterWorkspace& w, hipStream_t s, bool clear) {
     const int n_tiles = v.tile_base[v.n];
-    k_clear<<<blocks_for(std::max(n_tiles, (int)kStatCount), 256), 256, 0, s>>>(w.tile_count, n_tiles,
-                                                                                  w.stats);
+    if (clear)
+        k_clear<<<blocks_for(std::max(n_tiles, (int)kStatCount), 256), 256, 0, s>>>(w.tile_count, n_tiles,
+                                                                                      w.stats);
     const long long BT = (long long)v.n * T;
     if (BT > 0) {
         k_setup<<<blocks_for(BT, 256), 256, 0, s>>>(v, V, reinterpret_cast<const float4*>(pos), T, tri,
@@ -546,7 +524,8 @@ static hipError_t resolve_dispatch(const Views& v, int T, const AlphaSource& a, 
         default:
             return hipErrorInvalidValue;
     }
-    k_sum_covered<<<1, 1024, 0, s>>>(w.tile_cov, n_tiles, w.stats);
+    // the training step sums the per-tile counts in its epilogue instead
+    if (!SHADE) k_sum_covered<<<1, 1024, 0, s>>>(w.tile_cov, n_tiles, w.stats);
     return hipGetLastError();
 }
 

@@ -1,9 +1,10 @@
-// Multi-block prefix sums of int32 counts.
+// Prefix sums of int32 counts.
 //
-// Three launches, no host synchronisation: per-block sums of 4096 items, one
-// workgroup scans the block sums (int64), then every block rescans its items
-// with its carry.  Used for the per-triangle pixel offsets of the training
-// step and for the fragment offsets / compaction of the plain path.
+// Up to 65536 items: one workgroup (block_scan_1024 in ds_scan.h).  Larger inputs: three launches, no host
+// synchronisation - per-block sums of 4096 items, one workgroup scans the
+// block sums (int64), then every block rescans its items with its carry.
+// Used for the per-triangle pixel offsets of the training step and for the
+// fragment offsets / compaction of the plain path.
 #include "ds_kernels.h"
 #include "ds_scan.h"
 
@@ -11,6 +12,7 @@ namespace ds {
 namespace {
 
 constexpr int kItems = 4 * 1024;
+constexpr long long kOneBlock = 64 * 1024;   // single-workgroup scan up to this many items
 
 __global__ void __launch_bounds__(1024) k_block_sums(const int* __restrict__ in, long long n,
                                                      long long* __restrict__ sums) {
@@ -96,6 +98,17 @@ __global__ void __launch_bounds__(1024) k_block_scan(const int* __restrict__ in,
     if (write_total && blockIdx.x == gridDim.x - 1 && tid == 0) out[n] = static_cast<OUT>(carry);
 }
 
+// Inputs up to kOneBlock items: one workgroup (block_scan_1024).
+template <typename OUT>
+__global__ void __launch_bounds__(1024) k_scan_one(const int* in, int n, OUT* out, long long* total_slot,
+                                                   int inclusive, int write_total) {
+    const long long total = block_scan_1024<OUT>(in, n, out, inclusive != 0);
+    if (threadIdx.x == 0) {
+        *total_slot = total;
+        if (write_total) out[n] = static_cast<OUT>(total);
+    }
+}
+
 __global__ void k_zero_total(long long* tmp, int* out) {
     tmp[0] = 0;
     if (out) out[0] = 0;
@@ -116,6 +129,10 @@ static hipError_t scan_impl(const int* in, long long n, OUT* out, void* tmp, int
                             hipStream_t s) {
     long long* sums = static_cast<long long*>(tmp);
     const int nb = static_cast<int>((n + kItems - 1) / kItems);
+    if (n > 0 && n <= kOneBlock) {
+        k_scan_one<OUT><<<1, 1024, 0, s>>>(in, static_cast<int>(n), out, sums + nb, inclusive, write_total);
+        return hipGetLastError();
+    }
     if (nb == 0) {
         k_zero_total<<<1, 1, 0, s>>>(sums, write_total ? reinterpret_cast<int*>(out) : nullptr);
         return hipGetLastError();

@@ -7,10 +7,11 @@
 // contiguous ranges, one per wave.  For each triangle segment in its range a
 // wave stages the triangle's feature row (all levels, all channels) in LDS,
 // shades the segment's pixels (any view) in tiles of 16, accumulates the input
-// gradient into an LDS gradient row with LDS atomics, and writes that row
-// once - with plain coalesced stores when the triangle lies wholly in the
-// range, with global atomics only for the few triangles split between two
-// ranges.  Every layer is an MFMA product with
+// gradient into an LDS gradient row (the two coarsest lattice levels in
+// per-lane registers reduced at the end of the segment, finer levels with LDS
+// atomics), and writes that row once - with plain coalesced stores when the
+// triangle lies wholly in the range, with global atomics only for the few
+// triangles split between two ranges.  Every layer is an MFMA product with
 // the layer's output units on the M axis and the 16 pixels on the N axis.  In
 // the accumulator layout of v_mfma_f32_16x16x{4f32,16bf16} lane l holds rows
 // 4(l/16) + r, r = 0..3, of pixel l % 16, which is exactly the B-operand
@@ -20,7 +21,7 @@
 // staged in LDS.  Weight gradients contract over pixels, so the per-pixel
 // activations and deltas are transposed through a per-wave LDS tile and fed
 // to MFMAs whose accumulators persist across all tiles a wave shades; one
-// partial per workgroup is reduced by k_reduce_partials.
+// partial per workgroup is summed by the step's epilogue kernel.
 //
 // Reference behaviour: ColorMLP (python/diffsoup/multires.py:109-207),
 // multires_triangle_color (src/cuda/multires.cu:224-312, here per level
@@ -74,7 +75,6 @@ struct Cfg {
     static constexpr int STAGE = 16 * ((SH + SX > 2 * SH ? SH + SX : 2 * SH) > SD3 + SH
                                            ? (SH + SX > 2 * SH ? SH + SX : 2 * SH)
                                            : SD3 + SH);
-    static constexpr int CORN = 16 * 8 * NL;
     static_assert(C % 4 == 0, "channels per level must be a multiple of 4");
     static_assert(DH % 16 == 0, "hidden width must be a multiple of 16");
 };
@@ -82,13 +82,6 @@ struct Cfg {
 // One MFMA step: acc += A * B on a 16x16 tile.  For f32 an operand unit is a
 // single float (K = 4 per step over 4 lane groups), for bf16 four values
 // (K = 16 per step).
-// Weight of lattice vertex vtx in a pixel's cell (0 when it is not a corner):
-// corners staged as (idx0, idx1, idx2, w0, w1, w2) per pixel.
-__device__ __forceinline__ float corner_weight(const float* sc, int vtx) {
-    const int i0 = __float_as_int(sc[0]), i1 = __float_as_int(sc[1]), i2 = __float_as_int(sc[2]);
-    return i0 == vtx ? sc[3] : (i1 == vtx ? sc[4] : (i2 == vtx ? sc[5] : 0.f));
-}
-
 template <bool BF16>
 struct Mma;
 
@@ -110,18 +103,6 @@ struct Mma<false> {
         for (int s = 0; s < 4; ++s) {
             const int px = 4 * s + g;
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[px * sa + i], Bm[px * sb + i], acc, 0, 0, 0);
-        }
-        return acc;
-    }
-    // lattice scatter over 16 pixels: A[vertex][px] = corner weight, B[px][f] = dX
-    __device__ static __forceinline__ f32x4 scatter(const float* corners, const float* bx, int sb, int lane,
-                                                    int vtx) {
-        const int g = lane / 16, j = lane % 16;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int s = 0; s < 4; ++s) {
-            const int px = 4 * s + g;
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(corner_weight(corners + px * 8, vtx), bx[px * sb + j], acc,
-                                                       0, 0, 0);
         }
         return acc;
     }
@@ -151,17 +132,6 @@ struct Mma<true> {
         }
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pack(a), pack(b), acc, 0, 0, 0);
     }
-    __device__ static __forceinline__ f32x4 scatter(const float* corners, const float* bx, int sb, int lane,
-                                                    int vtx) {
-        const int g = lane / 16, j = lane % 16;
-        f32x4 a, b;
-        for (int e = 0; e < 4; ++e) {
-            const int px = 4 * g + e;
-            a[e] = corner_weight(corners + px * 8, vtx);
-            b[e] = bx[px * sb + j];
-        }
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pack(a), pack(b), f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-    }
 };
 
 // Offsets (in floats) of a layer tile pair inside the LDS blob copy.  A bf16
@@ -189,10 +159,9 @@ k_shade(ShadeArgs a, int rowp) {
     float* blobs = smem;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
-    float* frow = smem + K::BLOB_ELEMS + wave * (2 * rowp + K::STAGE + K::CORN);
+    float* frow = smem + K::BLOB_ELEMS + wave * (2 * rowp + K::STAGE);
     float* grow = frow + rowp;
     float* stage = grow + rowp;
-    float* scr = stage + K::STAGE;   // per (level, pixel): 3 lattice indices, 3 weights
     const int row = a.S_tot * C;
 
     // stage the weight blobs
@@ -253,6 +222,10 @@ k_shade(ShadeArgs a, int rowp) {
         }
         wave_lds_sync();
 
+        float racc[6][4];
+        for (int vtx = 0; vtx < 6; ++vtx)
+            for (int r = 0; r < 4; ++r) racc[vtx][r] = 0.f;
+
         int4 rec = j < npx ? a.list[beg + j] : make_int4(0, 0, 0, 0);
         for (int c0 = 0; c0 < npx; c0 += 16) {
             const int idx = c0 + j;
@@ -269,23 +242,15 @@ k_shade(ShadeArgs a, int rowp) {
 
             // ---- gather from the LDS row: lane (g, j) holds features 16 tt + 4 g + r of pixel j
             f32x4 x[NI];
+            Corners crs[NI];
             for (int tt = 0; tt < NI; ++tt) {
                 x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                crs[tt] = Corners{{0u, 0u, 0u}, {0.f, 0.f, 0.f}};
                 const int f0 = 16 * tt + 4 * g;
-                if (!valid && f0 < DIN && f0 % C == 0) {
-                    float* sc = scr + ((f0 / C) * 16 + j) * 8;
-                    sc[0] = sc[1] = sc[2] = __int_as_float(-1);
-                }
                 if (valid && f0 < DIN) {
                     const int lv = f0 / C, c = f0 % C;
                     const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
-                    if (c == 0) {
-                        float* sc = scr + (lv * 16 + j) * 8;
-                        sc[0] = __int_as_float(static_cast<int>(cr.idx[0]));
-                        sc[1] = __int_as_float(static_cast<int>(cr.idx[1]));
-                        sc[2] = __int_as_float(static_cast<int>(cr.idx[2]));
-                        sc[3] = cr.w[0]; sc[4] = cr.w[1]; sc[5] = cr.w[2];
-                    }
+                    crs[tt] = cr;
                     const float* fl = frow + loff[lv] * C + c;
                     const float4 v0 = *reinterpret_cast<const float4*>(fl + cr.idx[0] * C);
                     const float4 v1 = *reinterpret_cast<const float4*>(fl + cr.idx[1] * C);
@@ -353,38 +318,30 @@ k_shade(ShadeArgs a, int rowp) {
             if (g == 0)
                 for (int r = 0; r < 4; ++r) dx[0][r] += dxr[r];
 
-            // ---- scatter the input gradient onto the LDS gradient row.  Per
-            // level, G[v][f] = sum_px W[px][v] dX[px][f] is a 16-pixel contraction:
-            // A = barycentric weight of lattice vertex v in pixel px (rebuilt
-            // from the staged corners), B = dX transposed through LDS.  Every
-            // (vertex, channel) of the result belongs to one lane, so the row
-            // update has no conflicting lanes.
-            {
-                float* sdx = stage;
-                for (int tt = 0; tt < NI; ++tt)
-                    for (int r = 0; r < 4; ++r) sdx[j * K::SX + 16 * tt + 4 * g + r] = dx[tt][r];
-                wave_lds_sync();
-                for (int tt = 0; tt < NI; ++tt) {
-                    if (16 * tt >= DIN) break;
-                    const int lv_lo = (16 * tt) / C;
-                    const int lv_hi = min(K::NL - 1, (16 * tt + 15) / C);
-                    for (int lv = lv_lo; lv <= lv_hi; ++lv) {
-                        const int S = static_cast<int>(feats_at_level(a.lo + lv));
-                        for (int m = 0; m * 16 < S; ++m) {
-                            const f32x4 acc = Mma<K::BF16>::scatter(scr + lv * 16 * 8, sdx + 16 * tt, K::SX, lane,
-                                                                    16 * m + j);
-                            const int f = 16 * tt + j;
-                            if (f < DIN && f / C == lv) {
-                                float* gcol = grow + loff[lv] * C + (f % C);
-                                for (int r = 0; r < 4; ++r) {
-                                    const int vtx = 16 * m + 4 * g + r;
-                                    if (vtx < S) gcol[vtx * C] += acc[r];
-                                }
-                            }
-                        }
+            // ---- scatter the input gradient onto the triangle's gradient row.
+            // Lane (g, j) holds dX of pixel j for 4 channels of one level and
+            // that level's lattice corners.  The two coarsest lattices (3 and
+            // 6 vertices) are shared by nearly all pixels of a tile, so they
+            // accumulate per lane in registers over the whole segment and are
+            // reduced across the 16 pixel lanes once at its end; finer levels
+            // add into the LDS row with LDS atomics, which rarely collide there.
+            for (int tt = 0; tt < NI; ++tt) {
+                const int f0 = 16 * tt + 4 * g;
+                if (!valid || f0 >= DIN) continue;
+                const int lv = f0 / C, c = f0 % C;
+                const Corners& cr = crs[tt];
+                if (tt == 0 && a.lo + lv <= 1) {
+                    for (int vtx = 0; vtx < 6; ++vtx) {
+                        const float wv = (cr.idx[0] == (uint32_t)vtx ? cr.w[0] : 0.f) +
+                                         (cr.idx[1] == (uint32_t)vtx ? cr.w[1] : 0.f) +
+                                         (cr.idx[2] == (uint32_t)vtx ? cr.w[2] : 0.f);
+                        for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
                     }
+                } else {
+                    float* gl = grow + loff[lv] * C + c;
+                    for (int k = 0; k < 3; ++k)
+                        for (int r = 0; r < 4; ++r) atomicAdd(gl + cr.idx[k] * C + r, cr.w[k] * dx[tt][r]);
                 }
-                wave_lds_sync();
             }
 
             // ---- bias gradients stay per lane until the end
@@ -426,6 +383,26 @@ k_shade(ShadeArgs a, int rowp) {
                 wave_lds_sync();
             }
             rec = next;
+        }
+
+        // ---- the coarse-level sums: reduce over the 16 pixel lanes of each
+        // lane group, then one lane per group adds them to the LDS row
+        {
+            const int f0 = 4 * g;
+            const int lv = f0 / C, c = f0 % C;
+            const bool coarse = f0 < DIN && a.lo + lv <= 1;
+            const int S = coarse ? static_cast<int>(feats_at_level(a.lo + lv)) : 0;
+            for (int vtx = 0; vtx < 6; ++vtx)
+                for (int r = 0; r < 4; ++r)
+                    for (int d = 8; d > 0; d >>= 1) racc[vtx][r] += __shfl_xor(racc[vtx][r], d, 16);
+            wave_lds_sync();
+            if (coarse && j == 0) {
+                float* gl = grow + loff[lv] * C + c;
+                for (int vtx = 0; vtx < 6; ++vtx)
+                    if (vtx < S)
+                        for (int r = 0; r < 4; ++r) gl[vtx * C + r] += racc[vtx][r];
+            }
+            wave_lds_sync();
         }
 
         // ---- the triangle's feature gradient row
@@ -532,27 +509,10 @@ __global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ 
     }
 }
 
-__global__ void k_reduce_partials(const float* __restrict__ partials, int nblocks, int nparam,
-                                  float* __restrict__ out) {
-    // blockDim = (64, 16): x over 64 parameters, y strides over the partial blocks
-    __shared__ float acc[16][65];
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    float s = 0.f;
-    if (k < nparam)
-        for (int b = threadIdx.y; b < nblocks; b += 16) s += partials[(long long)b * nparam + k];
-    acc[threadIdx.y][threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.y == 0 && k < nparam) {
-        float t = 0.f;
-        for (int r = 0; r < 16; ++r) t += acc[r][threadIdx.x];
-        out[k] = t;
-    }
-}
-
 template <class K>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     const int rowp = (a.S_tot * K::C + 3) & ~3;
-    const size_t lds = sizeof(float) * (K::BLOB_ELEMS + kWavesPerBlock * (2 * rowp + K::STAGE + K::CORN));
+    const size_t lds = sizeof(float) * (K::BLOB_ELEMS + kWavesPerBlock * (2 * rowp + K::STAGE));
     const size_t red = sizeof(float) * (K::BLOB_ELEMS + K::NPARAM);
     const size_t bytes = lds > red ? lds : red;
     if (bytes > 160 * 1024) return hipErrorInvalidValue;
@@ -622,10 +582,5 @@ hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, in
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_reduce_partials(const float* partials, int nblocks, int nparam, float* out, hipStream_t s) {
-    dim3 block(64, 16);
-    k_reduce_partials<<<(nparam + 63) / 64, block, 0, s>>>(partials, nblocks, nparam, out);
-    return hipGetLastError();
-}
 
 }  // namespace ds

@@ -1,16 +1,15 @@
 // One fused training step (forward + backward) on the caller's stream:
 //
 //   k_step_prep     sigmoid of the opacity logits, zero the gradient
-//                   accumulators and the covered-pixel counter
+//                   accumulators and counters, clip-space projection
 //   k_blobs         per-lane MLP weight operands (shade.hip)
-//   k_project       clip = MVP [x y z 1] for every (view, vertex)
 //   raster bin      setup / scan / emit (raster.hip)
 //   resolve+shade   depth test, background shading, covered-pixel list
 //   k_shade         gather -> MLP -> blend -> MSE, MLP adjoint, feature scatter
 //   edge grad       silhouette / intersection gradients into clip positions
 //   opacity aux     score-function opacity gradient (same bins, no re-binning)
-//   reductions      MLP weight gradients and the loss
-//   k_finalize      clip -> world vertex gradients, sigmoid backward
+//   k_epilogue      MLP weight-gradient reduction, the loss, clip -> world
+//                   vertex gradients and the sigmoid backward
 //
 // No allocation, no host synchronisation: everything lives in the caller's
 // workspace and output buffers, so the sequence can be captured in a graph.
@@ -25,9 +24,17 @@
 namespace ds {
 namespace {
 
+// The step's prologue, grid-stride: sigmoid of the opacity logits, zeroed
+// gradient accumulators and per-triangle counts, clip = M_b [x, y, z, 1] for
+// every (view, vertex) (row-major M, examples/utils.py:412-426, summed in the
+// order ((m0 x + m1 y) + m2 z) + m3), and the raster bins' cleared tile
+// counters and stats block.
 __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict__ alpha, long long n_alpha,
                             float* __restrict__ galpha, float* __restrict__ gpos, long long n_pos,
-                            int* __restrict__ tri_count, int T, float* __restrict__ gfeat, long long n_feat) {
+                            int* __restrict__ tri_count, int T, float* __restrict__ gfeat, long long n_feat,
+                            int B, int V, const float* __restrict__ verts, const float* __restrict__ mvp,
+                            float4* __restrict__ pos, int* __restrict__ tile_count, int n_tiles,
+                            long long* __restrict__ stats) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     for (long long i = i0; i < n_alpha; i += stride) {
@@ -39,27 +46,83 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
     float4* gp4 = reinterpret_cast<float4*>(gpos);
     for (long long i = i0; i < n_pos / 4; i += stride) gp4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (long long i = i0; i < T; i += stride) tri_count[i] = 0;
+    for (long long i = i0; i < (long long)B * V; i += stride) {
+        const int b = static_cast<int>(i / V);
+        const int v = static_cast<int>(i - (long long)b * V);
+        const float x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
+        const float* m = mvp + 16 * b;
+        float o[4];
+        for (int r = 0; r < 4; ++r) o[r] = m[4 * r] * x + m[4 * r + 1] * y + m[4 * r + 2] * z + m[4 * r + 3];
+        pos[i] = make_float4(o[0], o[1], o[2], o[3]);
+    }
+    for (long long i = i0; i < n_tiles; i += stride) tile_count[i] = 0;
+    if (i0 < kStatCount) stats[i0] = 0;
 }
 
-// clip[b, v] = M_b [x, y, z, 1], row-major M (examples/utils.py:412-426)
-__global__ void k_project(int B, int V, const float* __restrict__ verts, const float* __restrict__ mvp,
-                          float4* __restrict__ pos) {
-    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (id >= (long long)B * V) return;
-    const int b = static_cast<int>(id / V);
-    const int v = static_cast<int>(id - (long long)b * V);
-    const float x = verts[3 * v], y = verts[3 * v + 1], z = verts[3 * v + 2];
-    const float* m = mvp + 16 * b;
-    float o[4];
-    for (int i = 0; i < 4; ++i) o[i] = m[4 * i] * x + m[4 * i + 1] * y + m[4 * i + 2] * z + m[4 * i + 3];
-    pos[id] = make_float4(o[0], o[1], o[2], o[3]);
-}
-
-// grad_verts[v] = sum_b (M_b^T grad_pos[b, v])[0:3]; grad_logits = scale * g * s (1 - s)
-__global__ void k_finalize(int B, int V, const float* __restrict__ mvp, const float4* __restrict__ gpos,
-                           float* __restrict__ gverts, long long n_alpha, const float* __restrict__ alpha,
-                           const float* __restrict__ galpha, float aux_scale, float* __restrict__ glogits) {
-    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+// The step's closing reductions in one launch of 1024-thread blocks:
+//   blocks [0, R)       grad_mlp[k] = sum over shading blocks of partials[.][k],
+//                       64 parameters per block, 16 partial rows per pass;
+//   block R             loss = (sum of the per-tile background losses + the
+//                       shaded-pixel loss column of the partials) * scale;
+//   block R + 1         stats[covered] = sum of the per-tile covered counts;
+//   blocks (R + 1, R + 1 + F]
+//                       grad_verts[v] = sum_b (M_b^T grad_pos[b, v])[0:3] and
+//                       grad_logits = scale * g * s (1 - s), s = sigmoid.
+__global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ partials, int nblocks, int nparam,
+                                                   float* __restrict__ grad_mlp, const float* __restrict__ bg,
+                                                   int n_bg, float scale, float* __restrict__ loss, int B, int V,
+                                                   const float* __restrict__ mvp, const float4* __restrict__ gpos,
+                                                   float* __restrict__ gverts, long long n_alpha,
+                                                   const float* __restrict__ alpha,
+                                                   const float* __restrict__ galpha, float* __restrict__ glogits,
+                                                   const int* __restrict__ tile_cov, int n_tiles,
+                                                   long long* __restrict__ stats) {
+    __shared__ float acc[16][65];
+    __shared__ long long cov[16];
+    const int R = (nparam + 63) / 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    if ((int)blockIdx.x < R) {
+        const int k = blockIdx.x * 64 + tx;
+        float sum = 0.f;
+        if (k < nparam)
+            for (int b = ty; b < nblocks; b += 16) sum += partials[(long long)b * nparam + k];
+        acc[ty][tx] = sum;
+        __syncthreads();
+        if (ty == 0 && k < nparam) {
+            float t = 0.f;
+            for (int r = 0; r < 16; ++r) t += acc[r][tx];
+            grad_mlp[k] = t;
+        }
+        return;
+    }
+    if ((int)blockIdx.x == R) {
+        float sum = 0.f;
+        for (int i = threadIdx.x; i < n_bg; i += blockDim.x) sum += bg[i];
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) sum += partials[(long long)b * nparam + nparam - 1];
+        for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if (tx == 0) acc[0][ty] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float t = 0.f;
+            for (int q = 0; q < 16; ++q) t += acc[0][q];
+            *loss = t * scale;
+        }
+        return;
+    }
+    if ((int)blockIdx.x == R + 1) {
+        long long c = 0;
+        for (int i = threadIdx.x; i < n_tiles; i += blockDim.x) c += tile_cov[i];
+        for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+        if (tx == 0) cov[ty] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long t = 0;
+            for (int q = 0; q < 16; ++q) t += cov[q];
+            stats[kStatCovered] = t;
+        }
+        return;
+    }
+    const long long id = ((long long)blockIdx.x - R - 2) * blockDim.x + threadIdx.x;
     if (id < V) {
         float gx = 0.f, gy = 0.f, gz = 0.f;
         for (int b = 0; b < B; ++b) {
@@ -74,25 +137,8 @@ __global__ void k_finalize(int B, int V, const float* __restrict__ mvp, const fl
         gverts[3 * id + 2] = gz;
     }
     if (id < n_alpha) {
-        const float s = alpha[id];
-        glogits[id] = aux_scale * galpha[id] * (s * (1.0f - s));
-    }
-}
-
-// loss = (sum of background partials + shaded-pixel loss) * scale; one workgroup.
-__global__ void __launch_bounds__(1024) k_loss(const float* __restrict__ bg, int n_bg,
-                                               const float* __restrict__ fg, float scale,
-                                               float* __restrict__ loss) {
-    __shared__ float red[16];
-    float s = 0.f;
-    for (int i = threadIdx.x; i < n_bg; i += blockDim.x) s += bg[i];
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.f;
-        for (int k = 0; k < (int)(blockDim.x / 64); ++k) t += red[k];
-        *loss = (t + *fg) * scale;
+        const float sa = alpha[id];
+        glogits[id] = scale * galpha[id] * (sa * (1.0f - sa));
     }
 }
 
@@ -216,25 +262,22 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
         return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
     };
 
-    const long long prep_n = std::max(std::max(n_alpha, (long long)d->T),
-                                      std::max(std::max(n_pos / 4, n_feat / 4), 1LL));
+    const long long prep_n = std::max(std::max(std::max(n_alpha, (long long)d->T), (long long)L.n_tiles),
+                                      std::max(std::max(n_pos / 4, n_feat / 4), std::max((long long)d->B * d->V,
+                                                                                         (long long)kStatCount)));
     DS_TRY(mark(DS_STAGE_PREP, true), "event");
     k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
-        d->alpha_logits, d->alpha, n_alpha, galpha, gpos, n_pos, tri_count, d->T, d->grad_feat, n_feat);
+        d->alpha_logits, d->alpha, n_alpha, galpha, gpos, n_pos, tri_count, d->T, d->grad_feat, n_feat, d->B, d->V,
+        d->verts, d->mvp, reinterpret_cast<float4*>(d->pos), rw.tile_count, L.n_tiles, rw.stats);
     DS_TRY(hipGetLastError(), "step prep");
     DS_TRY(mark(DS_STAGE_PREP, false), "event");
     DS_TRY(mark(DS_STAGE_BLOBS, true), "event");
     DS_TRY(launch_shade_blobs(nl, d->channels, d->hidden, d->bf16 != 0, d->w1, d->w2, d->w3, blobs, s), "blobs");
     DS_TRY(mark(DS_STAGE_BLOBS, false), "event");
-    DS_TRY(mark(DS_STAGE_PROJECT, true), "event");
-    if ((long long)d->B * d->V > 0) {
-        k_project<<<blocks_for((long long)d->B * d->V, 256), 256, 0, s>>>(d->B, d->V, d->verts, d->mvp,
-                                                                        reinterpret_cast<float4*>(d->pos));
-        DS_TRY(hipGetLastError(), "project");
-    }
+    DS_TRY(mark(DS_STAGE_PROJECT, true), "event");   // projection runs in the prologue
     DS_TRY(mark(DS_STAGE_PROJECT, false), "event");
     DS_TRY(mark(DS_STAGE_BIN, true), "event");
-    DS_TRY(launch_raster_bin(v, d->V, d->pos, d->T, d->tri, rw, s), "raster bin");
+    DS_TRY(launch_raster_bin(v, d->V, d->pos, d->T, d->tri, rw, s, /*clear=*/false), "raster bin");
     DS_TRY(mark(DS_STAGE_BIN, false), "event");
     AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(Sa)};
     BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, bgloss, tile_list, tile_ncov, tri_count,
@@ -262,18 +305,66 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     DS_TRY(launch_opacity_aux(v, d->T, a, d->rast, d->pix_loss, galpha, rw, s), "opacity aux");
     DS_TRY(mark(DS_STAGE_AUX, false), "event");
     DS_TRY(mark(DS_STAGE_FINAL, true), "event");
-    DS_TRY(launch_reduce_partials(partials, L.nblocks, L.nparam, d->grad_mlp, s), "reduce");
-    k_loss<<<1, 1024, 0, s>>>(bgloss, L.n_tiles, d->grad_mlp + (L.nparam - 1), inv_n, d->loss);
-    DS_TRY(hipGetLastError(), "loss");
-    const long long nf = std::max<long long>(d->V, n_alpha);
-    if (nf > 0) {
-        k_finalize<<<blocks_for(nf, 256), 256, 0, s>>>(d->B, d->V, d->mvp, reinterpret_cast<const float4*>(gpos),
-                                                       d->grad_verts, n_alpha, d->alpha, galpha, inv_n,
-                                                       d->grad_alpha_logits);
-        DS_TRY(hipGetLastError(), "finalize");
+    {
+        const long long nf = std::max<long long>(d->V, n_alpha);
+        const unsigned R = static_cast<unsigned>((L.nparam + 63) / 64);
+        k_epilogue<<<R + 2 + blocks_for(nf, 1024), 1024, 0, s>>>(
+            partials, L.nblocks, L.nparam, d->grad_mlp, bgloss, L.n_tiles, inv_n, d->loss, d->B, d->V, d->mvp,
+            reinterpret_cast<const float4*>(gpos), d->grad_verts, n_alpha, d->alpha, galpha, d->grad_alpha_logits,
+            rw.tile_cov, L.n_tiles, rw.stats);
+        DS_TRY(hipGetLastError(), "epilogue");
     }
     DS_TRY(mark(DS_STAGE_FINAL, false), "event");
 #undef DS_TRY
+    return DS_OK;
+}
+
+int ds_autotune_step(ds_step_desc* d, const int* tiles, int n_tiles, int reps, float* ms, void* stream) {
+    using namespace ds;
+    if (!d || !tiles || !ms || n_tiles < 1 || n_tiles > 8 || reps < 1)
+        return step_fail(DS_ERR_INVALID, "autotune: tiles[1..8], reps >= 1 and non-null outputs required");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int saved_stage = d->timed_stage;
+    const int orig_tile = d->tile;
+    d->timed_stage = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        d->timed_stage = saved_stage;
+        return step_fail(DS_ERR_HIP, "autotune: event creation");
+    }
+    int rc = DS_OK, best = -1;
+    for (int i = 0; i < n_tiles && rc == DS_OK; ++i) {
+        ms[i] = -1.f;
+        d->tile = tiles[i];
+        const size_t need = ds_step_workspace_size(d);
+        if (need == 0 || need > d->workspace_bytes) continue;   // candidate does not fit: skipped
+        rc = ds_train_step(d, stream);                         // warm-up (code objects, caches)
+        if (rc != DS_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = step_fail(DS_ERR_HIP, "autotune: event"); break; }
+        for (int r = 0; r < reps && rc == DS_OK; ++r) rc = ds_train_step(d, stream);
+        if (rc != DS_OK) break;
+        float t = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            rc = step_fail(DS_ERR_HIP, "autotune: timing");
+            break;
+        }
+        ms[i] = t / static_cast<float>(reps);
+        if (best < 0 || ms[i] < ms[best]) best = i;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    d->timed_stage = saved_stage;
+    if (rc != DS_OK) {
+        d->tile = orig_tile;
+        return rc;
+    }
+    if (best < 0) {
+        d->tile = orig_tile;
+        return step_fail(DS_ERR_WORKSPACE, "autotune: no candidate tile fits the workspace");
+    }
+    d->tile = tiles[best];
     return DS_OK;
 }
 

@@ -55,6 +55,8 @@ STAGES = {"prep": 1, "blobs": 2, "project": 3, "bin": 4, "resolve": 5, "shade": 
 _lib.register("ds_step_mlp_params", [ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int64)
 _lib.register("ds_step_workspace_size", [ctypes.POINTER(StepDesc)], ctypes.c_size_t)
 _lib.register("ds_train_step", [ctypes.POINTER(StepDesc), ctypes.c_void_p], ctypes.c_int)
+_lib.register("ds_autotune_step", [ctypes.POINTER(StepDesc), ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_float), ctypes.c_void_p], ctypes.c_int)
 
 
 @dataclass
@@ -152,8 +154,7 @@ class FusedStep:
         db3 = g[o:o + 3]
         return dW1, db1, dW2, db2, dW3, db3
 
-    def run(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int) -> torch.Tensor:
-        """Enqueue one forward + backward; returns the device loss tensor."""
+    def _bind(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int) -> None:
         assert mvp.shape == (self.B, 4, 4) and target.shape == (self.P, 3)
         for t in params.tensors() + [mvp, target]:
             assert t.is_contiguous() and t.dtype == torch.float32 and t.device == self.dev
@@ -164,8 +165,86 @@ class FusedStep:
         d.w3, d.b3 = params.w3.data_ptr(), params.b3.data_ptr()
         d.target = target.data_ptr()
         d.seed = int(seed) & (2**64 - 1)
-        _lib.check(_lib.lib().ds_train_step(ctypes.byref(d), _lib.stream_ptr(self.dev)), "ds_train_step")
+
+    def run(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int) -> torch.Tensor:
+        """Enqueue one forward + backward; returns the device loss tensor."""
+        self._bind(params, mvp, target, seed)
+        _lib.check(_lib.lib().ds_train_step(ctypes.byref(self.desc), _lib.stream_ptr(self.dev)), "ds_train_step")
         return self.loss
+
+    def workspace_bytes(self, tile: int) -> int:
+        d = self.desc
+        old = d.tile
+        d.tile = tile
+        n = int(_lib.lib().ds_step_workspace_size(ctypes.byref(d)))
+        d.tile = old
+        return n
+
+    def _ensure_workspace(self, nbytes: int) -> None:
+        if nbytes > self.desc.workspace_bytes:
+            self._ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.dev)
+            self.desc.workspace = (self._ws.data_ptr() + 255) & ~255
+            self.desc.workspace_bytes = nbytes
+
+    def set_tile(self, tile: int) -> None:
+        n = self.workspace_bytes(tile)
+        if n == 0:
+            raise _lib.DiffSoupError(f"tile {tile}: " + _lib.lib().ds_last_error().decode())
+        self._ensure_workspace(n)
+        self.tile = self.desc.tile = int(tile)
+
+    def tune_key(self) -> str:
+        """Cache key of the tile choice: device, views, soup size and shading
+        configuration (the quantities the best tile depends on)."""
+        name = torch.cuda.get_device_name(self.dev).replace(" ", "_")
+        res = ",".join(f"{h}x{w}" for h, w in sorted(self.res))
+        return (f"{name}|B{self.B}|{res}|T{self.T}|V{self.V}|a{self.alpha_level}|l{self.lo}-{self.hi}|C{self.C}"
+                f"|h{self.hidden}|{'bf16' if self.bf16 else 'fp32'}")
+
+    def autotune(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, tiles=(8, 16, 32),
+                 reps: int = 5, seed: int = 0, cache: str | None = "default") -> dict:
+        """Pick the screen tile edge by timing whole steps (ds_autotune_step),
+        and remember the choice in a JSON cache keyed by ``tune_key()``
+        (``cache="default"``: $DIFFSOUP_AMD_TUNE_CACHE or
+        ~/.cache/diffsoup_amd/tiles.json; None: no cache).  Returns
+        {tile: ms per step} (from the cache: {tile: None} for the cached
+        choice).  The timed steps write this step's outputs."""
+        import json
+        import os
+
+        path = None
+        if cache == "default":
+            path = os.environ.get("DIFFSOUP_AMD_TUNE_CACHE",
+                                  os.path.join(os.path.expanduser("~"), ".cache", "diffsoup_amd", "tiles.json"))
+        elif cache:
+            path = cache
+        key = self.tune_key()
+        table = {}
+        if path and os.path.exists(path):
+            try:
+                with open(path) as fh:
+                    table = json.load(fh)
+            except (OSError, ValueError):
+                table = {}
+        if key in table and int(table[key]) in tiles:
+            self.set_tile(int(table[key]))
+            return {int(table[key]): None}
+        self._ensure_workspace(max(self.workspace_bytes(t) for t in tiles))
+        self._bind(params, mvp, target, seed)
+        arr = (ctypes.c_int * len(tiles))(*tiles)
+        ms = (ctypes.c_float * len(tiles))()
+        _lib.check(_lib.lib().ds_autotune_step(ctypes.byref(self.desc), arr, len(tiles), int(reps), ms,
+                                               _lib.stream_ptr(self.dev)), "ds_autotune_step")
+        self.tile = int(self.desc.tile)
+        result = {int(t): (float(m) if m >= 0 else None) for t, m in zip(tiles, ms)}
+        if path:
+            table[key] = self.tile
+            os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+            tmp = path + ".tmp"
+            with open(tmp, "w") as fh:
+                json.dump(table, fh, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+        return result
 
     def time_stage(self, stage, ev_begin=None, ev_end=None):
         """Bracket one stage of the next run() with two torch.cuda.Events

@@ -223,6 +223,16 @@ size_t ds_step_workspace_size(const ds_step_desc* d);
 /* Enqueue one fused forward + backward step (see above). */
 int ds_train_step(const ds_step_desc* d, void* stream);
 
+/* Tile autotuner: for each candidate tile edge (8, 16 or 32; at most 8
+ * candidates) whose workspace fits d->workspace_bytes, runs one warm-up step
+ * and `reps` timed steps on `stream` (events; this call synchronises the
+ * host), writes the mean milliseconds per step to ms[i] (-1 for a skipped
+ * candidate) and leaves d->tile set to the fastest.  The steps write the
+ * descriptor's outputs like any ds_train_step.  Size the workspace with the
+ * largest ds_step_workspace_size over the candidates.  No counterpart in the
+ * reference (its rasterizer has no tiles). */
+int ds_autotune_step(ds_step_desc* d, const int* tiles, int n_tiles, int reps, float* ms, void* stream);
+
 /* ------------------------------------------------------------------------
  * Plain path: the reference's fragment-list rasterizer, one function per
  * compiled function its Python layer binds, for callers that drive the
