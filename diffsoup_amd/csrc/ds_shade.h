@@ -14,6 +14,7 @@ struct ShadeArgs {
     const int4* list;       // covered pixels grouped by triangle: (pixel, bits b0, bits b1, t)
     int T;
     const float* feat;      // [T, S_tot, C]
+    const unsigned short* feat_bf16;   // bf16 copy read instead of feat when non-null
     float* grad_feat;       // [T, S_tot, C] written (every row)
     int S_tot;
     int lo;                 // first feature level

@@ -25,6 +25,7 @@ constexpr int kResidentBlocks = 256 * 8;  // 256 CUs x 8 blocks of 4 waves
 struct Group {
     const float* p;
     float* out;       // updated parameter (== p for an in-place update)
+    unsigned short* out16;   // optional bf16 copy of the updated parameter
     const float* g;
     float* m;
     float* v;
@@ -39,6 +40,13 @@ struct Table {
     Group grp[kMaxGroups];
     int n;
 };
+
+// bf16 bits of x, rounded to nearest even (NaN stays NaN).
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+    const uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
 
 __device__ __forceinline__ float adam_update(const Group& G, float step_size, float bc2_sqrt, float p, float g,
                                              float& m, float& v) {
@@ -78,7 +86,8 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
             const float bc2_sqrt = static_cast<float>(sqrt(bc2));
             const bool vec = ((reinterpret_cast<uintptr_t>(G.p) | reinterpret_cast<uintptr_t>(G.out) |
                                reinterpret_cast<uintptr_t>(G.g) | reinterpret_cast<uintptr_t>(G.m) |
-                               reinterpret_cast<uintptr_t>(G.v)) & 15u) == 0;
+                               reinterpret_cast<uintptr_t>(G.v)) & 15u) == 0 &&
+                             (reinterpret_cast<uintptr_t>(G.out16) & 7u) == 0;
             long long i = u0;
             if (vec) {
                 // one float4 per thread: u0 is a multiple of kChunk, so the
@@ -99,6 +108,12 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                     *reinterpret_cast<float4*>(G.m + e) = m;
                     *reinterpret_cast<float4*>(G.v + e) = v;
                     *reinterpret_cast<float4*>(G.out + e) = o;
+                    if (G.out16) {
+                        uint2 h;
+                        h.x = bf16_bits(o.x) | (bf16_bits(o.y) << 16);
+                        h.y = bf16_bits(o.z) | (bf16_bits(o.w) << 16);
+                        *reinterpret_cast<uint2*>(G.out16 + e) = h;
+                    }
                 }
                 i = u0 + 4 * n4;
             }
@@ -108,6 +123,7 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                 G.m[i] = m;
                 G.v[i] = v;
                 G.out[i] = o;
+                if (G.out16) G.out16[i] = static_cast<unsigned short>(bf16_bits(o));
             }
         } else {
             // VectorAdam: g1 = b1 g1 + (1 - b1) g; g2 = b2 g2 + (1 - b2) |g|^2;
@@ -128,7 +144,9 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                     const float g = G.g[base + j];
                     const float g1 = G.m[base + j] * G.b1f + G.w1 * g;
                     G.m[base + j] = g1;
-                    G.out[base + j] = G.p[base + j] - G.lrf * ((g1 / ibc1) / den);
+                    const float o = G.p[base + j] - G.lrf * ((g1 / ibc1) / den);
+                    G.out[base + j] = o;
+                    if (G.out16) G.out16[base + j] = static_cast<unsigned short>(bf16_bits(o));
                 }
             }
         }
@@ -161,6 +179,7 @@ extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* 
         Group& G = tab.grp[i];
         G.p = s.param;
         G.out = s.param_out ? s.param_out : s.param;
+        G.out16 = reinterpret_cast<unsigned short*>(s.param_bf16_out);
         G.g = s.grad;
         G.m = s.exp_avg;
         G.v = s.exp_avg_sq;

@@ -216,7 +216,15 @@ k_shade(ShadeArgs a, int rowp) {
         float4* g4 = reinterpret_cast<float4*>(grow);
         float4* f4 = reinterpret_cast<float4*>(frow);
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        {
+        if (a.feat_bf16) {
+            // bf16 features: 4 values per 8-byte load, widened exactly to fp32
+            const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row);
+            for (int i = lane; i < row / 4; i += kWave) {
+                const uint2 u = src[i];
+                f4[i] = make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                                    __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+            }
+        } else {
             const float4* src = reinterpret_cast<const float4*>(a.feat + (long long)t * row);
             for (int i = lane; i < row / 4; i += kWave) f4[i] = src[i];
         }

@@ -222,14 +222,15 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     if (!d->workspace || d->workspace_bytes < L.total)
         return step_fail(DS_ERR_WORKSPACE, "workspace smaller than ds_step_workspace_size");
     if (reinterpret_cast<uintptr_t>(d->workspace) & 255u) return step_fail(DS_ERR_INVALID, "workspace must be 256-byte aligned");
-    const void* req[] = {d->verts, d->mvp, d->tri, d->alpha_logits, d->feat, d->w1, d->b1, d->w2, d->b2,
+    const void* feat_in = d->feat_bf16 ? static_cast<const void*>(d->feat_bf16) : static_cast<const void*>(d->feat);
+    const void* req[] = {d->verts, d->mvp, d->tri, d->alpha_logits, feat_in, d->w1, d->b1, d->w2, d->b2,
                          d->w3, d->b3, d->target, d->pos, d->alpha, d->rast, d->color, d->grad_color,
                          d->pix_loss, d->grad_verts, d->grad_alpha_logits, d->grad_feat, d->grad_mlp, d->loss};
     for (const void* p : req)
         if (!p && d->T > 0) return step_fail(DS_ERR_INVALID, "null pointer in step descriptor");
     if ((reinterpret_cast<uintptr_t>(d->pos) | reinterpret_cast<uintptr_t>(d->rast) |
-         reinterpret_cast<uintptr_t>(d->feat) | reinterpret_cast<uintptr_t>(d->grad_feat)) & 15u)
-        return step_fail(DS_ERR_INVALID, "pos, rast, feat and grad_feat must be 16-byte aligned");
+         reinterpret_cast<uintptr_t>(feat_in) | reinterpret_cast<uintptr_t>(d->grad_feat)) & 15u)
+        return step_fail(DS_ERR_INVALID, "pos, rast, feat (or feat_bf16) and grad_feat must be 16-byte aligned");
 
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(d->workspace);
@@ -290,7 +291,8 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     DS_TRY(mark(DS_STAGE_RESOLVE, false), "event");
     DS_TRY(launch_exclusive_scan(tri_count, d->T, tri_off, scan_tmp, s), "triangle scan");
     DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, tri_off, list, s), "triangle lists");
-    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), tri_off, list, d->T, d->feat, d->grad_feat,
+    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), tri_off, list, d->T, d->feat,
+                 reinterpret_cast<const unsigned short*>(d->feat_bf16), d->grad_feat,
                  static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->target, d->color,
                  d->grad_color, d->pix_loss, 2.0f * inv_n, partials};
     DS_TRY(mark(DS_STAGE_SHADE, true), "event");

@@ -44,7 +44,7 @@ class StepDesc(ctypes.Structure):
         ("grad_feat", ctypes.c_void_p), ("grad_mlp", ctypes.c_void_p), ("loss", ctypes.c_void_p),
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("capacity", ctypes.c_int64),
         ("timed_stage", ctypes.c_int), ("ev_begin", ctypes.c_void_p), ("ev_end", ctypes.c_void_p),
-        ("seed_device", ctypes.c_void_p),
+        ("seed_device", ctypes.c_void_p), ("feat_bf16", ctypes.c_void_p),
     ]
 
 
@@ -72,6 +72,7 @@ class SoupParams:
     b2: torch.Tensor
     w3: torch.Tensor
     b3: torch.Tensor
+    feat_bf16: torch.Tensor | None = None   # bf16 copy of feat read by steps with bf16 features
 
     @staticmethod
     def from_soup(s, dev) -> "SoupParams":
@@ -83,12 +84,18 @@ class SoupParams:
     def tensors(self):
         return [self.verts, self.alpha_logits, self.feat, self.w1, self.b1, self.w2, self.b2, self.w3, self.b3]
 
+    def with_bf16_features(self) -> "SoupParams":
+        """Attach (or refresh) the bf16 copy of the features, rounded to
+        nearest even from the fp32 master."""
+        self.feat_bf16 = self.feat.detach().to(torch.bfloat16).contiguous()
+        return self
+
 
 class FusedStep:
     """Pre-allocated buffers and workspace for ds_train_step on one set of views."""
 
     def __init__(self, resolutions, T, V, alpha_level, levels, channels, hidden, *, bf16=False,
-                 stochastic=True, tile=16, background=1.0, device="cuda", capacity=None):
+                 bf16_features=False, stochastic=True, tile=16, background=1.0, device="cuda", capacity=None):
         self.dev = torch.device(device)
         self.res = [(int(h), int(w)) for h, w in resolutions]
         self.B = len(self.res)
@@ -98,6 +105,7 @@ class FusedStep:
         self.lo, self.hi = levels
         self.C, self.hidden = channels, hidden
         self.bf16, self.stochastic, self.tile, self.background = bf16, stochastic, tile, background
+        self.bf16_features = bool(bf16_features)
         self.Sa = feats_at_level(alpha_level)
         self.S_tot = feats_in_levels(self.lo, self.hi)
         self.nl = self.hi - self.lo + 1
@@ -165,6 +173,14 @@ class FusedStep:
         d.w3, d.b3 = params.w3.data_ptr(), params.b3.data_ptr()
         d.target = target.data_ptr()
         d.seed = int(seed) & (2**64 - 1)
+        if self.bf16_features:
+            fb = params.feat_bf16
+            if fb is None or fb.dtype != torch.bfloat16 or fb.shape != params.feat.shape or not fb.is_contiguous():
+                raise ValueError("bf16_features: params.feat_bf16 must be a contiguous bfloat16 copy of feat "
+                                 "(SoupParams.with_bf16_features)")
+            d.feat_bf16 = fb.data_ptr()
+        else:
+            d.feat_bf16 = None
 
     def run(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int) -> torch.Tensor:
         """Enqueue one forward + backward; returns the device loss tensor."""
@@ -199,7 +215,7 @@ class FusedStep:
         name = torch.cuda.get_device_name(self.dev).replace(" ", "_")
         res = ",".join(f"{h}x{w}" for h, w in sorted(self.res))
         return (f"{name}|B{self.B}|{res}|T{self.T}|V{self.V}|a{self.alpha_level}|l{self.lo}-{self.hi}|C{self.C}"
-                f"|h{self.hidden}|{'bf16' if self.bf16 else 'fp32'}")
+                f"|h{self.hidden}|{'bf16' if self.bf16 else 'fp32'}|{'f16feat' if self.bf16_features else 'f32feat'}")
 
     def autotune(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, tiles=(8, 16, 32),
                  reps: int = 5, seed: int = 0, cache: str | None = "default") -> dict:
@@ -347,11 +363,13 @@ def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5
     ``step`` writes.  One FusedAdam, so one launch per iteration.  With
     ``out`` the updated parameters are written there and ``params`` stays
     unchanged (every iteration then sees the same scene, as a throughput
-    measurement of a fixed configuration needs)."""
+    measurement of a fixed configuration needs).  When the parameters carry a
+    bf16 feature copy, the update refreshes it in the same launch."""
     from .optimize import FusedAdam
 
     o = out
-    groups = [dict(param=params.feat, grad=step.grad_feat, lr=lr_feat, param_out=o and o.feat),
+    fb = (o if o is not None else params).feat_bf16
+    groups = [dict(param=params.feat, grad=step.grad_feat, lr=lr_feat, param_out=o and o.feat, param_bf16_out=fb),
               dict(param=params.alpha_logits, grad=step.grad_alpha_logits, lr=lr_alpha,
                    param_out=o and o.alpha_logits),
               dict(param=params.verts, grad=step.grad_verts, lr=lr_verts, vector_dim=3, param_out=o and o.verts)]

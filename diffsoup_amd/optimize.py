@@ -52,7 +52,7 @@ class AdamGroup(ctypes.Structure):
     _fields_ = [("param", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
                 ("exp_avg_sq", ctypes.c_void_p), ("n", ctypes.c_int64), ("vector_dim", ctypes.c_int),
                 ("lr", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
-                ("eps", ctypes.c_double), ("param_out", ctypes.c_void_p)]
+                ("eps", ctypes.c_double), ("param_out", ctypes.c_void_p), ("param_bf16_out", ctypes.c_void_p)]
 
 
 _lib.register("ds_adam_step", [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -68,7 +68,9 @@ class FusedAdam:
     k > 0 pools the second moment over the last dimension of width k, as
     VectorAdam does) and ``param_out`` (None: update in place; else a tensor
     of the parameter's shape that receives the updated values while
-    ``param`` stays as it is).  Gradients are read from ``grad`` (for example
+    ``param`` stays as it is) and ``param_bf16_out`` (None, or a bfloat16
+    tensor of the parameter's shape that also receives the updated values
+    rounded to bf16).  Gradients are read from ``grad`` (for example
     the buffers a fused training step writes), not from ``param.grad``.  The step
     counter lives in device memory, so a captured graph advances it.
     Hyper-parameters are baked into each launch: change them with
@@ -98,7 +100,14 @@ class FusedAdam:
             m = torch.zeros_like(p)
             v = torch.zeros(p.numel() // k if k else p.numel(), dtype=torch.float32, device=p.device)
             b1, b2 = g.get("betas", (0.9, 0.999))
-            self.groups.append(dict(param=p, grad=gr, param_out=po, exp_avg=m, exp_avg_sq=v, vector_dim=k,
+            pb = g.get("param_bf16_out")
+            if pb is not None:
+                _lib.require_device(pb, "param_bf16_out")
+                if pb.dtype != torch.bfloat16 or not pb.is_contiguous() or pb.shape != p.shape:
+                    raise TypeError("FusedAdam: param_bf16_out must be a contiguous bfloat16 tensor of the "
+                                    "parameter's shape")
+            self.groups.append(dict(param=p, grad=gr, param_out=po, param_bf16_out=pb, exp_avg=m, exp_avg_sq=v,
+                                    vector_dim=k,
                                     lr=float(g.get("lr", 1e-3)), betas=(float(b1), float(b2)),
                                     eps=float(g.get("eps", 1e-8))))
         self.device = self.groups[0]["param"].device
@@ -113,6 +122,7 @@ class FusedAdam:
             a.n, a.vector_dim = g["param"].numel(), g["vector_dim"]
             a.lr, (a.beta1, a.beta2), a.eps = g["lr"], g["betas"], g["eps"]
             a.param_out = None if g["param_out"] is None else g["param_out"].data_ptr()
+            a.param_bf16_out = None if g["param_bf16_out"] is None else g["param_bf16_out"].data_ptr()
         self._arr = arr
 
     def set_lr(self, index, lr):

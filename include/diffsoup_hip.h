@@ -200,6 +200,10 @@ typedef struct ds_step_desc {
                                     device uint64 when the step runs (seed is
                                     then ignored): a captured graph replays with
                                     whatever seed the caller last stored there */
+    const uint16_t* feat_bf16;   /* if non-null, the features are read from this
+                                    bf16 copy [T, S_tot, channels] (round to
+                                    nearest even of the fp32 master) instead of
+                                    feat; grad_feat stays fp32 */
 } ds_step_desc;
 
 /* Stages of ds_train_step that timed_stage can select. */
@@ -315,6 +319,9 @@ typedef struct ds_adam_group {
     double lr, beta1, beta2, eps;
     float* param_out; /* NULL: update param in place; else write the updated
                          values here and leave param unchanged            */
+    uint16_t* param_bf16_out; /* if non-null, also write the updated values
+                         rounded to bf16 (nearest even): the copy a step
+                         with feat_bf16 reads                             */
 } ds_adam_group;
 
 /* At most 16 groups. */

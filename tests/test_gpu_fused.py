@@ -5,8 +5,11 @@ differences on the continuous parameters.
 
 Tolerances: fp32 MLP - colour within 1e-5 of float64, parameter gradients
 within 1e-4 of their largest magnitude; bf16 MLP (fp32 accumulation) -
-colour within 1e-3 of float64 (the image-error bar), gradients within 2e-2.
-The raster output must be bit-identical in both precisions."""
+colour within 1e-3 of float64 (the image-error bar), gradients within 2e-2;
+bf16 MLP and bf16 features - the same against float64 on the rounded
+features, and within 1e-3 plus twice the features' own rounding bound against
+float64 on the unrounded ones.  The raster output must be bit-identical in
+every precision."""
 import numpy as np
 import pytest
 import torch
@@ -23,12 +26,14 @@ def _close(got, ref, rel, what):
     assert err <= rel * scale, f"{what}: max err {err:.3e} > {rel:.1e} x {scale:.3e}"
 
 
-def _run(dev, s, bf16, seed=5):
+def _run(dev, s, bf16, seed=5, bf16_features=False):
     from diffsoup_amd import fused
     prm = fused.SoupParams.from_soup(s, dev)
+    if bf16_features:
+        prm.with_bf16_features()
     nl = s.levels[1] - s.levels[0] + 1
     st = fused.FusedStep(s.resolutions, s.T, s.verts.shape[0], s.alpha_level, s.levels, s.feat.shape[2],
-                         s.mlp_dims[1], bf16=bf16, device=dev)
+                         s.mlp_dims[1], bf16=bf16, bf16_features=bf16_features, device=dev)
     loss = st.run(prm, torch.from_numpy(s.mvp).to(dev), torch.from_numpy(s.target).to(dev), seed)
     torch.cuda.synchronize()
     out = {k: getattr(st, k).cpu().numpy() for k in ("pos", "alpha", "rast", "color", "grad_color",
@@ -50,11 +55,20 @@ def medium():
     return scenes.sphere_torus_soup(0)
 
 
-@pytest.mark.parametrize("which,bf16", [("small", False), ("medium", False), ("medium", True)])
-def test_fused_step_matches_reference(dev, small, medium, which, bf16):
-    s = small if which == "small" else medium
-    print(f"fused step {s.name} bf16={bf16}", flush=True)
-    g = _run(dev, s, bf16)
+def _bf16_round(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+@pytest.mark.parametrize("which,bf16,bf16_feat", [("small", False, False), ("medium", False, False),
+                                                  ("medium", True, False), ("medium", True, True)])
+def test_fused_step_matches_reference(dev, small, medium, which, bf16, bf16_feat):
+    import dataclasses
+    s0 = small if which == "small" else medium
+    print(f"fused step {s0.name} bf16={bf16} bf16 features={bf16_feat}", flush=True)
+    g = _run(dev, s0, bf16, bf16_features=bf16_feat)
+    # with bf16 features the reference sees the same rounded features; the
+    # error against the unrounded float64 model is checked separately below
+    s = dataclasses.replace(s0, feat=_bf16_round(s0.feat)) if bf16_feat else s0
     # projection against float64
     pos64 = np.einsum("bij,nj->bni", s.mvp.astype(np.float64),
                       np.concatenate([s.verts, np.ones((s.verts.shape[0], 1), np.float32)], 1).astype(np.float64))
@@ -75,6 +89,16 @@ def test_fused_step_matches_reference(dev, small, medium, which, bf16):
     np.testing.assert_allclose(g["pix_loss"], np.abs(g["color"] - s.target).sum(1), rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(g["grad_color"], 2 * (g["color"] - s.target) / g["color"].size, rtol=1e-5,
                                atol=1e-12)
+    if bf16_feat:
+        # against the float64 model on the unrounded fp32 features the error
+        # adds the quantisation of the feature table itself: bf16 keeps 8
+        # significant bits, so a blended feature of magnitude |f| moves by up
+        # to |f| 2^-9, about 1e-3 for the largest of these N(0, 0.1) features
+        ref0 = O.reference_step(s0, 5, pos=g["pos"], alpha=g["alpha"])
+        err0 = np.abs(g["color"] - ref0["color"]).max()
+        fmax = np.abs(s0.feat).max()
+        print(f"  colour max abs err vs fp64 on unrounded features: {err0:.3e} (|f|max {fmax:.3f})", flush=True)
+        assert err0 <= 1e-3 + 2 * fmax * 2.0 ** -9
     rel = 2e-2 if bf16 else 1e-4
     _close(g["grad_mlp"][:-1], ref["grad_mlp"], rel, "MLP grads")
     assert abs(g["grad_mlp"][-1] - ((g["color"][ref["covered"]] - s.target[ref["covered"]]) ** 2).sum()) < 1e-3 * max(1.0, g["grad_mlp"][-1])

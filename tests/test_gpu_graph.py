@@ -58,6 +58,16 @@ def test_fused_adam_matches_torch_and_vector_adam(dev):
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
         np.testing.assert_array_equal(c.cpu().numpy(), d.cpu().numpy())
 
+    # the bf16 copy written beside the update is torch's round-to-nearest-even
+    p16 = [torch.empty(o.shape, dtype=torch.bfloat16, device=dev) for o in ours]
+    opt4 = FusedAdam([dict(param=ours[0], grad=grads[0], lr=5e-2, param_bf16_out=p16[0]),
+                      dict(param=ours[1], grad=grads[1], lr=1e-2, param_bf16_out=p16[1]),
+                      dict(param=ours[2], grad=grads[2], lr=1e-3, vector_dim=3, param_bf16_out=p16[2])])
+    opt4.step()
+    torch.cuda.synchronize()
+    for a, b in zip(p16, ours):
+        np.testing.assert_array_equal(a.view(torch.int16).cpu().numpy(), b.to(torch.bfloat16).view(torch.int16).cpu().numpy())
+
 
 def _snap(p):
     return {k: getattr(p, k).detach().clone() for k in ("verts", "alpha_logits", "feat", "w1", "b3")}
