@@ -68,6 +68,11 @@ struct Cfg {
     // operand units of one (m, t) tile pair: 4 f32 steps or one bf16 step of 4 values
     static constexpr int TILE_ELEMS = 4 * kWave;
     static constexpr int BLOB_ELEMS = NTILES * TILE_ELEMS;
+    // the weight operands are staged in LDS when they are small; the large
+    // configurations read them from global memory (L2-resident, 80 KB at
+    // 80->64->64->3) and leave the LDS to the feature rows
+    static constexpr bool BLOB_LDS = BLOB_ELEMS * 4 <= 32 * 1024;
+    static constexpr int LDS_BLOB = BLOB_LDS ? BLOB_ELEMS : 0;
     static constexpr int NPARAM = DH * DIN + DH * DH + 3 * DH + DH + DH + 3 + 1;
     // per-wave LDS staging for the weight-gradient transposes (floats): one
     // (delta, activation) pair at a time - (D1, X), (D2, H1), (D3, H2)
@@ -156,18 +161,18 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 k_shade(ShadeArgs a, int rowp) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
     extern __shared__ float smem[];
-    float* blobs = smem;
+    const float* blobs = K::BLOB_LDS ? smem : a.blobs;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
-    float* frow = smem + K::BLOB_ELEMS + wave * (2 * rowp + K::STAGE);
+    float* frow = smem + K::LDS_BLOB + wave * (2 * rowp + K::STAGE);
     float* grow = frow + rowp;
     float* stage = grow + rowp;
     const int row = a.S_tot * C;
 
     // stage the weight blobs
-    {
+    if constexpr (K::BLOB_LDS) {
         const float4* src = reinterpret_cast<const float4*>(a.blobs);
-        float4* dst = reinterpret_cast<float4*>(blobs);
+        float4* dst = reinterpret_cast<float4*>(smem);
         for (int i = threadIdx.x; i < K::BLOB_ELEMS / 4; i += blockDim.x) dst[i] = src[i];
     }
     __syncthreads();
@@ -440,7 +445,7 @@ k_shade(ShadeArgs a, int rowp) {
 
     // ---- workgroup reduction into LDS (reusing the staging area), one partial per block
     __syncthreads();
-    float* red = smem + K::BLOB_ELEMS;
+    float* red = smem + K::LDS_BLOB;
     for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) red[i] = 0.f;
     __syncthreads();
     float* rW1 = red;
@@ -520,8 +525,8 @@ __global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ 
 template <class K>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     const int rowp = (a.S_tot * K::C + 3) & ~3;
-    const size_t lds = sizeof(float) * (K::BLOB_ELEMS + kWavesPerBlock * (2 * rowp + K::STAGE));
-    const size_t red = sizeof(float) * (K::BLOB_ELEMS + K::NPARAM);
+    const size_t lds = sizeof(float) * (K::LDS_BLOB + kWavesPerBlock * (2 * rowp + K::STAGE));
+    const size_t red = sizeof(float) * (K::LDS_BLOB + K::NPARAM);
     const size_t bytes = lds > red ? lds : red;
     if (bytes > 160 * 1024) return hipErrorInvalidValue;
     static size_t opted_in = 65536;
@@ -549,7 +554,8 @@ hipError_t blobs_dispatch(const float* w1, const float* w2, const float* w3, flo
     X(4, 8, 32)             \
     X(1, 8, 32)             \
     X(4, 4, 32)             \
-    X(2, 8, 32)
+    X(2, 8, 32)             \
+    X(5, 16, 64)
 
 }  // namespace
 

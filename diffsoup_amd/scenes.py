@@ -61,8 +61,9 @@ def _tangent_frame(n):
 
 
 def _planar_triangles(rng, centres, normals, edge, jitter):
-    """Equilateral triangles of the given edge in the tangent planes, with a
-    random in-plane rotation and per-vertex uniform jitter."""
+    """Equilateral triangles of the given edge (a scalar or one per centre,
+    shape [N,1]) in the tangent planes, with a random in-plane rotation and
+    per-vertex uniform jitter."""
     t1, t2 = _tangent_frame(normals)
     th = rng.uniform(0, 2 * math.pi, size=(len(centres), 1))
     rad = edge / math.sqrt(3.0)
@@ -192,6 +193,87 @@ def sphere_torus_soup(seed=0, n_surface=12000, n_float=3000, views=8, res=(800, 
     din = (hi - lo + 1) * C
     return Soup(verts, tri, logits, alpha_level, feat, levels, mvp, resolutions, target, (din, dh, 3),
                 _mlp(rng, din, dh, 3), "concat", f"sphere_torus_{T // 1000}k")
+
+
+MIXED_RESOLUTIONS = [(840, 1297), (546, 979), (822, 1237), (800, 800)]   # (H, W)
+
+
+def _features_and_mlp(rng, T, levels, C, alpha_level, dh, feat_std=0.1):
+    lo, hi = levels
+    S_tot = sum(feats_at_level(l) for l in range(lo, hi + 1))
+    logits = rng.normal(0, 1, size=(T, feats_at_level(alpha_level))).astype(np.float32)
+    feat = rng.normal(0, feat_std, size=(T, S_tot, C)).astype(np.float32)
+    din = (hi - lo + 1) * C
+    return logits, feat, din, _mlp(rng, din, dh, 3)
+
+
+def unbounded_soup(seed=0, T=50_000, views=32, resolutions=None, levels=(0, 3), C=8, alpha_level=3, dh=32,
+                   layered=0.0, res_scale=1.0, radius=4.0, edge_scale=1.0, name=None) -> Soup:
+    """An unbounded scene: 70% of the triangles within radius 2 of the origin
+    (edge 0.01), 30% at radii log-uniform in [2, 100] with edge proportional
+    to the radius (0.005 r); near/far 0.1/100.  ``layered`` > 0 turns that
+    fraction of the triangles into copies of others pushed outward by 1-3%
+    of their radius, stacking depth layers for depth-test contention.  Views
+    on a radius-4 orbit at fovy 50, each with a resolution drawn by the seed
+    from ``resolutions`` (default: 1297x840, 979x546, 1237x822, 800x800),
+    scaled by ``res_scale`` (scaled-down tests pair it with ``edge_scale`` to
+    keep the pixels per triangle); features N(0, 0.1) per level and channel,
+    MLP (levels x C) -> dh -> dh -> 3 (Xavier); MSE target U[0,1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_lay = int(round(T * layered))
+    n_base = T - n_lay
+    n_in = int(round(0.7 * n_base))
+    n_out = n_base - n_in
+    c_in = rng.normal(size=(n_in, 3))
+    c_in *= (2.0 * rng.uniform(0, 1, size=(n_in, 1)) ** (1 / 3)) / np.linalg.norm(c_in, axis=-1, keepdims=True)
+    tri_in = _planar_triangles(rng, c_in, rng.normal(size=(n_in, 3)), 0.01 * edge_scale, 0.0)
+    d = rng.normal(size=(n_out, 3))
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    rad = np.exp(rng.uniform(math.log(2.0), math.log(100.0), size=(n_out, 1)))
+    c_out = d * rad
+    tri_out = _planar_triangles(rng, c_out, rng.normal(size=(n_out, 3)), 0.005 * edge_scale * rad, 0.0)
+    tris = np.concatenate([tri_in, tri_out], 0)
+    if n_lay:
+        src = rng.integers(0, n_base, size=n_lay)
+        cen = tris[src].mean(axis=1, keepdims=True)
+        push = 1.0 + rng.uniform(0.01, 0.03, size=(n_lay, 1, 1))
+        tris = np.concatenate([tris, tris[src] + cen * (push - 1.0)], 0)
+    verts, tri = _soup_from_tris(tris)
+    T = tri.shape[0]
+    pool = resolutions or MIXED_RESOLUTIONS
+    pick = rng.integers(0, len(pool), size=views)
+    res = [(max(8, int(round(pool[k][0] * res_scale))), max(8, int(round(pool[k][1] * res_scale)))) for k in pick]
+    logits, feat, din, mlp = _features_and_mlp(rng, T, levels, C, alpha_level, dh)
+    mvp = orbit_cameras(views, radius, res, fovy=50.0, near=0.1, far=100.0)
+    P = sum(h * w for h, w in res)
+    target = rng.uniform(0, 1, size=(P, 3)).astype(np.float32)
+    return Soup(verts, tri, logits, alpha_level, feat, levels, mvp, res, target, (din, dh, 3), mlp, "concat",
+                name or f"unbounded_{T // 1000}k")
+
+
+def batched_uneven_soup(seed=0, **kw) -> Soup:
+    """50K-triangle unbounded scene, 32 views of mixed resolution, features
+    and MLP as the 15K-triangle soup (levels 0-3 x 8 channels, 32-32-32-3)."""
+    return unbounded_soup(seed, **{"T": 50_000, "views": 32, **kw})
+
+
+def bf16_large_soup(seed=0, **kw) -> Soup:
+    """100K-triangle unbounded scene, 16 views at 1920x1080, levels 0-4 with
+    16 channels (stored in bf16 by the step), MLP 80->64->64->3."""
+    args = {"T": 100_000, "views": 16, "resolutions": [(1080, 1920)], "levels": (0, 4), "C": 16,
+            "alpha_level": 4, "dh": 64}
+    args.update(kw)
+    return unbounded_soup(seed, **args)
+
+
+def stress_soup(seed=0, **kw) -> Soup:
+    """300K triangles from the unbounded generator, 40% of them in stacked
+    depth layers, 64 views of mixed resolution up to 1920x1080, features and
+    MLP as the 100K-triangle bf16 scene."""
+    args = {"T": 300_000, "views": 64, "resolutions": MIXED_RESOLUTIONS + [(1080, 1920)], "levels": (0, 4),
+            "C": 16, "alpha_level": 4, "dh": 64, "layered": 0.4}
+    args.update(kw)
+    return unbounded_soup(seed, **args)
 
 
 def project_vertices_np(verts, mvp):

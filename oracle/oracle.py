@@ -144,21 +144,27 @@ def multires_alpha(frag_attrs, level, alpha_src):
     return out[:N]
 
 
-def thresholds(frag_pix, frag_attrs, stochastic, seed):
+def thresholds(frag_pix, frag_attrs, stochastic, seed, view_base=0):
+    """Mask thresholds; the hash sees view index frag_pix[:,0] + view_base."""
     N = frag_pix.shape[0]
     out = np.zeros(max(N, 1), np.float32)
-    lib().dso_mask_thresholds(N, _i32(frag_pix), _f32(frag_attrs), int(bool(stochastic)), seed, out)
+    fp = _i32(frag_pix)
+    if view_base:
+        fp = fp.copy()
+        fp[:, 0] += view_base
+    lib().dso_mask_thresholds(N, fp, _f32(frag_attrs), int(bool(stochastic)), seed, out)
     return out[:N]
 
 
-def rasterize(resolution, pos, tri, level, alpha_src, stochastic=True, seed=0, return_frags=False):
-    """rast [B,H,W,4] for one resolution shared by all views."""
+def rasterize(resolution, pos, tri, level, alpha_src, stochastic=True, seed=0, return_frags=False, view_base=0):
+    """rast [B,H,W,4] for one resolution shared by all views; view b of this
+    call draws the mask of view view_base + b."""
     H, W = resolution
     pos = _f32(pos)
     B = pos.shape[0]
     fp, fa = fragments(H, W, pos, tri)
     alpha = multires_alpha(fa, level, alpha_src)
-    thr = thresholds(fp, fa, stochastic, seed)
+    thr = thresholds(fp, fa, stochastic, seed, view_base)
     rast = np.zeros((B, H, W, 4), np.float32)
     lib().dso_depth_test(B, H, W, fp.shape[0], _i32(fp) if fp.size else np.zeros((1, 3), np.int32), _f32(fa) if fa.size else np.zeros((1, 4), np.float32),
                          alpha if alpha.size else np.zeros(1, np.float32), thr if thr.size else np.zeros(1, np.float32), rast)
@@ -288,7 +294,7 @@ def project(verts, mvp):
 
 
 def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, grad_color_in=None):
-    """Full forward + backward of the fused step for equal-resolution views.
+    """Full forward + backward of the fused step (views of any resolution).
 
     Returns a dict with rast, color, grad_color, loss and the gradients of the
     MLP (torch layout), features, world vertices and opacity logits.  With
@@ -298,13 +304,16 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
 
     dtype = dtype or torch.float64
     s = soup
-    H, W = s.resolutions[0]
     B = s.B
+    res = [tuple(r) for r in s.resolutions]
+    off = np.cumsum([0] + [h * w for h, w in res])
     if pos is None:
         pos = project(s.verts, s.mvp)
     if alpha is None:
         alpha = (1.0 / (1.0 + np.exp(-s.alpha_logits.astype(np.float64)))).astype(np.float32)
-    rast = rasterize((H, W), pos, s.tri, s.alpha_level, alpha, True, seed).reshape(-1, 4)
+    # views one at a time (any mix of resolutions), packed like the step's buffers
+    rast = np.concatenate([rasterize((H, W), pos[b:b + 1], s.tri, s.alpha_level, alpha, True, seed,
+                                     view_base=b).reshape(-1, 4) for b, (H, W) in enumerate(res)])
     lo, hi = s.levels
     P = rast.shape[0]
     X = multires_color(rast, lo, hi, s.feat, True).reshape(P, -1)
@@ -317,8 +326,8 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
     for i, (w, b) in enumerate(Ws):
         h = h @ w.T + b
         h = torch.relu(h) if i < len(Ws) - 1 else torch.sigmoid(h)
-    res = Xt[:, 3:4]
-    blend = (1 - res) * Xt[:, :3] + res * h
+    r3 = Xt[:, 3:4]
+    blend = (1 - r3) * Xt[:, :3] + r3 * h
     tgt = torch.tensor(s.target, dtype=dtype)
     color = torch.ones(P, 3, dtype=dtype).index_put((idx,), blend)
     loss = ((color - tgt) ** 2).mean()
@@ -330,10 +339,15 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
     grad_color32 = (2.0 * (color.detach() - tgt) / color.numel()).numpy().astype(np.float32)
     ci = color32 if color_in is None else color_in
     gi = grad_color32 if grad_color_in is None else grad_color_in
-    gpos = edge_grad(ci.reshape(B, H, W, 3), gi.reshape(B, H, W, 3), rast.reshape(B, H, W, 4), pos, s.tri)
+    gpos = np.zeros((B, pos.shape[1], 4), np.float32)
+    ga = np.zeros((s.T, feats_at_level(s.alpha_level)), np.float32)
+    for b, (H, W) in enumerate(res):
+        sl = slice(off[b], off[b + 1])
+        rb = rast[sl].reshape(1, H, W, 4)
+        gpos[b:b + 1] = edge_grad(ci[sl].reshape(1, H, W, 3), gi[sl].reshape(1, H, W, 3), rb, pos[b:b + 1], s.tri)
+        ga += opacity_aux_grad(ci[sl].reshape(1, H, W, 3), s.target[sl].reshape(1, H, W, 3), rb, pos[b:b + 1],
+                               s.tri, s.alpha_level, alpha)
     gverts = np.einsum("bij,bvi->vj", s.mvp.astype(np.float64), gpos.astype(np.float64))[:, :3]
-    ga = opacity_aux_grad(ci.reshape(B, H, W, 3), s.target.reshape(B, H, W, 3), rast.reshape(B, H, W, 4), pos,
-                          s.tri, s.alpha_level, alpha)
     galog = ga / (3 * P) * alpha * (1 - alpha)
     gmlp = np.concatenate([Ws[0][0].grad.numpy().ravel(), Ws[1][0].grad.numpy().ravel(),
                            Ws[2][0].grad.numpy().ravel(), Ws[0][1].grad.numpy(), Ws[1][1].grad.numpy(),
