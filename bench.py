@@ -33,10 +33,13 @@ METRIC = "fwd+bwd iters/s @15K tris, 8x800^2 views"
 PROFILE_MD = os.path.join(ROOT, "profiles", "fused_step_sphere_torus_15k_8x800.md")
 
 
-def stage_kernel(stage, tile):
+def stage_kernel(stage, step):
     """Kernel behind a timed stage of ds_train_step, as named in the rocprofv3
-    summary (template arguments: tile edge or threads per tile)."""
-    return {"resolve": f"k_resolve<{tile},true>", "shade": "k_shade<4,8,32,false>",
+    summary (template arguments: tile edge or threads per tile; shading
+    levels, channels, hidden width and precision)."""
+    tile = step.tile
+    shade = f"k_shade<{step.nl},{step.C},{step.hidden},{'true' if step.bf16 else 'false'}>"
+    return {"resolve": f"k_resolve<{tile},true>", "shade": shade,
             "edge": f"k_edge_grad_tiles<{tile * tile}>", "aux": f"k_opacity_aux<{tile}>"}[stage]
 
 
@@ -54,6 +57,8 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--profile-driver", action="store_true",
                     help="run four fused steps and a 256 MiB calibration kernel (for rocprofv3 --pmc), print nothing")
+    ap.add_argument("--workload", default="soup15k", choices=["soup15k"] + sorted(WORKLOADS),
+                    help="soup15k (default) is the headline measurement; the others print their own line")
     ap.add_argument("--summarize-rocprof", nargs=4, metavar=("KERNEL_STATS_CSV", "FETCH_CSV", "WRITE_CSV", "OUT_MD"),
                     help="write the markdown summary of a kernel trace and two counter runs, then exit")
     return ap.parse_args()
@@ -73,6 +78,9 @@ def main():
     from diffsoup_amd import fused, scenes
 
     dev = torch.device("cuda:0")
+    if args.workload != "soup15k":
+        other_workload(args, dev)
+        return
     soup = scenes.sphere_torus_soup(0)
     if args.profile_driver:
         profile_driver(soup, dev)
@@ -211,11 +219,86 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+WORKLOADS = {
+    "uneven50k": ("batched_uneven_soup", False,
+                  "50K-triangle unbounded scene, 32 views of mixed resolution (1297x840, 979x546, 1237x822, "
+                  "800x800), levels 0-3 x 8 ch fp32, MLP 32-32-32-3 fp32"),
+    "bf16_100k": ("bf16_large_soup", True,
+                  "100K-triangle unbounded scene, 16 views 1920x1080, levels 0-4 x 16 ch bf16 features, "
+                  "MLP 80-64-64-3 on bf16 MFMA with fp32 accumulation"),
+    "stress300k": ("stress_soup", True,
+                   "300K triangles (40% in stacked depth layers), 64 views of mixed resolution up to 1920x1080, "
+                   "bf16 features and MLP as bf16_100k"),
+}
+
+
+def other_workload(args, dev):
+    """One of the larger workloads: the captured training iteration
+    (forward + backward + Adam/VectorAdam written out of place) with the tile
+    chosen by the autotuner, per-stage kernel times and the shading roofline.
+    Prints one JSON line."""
+    import torch
+
+    from diffsoup_amd import fused, scenes
+
+    gen, bf16, desc = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    soup = getattr(scenes, gen)(0)
+    gen_s = time.perf_counter() - t0
+    mvp = torch.from_numpy(soup.mvp).to(dev)
+    target = torch.from_numpy(soup.target).to(dev)
+    step = fused.FusedStep(soup.resolutions, soup.T, soup.verts.shape[0], soup.alpha_level, soup.levels,
+                           soup.feat.shape[2], soup.mlp_dims[1], bf16=bf16, bf16_features=bf16, device=dev)
+    params = fused.SoupParams.from_soup(soup, dev)
+    updated = fused.SoupParams.from_soup(soup, dev)
+    if bf16:
+        params.with_bf16_features()
+        updated.with_bf16_features()
+    tune = step.autotune(params, mvp, target, tiles=(8, 16, 32), reps=2, cache=None)
+    opt = fused.soup_optimizer(step, params, out=updated)
+    cap = fused.CapturedStep(step, params, mvp, target, opt, seed=1000)
+    steps, warmup = min(args.steps, 20), min(args.warmup, 3)
+    for _ in range(warmup):
+        cap.replay()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        cap.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    covered = int(step.stats()[0].item())
+    # per-stage kernel times on the eager step (same buffers, fixed parameters)
+    stages = {}
+    for stage in ("resolve", "shade", "edge", "aux"):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+        for i in range(3):
+            step.time_stage(stage, *evs[i])
+            step.run(params, mvp, target, seed=3000 + i)
+        torch.cuda.synchronize()
+        kms = sum(a.elapsed_time(b) for a, b in evs) / 3
+        step.time_stage(None)
+        stages[stage] = roofline(step, stage, kms, {})
+    out = {"metric": f"fwd+bwd iters/s ({args.workload})", "value": round(1000.0 / ms, 2), "unit": "it/s",
+           "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+           "dtype": "bf16" if bf16 else "f32", "data": "synthetic",
+           "config": {"workload": desc, "triangles": soup.T, "views": soup.B, "pixels_per_step": soup.pixels,
+                      "tile": step.tile, "parallelism": "single GPU"},
+           "autotune_ms_per_step": {str(k): (None if v is None else round(v, 4)) for k, v in tune.items()},
+           "covered_fragments_per_step": covered, "gfrag_per_s": round(covered * 1000.0 / ms / 1e9, 4),
+           "stages": {k: {"avg_launch_us": round(v["avg_launch_ms"] * 1e3, 1), "bound": v["bound"],
+                          "achieved": v["achieved"], "unit": v["unit"], "frac": v["frac"]} for k, v in stages.items()},
+           "scene_generation_s": round(gen_s, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def roofline(step, stage, kms, traffic):
     """Roofline entry of one stage's kernel: shading is compared with the fp32
     MFMA peak (its arithmetic intensity is above the fp32 ridge of 19.7
     flop/B), the raster kernels with HBM bandwidth."""
-    kernel = stage_kernel(stage, step.tile)
+    kernel = stage_kernel(stage, step)
     tr = traffic.get(kernel)
     if stage == "shade":
         flops = step.algorithmic_flops(stage)

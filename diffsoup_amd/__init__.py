@@ -22,6 +22,7 @@ from .multires import (
     rasterize_views,
 )
 from .rasterize import count_triangle_ids, edge_grad, encode_view_dir_sh2, opacity_aux_loss
+from .visibility import build_keep_map, count_visible_triangles
 
 
 def project_vertices(verts, mvp):

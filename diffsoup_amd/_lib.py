@@ -44,6 +44,7 @@ _SIGNATURES = {
     "ds_sh2": ([_i, _ip, _ip, _vp, _vp, _vp, _vp], _i),
     "ds_edge_grad_bwd": ([_i, _ip, _ip, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp], _i),
     "ds_fill_f32": ([_vp, _i64, _f, _vp], _i),
+    "ds_count_triangle_ids": ([_i64, _vp, _i, _vp, _i, _vp], _i),
 }
 
 # Entry points added by later translation units bind when present.

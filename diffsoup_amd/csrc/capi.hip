@@ -85,6 +85,7 @@ int step_make_views(const ds_step_desc* d, Views& v) { return make_views(d->B, d
 int step_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 int plain_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 int optim_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+int visibility_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 }  // namespace ds
 
 extern "C" {

@@ -286,19 +286,20 @@ class FusedStep:
         append their index (4 B), every (view, triangle) record is read once
         (64 B) and every bin entry once (4 B).
         shade: per covered pixel 60 B (rast, index, target, colour, colour
-        gradient, L1) plus one read of the feature table and one write of its
-        gradient.
+        gradient, L1) plus one read of the feature table (fp32 or bf16) and
+        one write of its fp32 gradient.
         edge: rast + colour + colour gradient of every pixel (40 B) and the
         clip-position gradient (16 B per view-vertex).
         """
         P = self.P
         cov = self.covered_pixels()
         pairs = int(self.stats()[1].item())
-        feat_bytes = 4.0 * self.T * self.S_tot * self.C
+        n_feat = float(self.T * self.S_tot * self.C)
+        feat_read = (2.0 if self.bf16_features else 4.0) * n_feat
         if stage == "resolve":
             return 16.0 * P + 40.0 * (P - cov) + 4.0 * cov + 64.0 * self.B * self.T + 4.0 * pairs
         if stage == "shade":
-            return 60.0 * cov + 2.0 * feat_bytes
+            return 60.0 * cov + feat_read + 4.0 * n_feat
         if stage == "edge":
             return 40.0 * P + 16.0 * self.B * self.V
         if stage == "aux":

@@ -135,9 +135,16 @@ def encode_view_dir_sh2(rast: torch.Tensor, inv_mvp: torch.Tensor) -> torch.Tens
 # Utilities (python/diffsoup/rasterize.py:348-366)
 # ---------------------------------------------------------------------------
 
-def count_triangle_ids(rast: torch.Tensor, num_tris: int) -> torch.Tensor:
-    tri_ids = rast[..., -1].long()
-    tri_ids = tri_ids[tri_ids > 0] - 1
-    count = torch.bincount(tri_ids, minlength=num_tris)
-    assert count.shape[0] == num_tris
-    return count
+def count_triangle_ids(rast: torch.Tensor, num_tris: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Pixels per triangle in a raster (python/diffsoup/rasterize.py:348-366):
+    the bincount of the 1-based ids in rast[..., 3], as a (num_tris,) int64
+    tensor.  With ``out`` the counts are added to it (ds_count_triangle_ids)."""
+    assert rast.shape[-1] == 4 and rast.dtype == torch.float32 and rast.is_contiguous()
+    _lib.require_device(rast, "rast")
+    acc = out is not None
+    if out is None:
+        out = torch.empty(num_tris, dtype=torch.int64, device=rast.device)
+    assert out.shape == (num_tris,) and out.dtype == torch.int64 and out.is_contiguous()
+    _lib.check(_lib.lib().ds_count_triangle_ids(rast.numel() // 4, _lib.ptr(rast), int(num_tris), _lib.ptr(out),
+                                                int(acc), _lib.stream_ptr(rast.device)), "ds_count_triangle_ids")
+    return out

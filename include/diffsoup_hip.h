@@ -327,6 +327,13 @@ typedef struct ds_adam_group {
 /* At most 16 groups. */
 int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream);
 
+/* Per-triangle pixel counts of a raster rast [P,4] (channel 3 = triangle id
+ * + 1, 0 = background): counts [T] int64 receives the bincount of the ids
+ * (added to the existing values when accumulate != 0, else overwritten).
+ * Replaces count_triangle_ids (python/diffsoup/rasterize.py:348-366), the
+ * kernel of count_visible_triangles (examples/utils.py:445-482). */
+int ds_count_triangle_ids(int64_t P, const float* rast, int T, int64_t* counts, int accumulate, void* stream);
+
 /* Fill n floats with value (a kernel, so it is safe inside graph capture). */
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 
