@@ -10,6 +10,13 @@
 
 namespace ds {
 
+// Orders this wave's LDS writes before its later LDS reads (lanes of one
+// wave exchanging data through LDS; no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 constexpr int kMaxViews = 64;
 
 // Per-launch description of the views packed into one batch.  Passed to the

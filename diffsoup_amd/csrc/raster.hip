@@ -206,6 +206,72 @@ __device__ __forceinline__ bool in_rect(int4 r, int px, int py) {
     return px >= r.z && px < r.w && py >= r.x && py < r.y;
 }
 
+// Dense (triangle, pixel) candidates of one wave's 8x8 quad.  The staged
+// triangles are taken 32 at a time; lane i < 32 intersects triangle g0 + i's
+// rectangle with the quad, a wave scan of those areas numbers the candidate
+// pixels, and each pass hands one candidate to every lane (the triangle by a
+// binary search over the prefix, the pixel row-major inside the
+// intersection).  Every in-rectangle pixel of every triangle is visited
+// exactly once, as the per-pixel walk over the bin visits it, but the lanes
+// stay busy: a triangle covers ~10 of a quad's 64 pixels.
+struct CandGroup {
+    int start[32];    // exclusive prefix of the candidate counts
+    int4 box[32];     // (x0, y0, width, staged index) of each intersection
+    float ndcx[8];    // pixel-centre NDC of the quad's columns / rows
+    float ndcy[8];
+};
+
+__device__ __forceinline__ void cand_setup(CandGroup& c, const TileWalk& w, int lane) {
+    if (lane < 8) c.ndcx[lane] = w.ndc_x;
+    if (lane % 8 == 0) c.ndcy[lane / 8] = w.ndc_y;
+}
+
+template <int TS, class F>
+__device__ __forceinline__ void for_each_candidate(const TileStage<TS>& st, int m, const TileWalk& w, CandGroup& c,
+                                                   int lane, F&& f) {
+    for (int g0 = 0; g0 < m; g0 += 32) {
+        int area = 0, ix0 = 0, iy0 = 0, wd = 1;
+        const int j = g0 + lane;
+        if (lane < 32 && j < m) {
+            const int4 r = st.rect[j];
+            ix0 = max(r.z, w.qx0);
+            iy0 = max(r.x, w.qy0);
+            const int ix1 = min(r.w, w.qx0 + 8), iy1 = min(r.y, w.qy0 + 8);
+            if (ix1 > ix0 && iy1 > iy0) {
+                wd = ix1 - ix0;
+                area = wd * (iy1 - iy0);
+            }
+        }
+        int inc = area;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int o = __shfl_up(inc, d, kWave);
+            if (lane >= d) inc += o;
+        }
+        const int total = __shfl(inc, 31, kWave);
+        if (lane < 32) {
+            c.start[lane] = inc - area;
+            c.box[lane] = make_int4(ix0, iy0, wd, j);
+        }
+        wave_lds_sync();
+        for (int k0 = 0; k0 < total; k0 += kWave) {
+            const int k = k0 + lane;
+            if (k < total) {
+                int lo = 0, hi = 31;   // last entry whose candidates start at or before k
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (c.start[mid] <= k) lo = mid; else hi = mid - 1;
+                }
+                const int4 bx = c.box[lo];
+                const int local = k - c.start[lo];
+                const int dy = local / bx.z, dx = local - dy * bx.z;
+                const int px = bx.x + dx, py = bx.y + dy;
+                f(bx.w, px, py, (py - w.qy0) * 8 + (px - w.qx0), c.ndcx[px - w.qx0], c.ndcy[py - w.qy0]);
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
 __device__ __forceinline__ long long wave_sum_ll(long long x) {
     for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
     return x;
@@ -223,10 +289,16 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
     __shared__ float wave_loss[NW];
     __shared__ int wave_cnt[NW];
     __shared__ int wave_cov[NW];
+    __shared__ CandGroup cg[NW];
+    __shared__ unsigned long long zb[NW][kWave];   // per-wave z-buffer of packed keys
     const TileWalk w = tile_walk<TS>(v, T, tile_off, stats);
     const unsigned long long seed = mseed.device ? *mseed.device : mseed.value;
+    const int wl = threadIdx.x % kWave, wq = threadIdx.x / kWave;
+    CandGroup& cgw = cg[wq];
+    unsigned long long* zw = zb[wq];
+    cand_setup(cgw, w, wl);
+    zw[wl] = static_cast<unsigned long long>(kKeyEmpty);
 
-    long long best = kKeyEmpty;
     float bb0 = 0.f, bb1 = 0.f, bd = 0.f;
     int bt = -1;
     long long covered = 0;
@@ -236,25 +308,35 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
         __syncthreads();
         stage_chunk<TS>(st, w, c0, m, T, bins, recs);
         __syncthreads();
-        for (int j = 0; j < m; ++j) {
-            const int4 r = st.rect[j];
-            if (!quad_hit(r, w.qx0, w.qy0)) continue;
-            if (!in_rect(r, w.px, w.py)) continue;
+        for_each_candidate<TS>(st, m, w, cgw, wl, [&](int j, int px, int py, int pi, float ndc_x, float ndc_y) {
             Hit h;
-            if (!cover(w.ndc_y, w.ndc_x, st.p0[j], st.p1[j], st.p2[j], h)) continue;
+            if (!cover(ndc_y, ndc_x, st.p0[j], st.p1[j], st.p2[j], h)) return;
             float d;
-            if (!fragment_depth(h, d)) continue;
+            if (!fragment_depth(h, d)) return;
             ++covered;
-            if (!depth_testable(d)) continue;
+            if (!depth_testable(d)) return;
             const int t = st.tri[j];
-            const long long key = depth_key(d, static_cast<uint32_t>(t));
-            if (key >= best) continue;  // cannot win whether or not the mask keeps it
+            const unsigned long long key = static_cast<unsigned long long>(depth_key(d, static_cast<uint32_t>(t)));
+            // keys are positive, so unsigned order is the signed order; a key
+            // not below the current minimum cannot win whatever the mask says
+            if (key >= zw[pi]) return;
             const float alpha = lattice_alpha(a, t, h.b0, h.b1);
-            const float u = mask_mode == kMaskHash
-                                ? mask_threshold(seed, w.b, t, w.py, w.px)
-                                : 0.5f;
-            if (alpha < u) continue;
-            best = key; bb0 = h.b0; bb1 = h.b1; bd = d; bt = t;
+            const float u = mask_mode == kMaskHash ? mask_threshold(seed, w.b, t, py, px) : 0.5f;
+            if (alpha < u) return;
+            atomicMin(&zw[pi], key);
+        });
+    }
+    wave_lds_sync();
+    {   // this lane's pixel: the winning triangle's hit, recomputed exactly as in the walk
+        const unsigned long long kb = zw[wl];
+        if (kb != static_cast<unsigned long long>(kKeyEmpty)) {
+            bt = static_cast<int>(kb & 0xffffffffull);
+            const TriRecord& r = recs[(long long)w.b * T + bt];
+            Hit h;
+            cover(w.ndc_y, w.ndc_x, r.p0, r.p1, r.p2, h);
+            fragment_depth(h, bd);
+            bb0 = h.b0;
+            bb1 = h.b1;
         }
     }
 
@@ -389,17 +471,29 @@ k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
               const long long* __restrict__ stats, const float4* __restrict__ rast,
               const float* __restrict__ pix_loss, float* __restrict__ grad_alpha) {
     constexpr int NT = TS * TS;
+    constexpr int NW = NT / kWave;
     __shared__ TileStage<TS> st;
+    __shared__ CandGroup cg[NW];
+    __shared__ float pix_depth[NW][kWave], pix_l1[NW][kWave];
+    __shared__ int pix_tri[NW][kWave];
     const TileWalk w = tile_walk<TS>(v, T, tile_off, stats);
-    const bool inside_view = w.px < v.W[w.b] && w.py < v.H[w.b];
-    float depth_pix = 0.f, loss = 0.f;
-    int tri_pix = -1;
-    if (inside_view) {
-        const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
-        const float4 rp = rast[p];
-        depth_pix = rp.z;
-        tri_pix = static_cast<int>(rp.w) - 1;
-        loss = pix_loss[p];
+    const int wl = threadIdx.x % kWave, wq = threadIdx.x / kWave;
+    CandGroup& cgw = cg[wq];
+    cand_setup(cgw, w, wl);
+    {
+        const bool inside_view = w.px < v.W[w.b] && w.py < v.H[w.b];
+        float depth_pix = 0.f, loss = 0.f;
+        int tri_pix = -1;
+        if (inside_view) {
+            const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
+            const float4 rp = rast[p];
+            depth_pix = rp.z;
+            tri_pix = static_cast<int>(rp.w) - 1;
+            loss = pix_loss[p];
+        }
+        pix_depth[wq][wl] = depth_pix;
+        pix_l1[wq][wl] = loss;
+        pix_tri[wq][wl] = tri_pix;
     }
     constexpr float eps = 1e-5f;
     for (int c0 = 0; c0 < w.n; c0 += NT) {
@@ -407,15 +501,14 @@ k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
         __syncthreads();
         stage_chunk<TS>(st, w, c0, m, T, bins, recs);
         __syncthreads();
-        for (int j = 0; j < m; ++j) {
-            const int4 r = st.rect[j];
-            if (!quad_hit(r, w.qx0, w.qy0)) continue;
-            if (!in_rect(r, w.px, w.py)) continue;
+        for_each_candidate<TS>(st, m, w, cgw, wl, [&](int j, int px, int py, int pi, float ndc_x, float ndc_y) {
             Hit h;
-            if (!cover(w.ndc_y, w.ndc_x, st.p0[j], st.p1[j], st.p2[j], h)) continue;
+            if (!cover(ndc_y, ndc_x, st.p0[j], st.p1[j], st.p2[j], h)) return;
             float d;
-            if (!fragment_depth(h, d)) continue;
-            if (tri_pix >= 0 && d > depth_pix) continue;
+            if (!fragment_depth(h, d)) return;
+            const int tri_pix = pix_tri[wq][pi];
+            if (tri_pix >= 0 && d > pix_depth[wq][pi]) return;
+            const float loss = pix_l1[wq][pi];
             const int t = st.tri[j];
             const float alpha = lattice_alpha(a, t, h.b0, h.b1);
             float g;
@@ -430,7 +523,7 @@ k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
             unsafeAtomicAdd(&dst[c.idx[0]], g * c.w[0]);
             unsafeAtomicAdd(&dst[c.idx[1]], g * c.w[1]);
             unsafeAtomicAdd(&dst[c.idx[2]], g * c.w[2]);
-        }
+        });
     }
 }
 

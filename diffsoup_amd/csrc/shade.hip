@@ -48,10 +48,6 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Compile-time shape of one shading configuration.
 template <int NL_, int C_, int DH_, bool BF16_>
