@@ -40,7 +40,7 @@ def stage_kernel(stage, step):
     tile = step.tile
     shade = f"k_shade<{step.nl},{step.C},{step.hidden},{'true' if step.bf16 else 'false'}>"
     return {"resolve": f"k_resolve<{tile},true>", "shade": shade,
-            "edge": f"k_edge_grad_tiles<{tile * tile}>", "aux": f"k_opacity_aux<{tile}>"}[stage]
+            "edge": "k_edge_grad_sorted", "aux": f"k_opacity_aux<{tile}>"}[stage]
 
 
 def parse():

@@ -133,17 +133,12 @@ hipError_t launch_edge_grad(const Views& v, int C, const float* color, const flo
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
                                       const float* rast, const int* tri_off, int4* list, hipStream_t s);
 
-hipError_t launch_edge_grad_list(const Views& v, int C, const float* color, const float* grad_color,
-                                 const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                 const int* tri, const int* list, const int* n_list, long long max_n,
-                                 hipStream_t s);
-
-// Same gradients over the per-tile covered lists of the training-step
-// resolve (raster order within a tile keeps neighbours in cache).
-hipError_t launch_edge_grad_tiles(const Views& v, int C, const float* color, const float* grad_color,
-                                  const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                  const int* tri, const TriRecord* recs, const int2* tile_list,
-                                  const int* tile_ncov, hipStream_t s);
+// Over the triangle-grouped covered-pixel list (int4 (pixel, b0, b1, t)),
+// *n_list entries (read on the device, at most max_n).
+hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, const float* grad_color,
+                                   const float* rast, int V, const float* pos, float* grad_pos, int T,
+                                   const int* tri, const TriRecord* recs, const int4* list, const int* n_list,
+                                   long long max_n, hipStream_t s);
 
 hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long long n, float value, hipStream_t s);

@@ -8,6 +8,8 @@
 // geometric edge of the pixel's triangle.  This follows
 // src/cuda/rasterize.cu:489-903 case for case, with correctly rounded
 // 1/sqrt in place of the approximate reciprocal square root.
+#include <algorithm>
+
 #include "ds_kernels.h"
 #include "ds_math.h"
 
@@ -311,113 +313,71 @@ __global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ col
     edge_at<false>(v, T, C, color, grad_color, rast, V, pos, grad_pos, tri, nullptr, p0, -1);
 }
 
-// Same gradients over a list of covered pixels (n_list read on the device).
-__global__ void k_edge_grad_list(Views v, int T, int C, const float* __restrict__ color,
-                                 const float* __restrict__ grad_color, const float4* __restrict__ rast,
-                                 int V, const float4* __restrict__ pos, float* __restrict__ grad_pos,
-                                 const int* __restrict__ tri, const int* __restrict__ list,
-                                 const int* __restrict__ n_list) {
+// Edge gradients over the covered pixels in the triangle-grouped list that
+// the shading kernel consumes (records (pixel, b0, b1, triangle)); the count
+// is read on the device.  Consecutive entries mostly share a triangle and a
+// view, so a wave groups its lanes by (view, gradient triangle), adds each
+// lane's 12 clip-space components into its group's LDS slot, and flushes one
+// global atomic per (group, component).  A persistent grid strides over the
+// list, so every lane of a wave has a pixel except in the last stride.
+__global__ void __launch_bounds__(256)
+k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const float* __restrict__ grad_color,
+                   const float4* __restrict__ rast, int V, const float4* __restrict__ pos,
+                   float* __restrict__ grad_pos, const int* __restrict__ tri, const TriRecord* __restrict__ recs,
+                   const int4* __restrict__ list, const int* __restrict__ n_list) {
+    __shared__ float slot[4][64][13];
+    __shared__ long long sv[4][64][3];
     const int n = *n_list;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        edge_at<false>(v, T, C, color, grad_color, rast, V, pos, grad_pos, tri, nullptr, list[i], -1);
-}
-
-// One workgroup per screen tile, one lane per covered pixel of the tile.
-template <int NT>
-__global__ void __launch_bounds__(NT)
-k_edge_grad_tiles(Views v, int T, int C, const float* __restrict__ color, const float* __restrict__ grad_color,
-                  const float4* __restrict__ rast, int V, const float4* __restrict__ pos,
-                  float* __restrict__ grad_pos, const int* __restrict__ tri, const TriRecord* __restrict__ recs,
-                  const int2* __restrict__ tile_list, const int* __restrict__ tile_ncov) {
-    const int n = tile_ncov[blockIdx.x];
-    const int wave_first = (threadIdx.x / 64) * 64;
-    if (wave_first >= n) return;          // whole wave idle (wave-uniform exit)
-    int b = 0;
-    while (b + 1 < v.n && v.tile_base[b + 1] <= static_cast<int>(blockIdx.x)) ++b;
-    const int lane = threadIdx.x % 64;
-    float g[12];
-    int v0[3] = {0, 0, 0};
-    int it0 = -1;
-    bool act = false;
-    if (static_cast<int>(threadIdx.x) < n) {
-        const long long p0 = tile_list[(long long)blockIdx.x * NT + threadIdx.x].x;
-        act = edge_eval<true>(v, T, C, color, grad_color, rast, V, pos, tri, recs, p0, b, g, v0, it0);
-    }
-    // Pixels of one triangle add into the same 12 clip-space components.
-    // Group the wave's lanes by triangle, add each lane's components into
-    // its group's LDS slot, then flush: one global atomic per (group,
-    // component), spread over distinct lanes.
-    __shared__ float slot[NT / 64][64][13];
-    __shared__ int sv[NT / 64][64][3];
-    const int w = threadIdx.x / 64;
-    int grp = -1, ng = 0;
-    unsigned long long todo = __ballot(act);
-    while (todo) {
-        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
-        const int key = __shfl(it0, leader, 64);
-        const unsigned long long same = __ballot(act && it0 == key);
-        if (act && it0 == key) grp = ng;
-        if (lane == leader) { sv[w][ng][0] = v0[0]; sv[w][ng][1] = v0[1]; sv[w][ng][2] = v0[2]; }
-        ++ng;
-        todo &= ~same;
-    }
-    for (int i = lane; i < ng * 13; i += 64) (&slot[w][0][0])[i] = 0.f;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (act)
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const long long stride = (long long)gridDim.x * 256;
+    // the loop bound depends on the wave's first index only: wave-uniform
+    for (long long base = (long long)blockIdx.x * 256 + w * 64; base < n; base += stride) {
+        const long long i = base + lane;
+        float g[12];
+        int v0[3] = {0, 0, 0};
+        int it0 = -1, b = -1;
+        bool act = false;
+        if (i < n) act = edge_eval<true>(v, T, C, color, grad_color, rast, V, pos, tri, recs, list[i].x, b, g, v0, it0);
+        const long long key = act ? (long long)b * T + it0 : -1;
+        int grp = -1, ng = 0;
+        unsigned long long todo = __ballot(act);
+        while (todo) {
+            const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+            const long long k = __shfl(key, leader, 64);
+            const unsigned long long same = __ballot(act && key == k);
+            if (act && key == k) grp = ng;
+            if (lane == leader)
+                for (int c = 0; c < 3; ++c) sv[w][ng][c] = (long long)b * V + v0[c];
+            ++ng;
+            todo &= ~same;
+        }
+        for (int q = lane; q < ng * 13; q += 64) (&slot[w][0][0])[q] = 0.f;
+        wave_lds_sync();
+        if (act)
 #pragma unroll
-        for (int k = 0; k < 12; ++k)
-            if (g[k] != 0.f) atomicAdd(&slot[w][grp][k], g[k]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    float* gp = grad_pos + (long long)b * V * 4;
-    for (int i = lane; i < ng * 12; i += 64) {
-        const int q = i / 12, k = i - q * 12;
-        const float x = slot[w][q][k];
-        if (x != 0.f) unsafeAtomicAdd(&gp[sv[w][q][k / 4] * 4 + (k % 4)], x);
+            for (int k = 0; k < 12; ++k)
+                if (g[k] != 0.f) atomicAdd(&slot[w][grp][k], g[k]);
+        wave_lds_sync();
+        for (int q = lane; q < ng * 12; q += 64) {
+            const int gq = q / 12, k = q - gq * 12;
+            const float x = slot[w][gq][k];
+            if (x != 0.f) unsafeAtomicAdd(&grad_pos[sv[w][gq][k / 4] * 4 + (k % 4)], x);
+        }
+        wave_lds_sync();   // the slots are reused by the next stride
     }
 }
 
 }  // namespace
 
-hipError_t launch_edge_grad_tiles(const Views& v, int C, const float* color, const float* grad_color,
-                                  const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                  const int* tri, const TriRecord* recs, const int2* tile_list,
-                                  const int* tile_ncov, hipStream_t s) {
-    const int n_tiles = v.tile_base[v.n];
-    if (n_tiles == 0) return hipSuccess;
-    const float4* r4 = reinterpret_cast<const float4*>(rast);
-    const float4* p4 = reinterpret_cast<const float4*>(pos);
-    switch (v.tile) {
-        case 8:
-            k_edge_grad_tiles<64><<<n_tiles, 64, 0, s>>>(v, T, C, color, grad_color, r4, V, p4, grad_pos, tri,
-                                                         recs, tile_list, tile_ncov);
-            break;
-        case 16:
-            k_edge_grad_tiles<256><<<n_tiles, 256, 0, s>>>(v, T, C, color, grad_color, r4, V, p4, grad_pos, tri,
-                                                           recs, tile_list, tile_ncov);
-            break;
-        case 32:
-            k_edge_grad_tiles<1024><<<n_tiles, 1024, 0, s>>>(v, T, C, color, grad_color, r4, V, p4, grad_pos, tri,
-                                                             recs, tile_list, tile_ncov);
-            break;
-        default:
-            return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_edge_grad_list(const Views& v, int C, const float* color, const float* grad_color,
-                                 const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                 const int* tri, const int* list, const int* n_list, long long max_n,
-                                 hipStream_t s) {
+hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, const float* grad_color,
+                                   const float* rast, int V, const float* pos, float* grad_pos, int T,
+                                   const int* tri, const TriRecord* recs, const int4* list, const int* n_list,
+                                   long long max_n, hipStream_t s) {
     if (max_n <= 0) return hipSuccess;
-    long long blocks = (max_n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    k_edge_grad_list<<<static_cast<unsigned>(blocks), 256, 0, s>>>(
-        v, T, C, color, grad_color, reinterpret_cast<const float4*>(rast), V,
-        reinterpret_cast<const float4*>(pos), grad_pos, tri, list, n_list);
+    const long long blocks = std::min<long long>((max_n + 255) / 256, 2048);
+    k_edge_grad_sorted<<<static_cast<unsigned>(blocks), 256, 0, s>>>(
+        v, T, C, color, grad_color, reinterpret_cast<const float4*>(rast), V, reinterpret_cast<const float4*>(pos),
+        grad_pos, tri, recs, list, n_list);
     return hipGetLastError();
 }
 

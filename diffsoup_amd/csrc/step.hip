@@ -299,8 +299,8 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade");
     DS_TRY(mark(DS_STAGE_SHADE, false), "event");
     DS_TRY(mark(DS_STAGE_EDGE, true), "event");
-    DS_TRY(launch_edge_grad_tiles(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri,
-                                  rw.recs, tile_list, tile_ncov, s),
+    DS_TRY(launch_edge_grad_sorted(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri,
+                                   rw.recs, list, tri_off + d->T, L.P, s),
            "edge grad");
     DS_TRY(mark(DS_STAGE_EDGE, false), "event");
     DS_TRY(mark(DS_STAGE_AUX, true), "event");

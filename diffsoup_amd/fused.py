@@ -288,7 +288,8 @@ class FusedStep:
         shade: per covered pixel 60 B (rast, index, target, colour, colour
         gradient, L1) plus one read of the feature table (fp32 or bf16) and
         one write of its fp32 gradient.
-        edge: rast + colour + colour gradient of every pixel (40 B) and the
+        edge: per covered pixel its list record (16 B) and the rast, colour
+        and colour gradient (40 B) of itself and its 4 neighbours, plus the
         clip-position gradient (16 B per view-vertex).
         """
         P = self.P
@@ -301,7 +302,7 @@ class FusedStep:
         if stage == "shade":
             return 60.0 * cov + feat_read + 4.0 * n_feat
         if stage == "edge":
-            return 40.0 * P + 16.0 * self.B * self.V
+            return (16.0 + 5 * 40.0) * cov + 16.0 * self.B * self.V
         if stage == "aux":
             return 16.0 * P + 4.0 * P + 64.0 * self.B * self.T + 4.0 * pairs + 8.0 * self.T * self.Sa
         raise ValueError(stage)
