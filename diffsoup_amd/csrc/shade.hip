@@ -48,6 +48,17 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
 
+// Sum over the 16 lanes of this lane's DPP row (lanes 16 r .. 16 r + 15),
+// every lane receiving the total: four row rotations (row_ror 8, 4, 2, 1)
+// on the VALU instead of LDS permutes.  All lanes must be active.
+__device__ __forceinline__ float row16_sum(float x) {
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x128, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x124, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x122, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x121, 0xf, 0xf, false));
+    return x;
+}
+
 
 // Compile-time shape of one shading configuration.
 template <int NL_, int C_, int DH_, bool BF16_>
@@ -403,7 +414,7 @@ k_shade(ShadeArgs a, int rowp) {
             const int S = coarse ? static_cast<int>(feats_at_level(a.lo + lv)) : 0;
             for (int vtx = 0; vtx < 6; ++vtx)
                 for (int r = 0; r < 4; ++r)
-                    for (int d = 8; d > 0; d >>= 1) racc[vtx][r] += __shfl_xor(racc[vtx][r], d, 16);
+                    racc[vtx][r] = row16_sum(racc[vtx][r]);
             wave_lds_sync();
             if (coarse && j == 0) {
                 float* gl = grow + loff[lv] * C + c;
@@ -430,13 +441,11 @@ k_shade(ShadeArgs a, int rowp) {
 
     // ---- reduce the bias gradients over the 16 pixels of each lane group
     for (int m = 0; m < NH; ++m)
-        for (int r = 0; r < 4; ++r)
-            for (int d = 8; d > 0; d >>= 1) {
-                gb1[m][r] += __shfl_xor(gb1[m][r], d, 16);
-                gb2[m][r] += __shfl_xor(gb2[m][r], d, 16);
-            }
-    for (int r = 0; r < 4; ++r)
-        for (int d = 8; d > 0; d >>= 1) gb3[r] += __shfl_xor(gb3[r], d, 16);
+        for (int r = 0; r < 4; ++r) {
+            gb1[m][r] = row16_sum(gb1[m][r]);
+            gb2[m][r] = row16_sum(gb2[m][r]);
+        }
+    for (int r = 0; r < 4; ++r) gb3[r] = row16_sum(gb3[r]);
     for (int d = 32; d > 0; d >>= 1) loss += __shfl_xor(loss, d, kWave);
 
     // ---- workgroup reduction into LDS (reusing the staging area), one partial per block
