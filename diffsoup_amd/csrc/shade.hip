@@ -32,6 +32,8 @@
 #include "ds_math.h"
 #include "ds_shade.h"
 
+#include <type_traits>
+
 namespace ds {
 namespace {
 
@@ -72,24 +74,70 @@ struct Cfg {
     // blob layer table: (out tiles, in tiles) for L1, L2, L3, L3T, L2T, L1T
     static constexpr int L1 = 0, L2 = NH * NI, L3 = L2 + NH * NH, L3T = L3 + NH, L2T = L3T + NH,
                          L1T = L2T + NH * NH, NTILES = L1T + NI * NH;
-    // operand units of one (m, t) tile pair: 4 f32 steps or one bf16 step of 4 values
-    static constexpr int TILE_ELEMS = 4 * kWave;
+    // operand units of one (m, t) tile pair, in floats: 4 f32 steps of 64
+    // lanes, or one bf16 step of 64 lanes x 4 bf16 (128 floats' worth)
+    static constexpr int TILE_ELEMS = (BF16 ? 2 : 4) * kWave;
     static constexpr int BLOB_ELEMS = NTILES * TILE_ELEMS;
-    // the weight operands are staged in LDS when they are small; the large
-    // configurations read them from global memory (L2-resident, 80 KB at
-    // 80->64->64->3) and leave the LDS to the feature rows
-    static constexpr bool BLOB_LDS = BLOB_ELEMS * 4 <= 32 * 1024;
-    static constexpr int LDS_BLOB = BLOB_LDS ? BLOB_ELEMS : 0;
     static constexpr int NPARAM = DH * DIN + DH * DH + 3 * DH + DH + DH + 3 + 1;
-    // per-wave LDS staging for the weight-gradient transposes (floats): one
-    // (delta, activation) pair at a time - (D1, X), (D2, H1), (D3, H2)
-    static constexpr int SX = DINP + 4, SH = DH + 4, SD3 = 16 + 4;
-    static constexpr int STAGE = 16 * ((SH + SX > 2 * SH ? SH + SX : 2 * SH) > SD3 + SH
-                                           ? (SH + SX > 2 * SH ? SH + SX : 2 * SH)
-                                           : SD3 + SH);
+    // per-wave LDS staging for the weight-gradient transposes: one (delta,
+    // activation) pair at a time - (D1, X), (D2, H1), (D3, H2).  The f32
+    // products stage floats as [pixel][unit] rows (width + 4 apart); the bf16
+    // products stage bf16 values (their MFMA operands are rounded to bf16
+    // anyway) as [unit][pixel] rows of SPX, so that a lane's four K values
+    // (pixels 4g .. 4g+3) are one 8-byte read.
+    using ST = typename std::conditional<BF16, unsigned short, float>::type;
+    static constexpr int SPX = 20;
+    static constexpr int region(int width) { return BF16 ? width * SPX : 16 * (width + 4); }
+    static constexpr int STAGE_ = region(DH) + region(DINP) > 2 * region(DH) ? region(DH) + region(DINP)
+                                                                              : 2 * region(DH);
+    static constexpr int STAGE = STAGE_ > region(16) + region(DH) ? STAGE_ : region(16) + region(DH);
     static_assert(C % 4 == 0, "channels per level must be a multiple of 4");
     static_assert(DH % 16 == 0, "hidden width must be a multiple of 16");
 };
+
+constexpr int up16(int b) { return (b + 15) & ~15; }
+
+// Feature-row length (floats) of levels 0 .. nl-1 with c channels.
+constexpr int row_floats_from_zero(int nl, int c) {
+    int s = 0;
+    for (int l = 0; l < nl; ++l) s += l == 0 ? 3 : ((1 << (l - 1)) + 1) * ((1 << l) + 1);
+    return s * c;
+}
+
+// LDS plan of one configuration and feature-row precision: the weight
+// operands, then per wave the feature row (kept in bf16 when the features are
+// given in bf16 - they are widened exactly on use), the fp32 gradient row and
+// the transpose stage.  The weight operands go to LDS when the whole plan
+// fits in 160 KB for rows starting at level 0; otherwise they are read from
+// global memory, where they stay L2-resident.
+template <class K, bool FB>
+struct Plan {
+    using FT = typename std::conditional<FB, unsigned short, float>::type;
+    __host__ __device__ static constexpr int frow_bytes(int rowp) { return up16(rowp * static_cast<int>(sizeof(FT))); }
+    __host__ __device__ static constexpr int wave_bytes(int rowp) {
+        return frow_bytes(rowp) + 4 * rowp + up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)));
+    }
+    static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::C) + 3) & ~3;
+    static constexpr bool LB = 4 * K::BLOB_ELEMS + kWavesPerBlock * wave_bytes(ROW0) <= 160 * 1024;
+    static constexpr int BLOB_BYTES = LB ? 4 * K::BLOB_ELEMS : 0;
+};
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float4 ld4(const unsigned short* p) {
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                       __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+}
+// Stage lane (g, j)'s four values (units u0 .. u0+3 of pixel j) into a
+// transpose region of `width` units.
+template <class K>
+__device__ __forceinline__ void stage_put(typename K::ST* reg, int width, int u0, int j, f32x4 v) {
+    if constexpr (K::BF16) {
+        for (int r = 0; r < 4; ++r) reg[(u0 + r) * K::SPX + j] = bf16_rne(v[r]);
+    } else {
+        *reinterpret_cast<float4*>(reg + j * (width + 4) + u0) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
 
 // One MFMA step: acc += A * B on a 16x16 tile.  For f32 an operand unit is a
 // single float (K = 4 per step over 4 lane groups), for bf16 four values
@@ -134,24 +182,31 @@ struct Mma<true> {
         const i16x4 a = reinterpret_cast<const i16x4*>(blob)[lane];
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, pack(in), acc, 0, 0, 0);
     }
-    __device__ static __forceinline__ f32x4 outer(const float* A, int sa, const float* Bm, int sb, int lane,
-                                                  f32x4 acc) {
+    // staged bf16 values, [unit][pixel] rows of spx: A(i, k) = At[i][k], B(k, j) = Bt[j][k]
+    __device__ static __forceinline__ f32x4 outer(const unsigned short* At, const unsigned short* Bt, int spx,
+                                                  int lane, f32x4 acc) {
         const int i = lane % 16, g = lane / 16;
-        f32x4 a, b;
-        for (int e = 0; e < 4; ++e) {
-            a[e] = A[(4 * g + e) * sa + i];
-            b[e] = Bm[(4 * g + e) * sb + i];
-        }
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pack(a), pack(b), acc, 0, 0, 0);
+        const i16x4 a = *reinterpret_cast<const i16x4*>(At + i * spx + 4 * g);
+        const i16x4 b = *reinterpret_cast<const i16x4*>(Bt + i * spx + 4 * g);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
 };
 
-// Offsets (in floats) of a layer tile pair inside the LDS blob copy.  A bf16
-// tile pair holds 64 lanes x 4 bf16 = 128 floats' worth of storage but is
-// laid out in a 256-float slot so that both precisions share the table.
+// Offset (in floats) of a layer tile pair inside the blob table.
 template <class K>
 __device__ __forceinline__ const float* blob_at(const float* blobs, int layer_base, int m, int nin, int t) {
     return blobs + (layer_base + m * nin + t) * K::TILE_ELEMS;
+}
+
+// acc += (delta units ua .. ua+15) x (activation units ub .. ub+15)^T over
+// the 16 staged pixels; A, B are transpose regions of widths wa, wb.
+template <class K>
+__device__ __forceinline__ f32x4 stage_outer(const typename K::ST* A, int wa, int ua, const typename K::ST* B,
+                                             int wb, int ub, int lane, f32x4 acc) {
+    if constexpr (K::BF16)
+        return Mma<true>::outer(A + ua * K::SPX, B + ub * K::SPX, K::SPX, lane, acc);
+    else
+        return Mma<false>::outer(A + ua, wa + 4, B + ub, wb + 4, lane, acc);
 }
 
 // y[m] = bias + sum_t W(m, t) * in[t] for NOUT output tiles and NIN input tiles.
@@ -163,21 +218,25 @@ __device__ __forceinline__ void dense(const float* blobs, int base, int lane, co
             out[m] = Mma<K::BF16>::tile(blob_at<K>(blobs, base, m, NIN, t), lane, in[t], out[m]);
 }
 
-template <class K>
+template <class K, bool FB>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 k_shade(ShadeArgs a, int rowp) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
+    using P = Plan<K, FB>;
+    using FT = typename P::FT;
+    using ST = typename K::ST;
     extern __shared__ float smem[];
-    const float* blobs = K::BLOB_LDS ? smem : a.blobs;
+    const float* blobs = P::LB ? smem : a.blobs;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
-    float* frow = smem + K::LDS_BLOB + wave * (2 * rowp + K::STAGE);
-    float* grow = frow + rowp;
-    float* stage = grow + rowp;
+    unsigned char* wbase = reinterpret_cast<unsigned char*>(smem) + P::BLOB_BYTES + wave * P::wave_bytes(rowp);
+    FT* frow = reinterpret_cast<FT*>(wbase);
+    float* grow = reinterpret_cast<float*>(wbase + P::frow_bytes(rowp));
+    ST* stage = reinterpret_cast<ST*>(wbase + P::frow_bytes(rowp) + 4 * rowp);
     const int row = a.S_tot * C;
 
     // stage the weight blobs
-    if constexpr (K::BLOB_LDS) {
+    if constexpr (P::LB) {
         const float4* src = reinterpret_cast<const float4*>(a.blobs);
         float4* dst = reinterpret_cast<float4*>(smem);
         for (int i = threadIdx.x; i < K::BLOB_ELEMS / 4; i += blockDim.x) dst[i] = src[i];
@@ -226,9 +285,14 @@ k_shade(ShadeArgs a, int rowp) {
         const bool owned = tb >= r0 && te <= r1;
         pos = end;
         float4* g4 = reinterpret_cast<float4*>(grow);
-        float4* f4 = reinterpret_cast<float4*>(frow);
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (a.feat_bf16) {
+        if constexpr (FB) {
+            // bf16 features stay bf16 in the row: 4 values per 8-byte copy
+            const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row);
+            uint2* f2 = reinterpret_cast<uint2*>(frow);
+            for (int i = lane; i < row / 4; i += kWave) f2[i] = src[i];
+        } else if (a.feat_bf16) {
+            float4* f4 = reinterpret_cast<float4*>(frow);
             // bf16 features: 4 values per 8-byte load, widened exactly to fp32
             const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row);
             for (int i = lane; i < row / 4; i += kWave) {
@@ -237,6 +301,7 @@ k_shade(ShadeArgs a, int rowp) {
                                     __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
             }
         } else {
+            float4* f4 = reinterpret_cast<float4*>(frow);
             const float4* src = reinterpret_cast<const float4*>(a.feat + (long long)t * row);
             for (int i = lane; i < row / 4; i += kWave) f4[i] = src[i];
         }
@@ -271,10 +336,10 @@ k_shade(ShadeArgs a, int rowp) {
                     const int lv = f0 / C, c = f0 % C;
                     const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
                     crs[tt] = cr;
-                    const float* fl = frow + loff[lv] * C + c;
-                    const float4 v0 = *reinterpret_cast<const float4*>(fl + cr.idx[0] * C);
-                    const float4 v1 = *reinterpret_cast<const float4*>(fl + cr.idx[1] * C);
-                    const float4 v2 = *reinterpret_cast<const float4*>(fl + cr.idx[2] * C);
+                    const FT* fl = frow + loff[lv] * C + c;
+                    const float4 v0 = ld4(fl + cr.idx[0] * C);
+                    const float4 v1 = ld4(fl + cr.idx[1] * C);
+                    const float4 v2 = ld4(fl + cr.idx[2] * C);
                     float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
                     e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
                     e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;
@@ -371,35 +436,32 @@ k_shade(ShadeArgs a, int rowp) {
             // ---- weight gradients: transpose (delta, activation) pairs through
             // LDS one at a time and contract over the 16 pixels
             {
-                float* sD = stage;
-                float* sA = stage + 16 * K::SH;
-                for (int m = 0; m < NH; ++m)
-                    for (int r = 0; r < 4; ++r) sD[j * K::SH + 16 * m + 4 * g + r] = dh1[m][r];
-                for (int tt = 0; tt < NI; ++tt)
-                    for (int r = 0; r < 4; ++r) sA[j * K::SX + 16 * tt + 4 * g + r] = x[tt][r];
+                constexpr int DINP = K::DINP;
+                ST* sD = stage;
+                ST* sA = stage + K::region(DH);
+                for (int m = 0; m < NH; ++m) stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh1[m]);
+                for (int tt = 0; tt < NI; ++tt) stage_put<K>(sA, DINP, 16 * tt + 4 * g, j, x[tt]);
                 wave_lds_sync();
                 for (int m = 0; m < NH; ++m)
                     for (int nn = 0; nn < NI; ++nn)
-                        gW1[m][nn] = Mma<K::BF16>::outer(sD + 16 * m, K::SH, sA + 16 * nn, K::SX, lane, gW1[m][nn]);
+                        gW1[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DINP, 16 * nn, lane, gW1[m][nn]);
                 wave_lds_sync();
-                for (int m = 0; m < NH; ++m)
-                    for (int r = 0; r < 4; ++r) {
-                        sD[j * K::SH + 16 * m + 4 * g + r] = dh2[m][r];
-                        sA[j * K::SH + 16 * m + 4 * g + r] = h1[m][r];
-                    }
+                for (int m = 0; m < NH; ++m) {
+                    stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh2[m]);
+                    stage_put<K>(sA, DH, 16 * m + 4 * g, j, h1[m]);
+                }
                 wave_lds_sync();
                 for (int m = 0; m < NH; ++m)
                     for (int nn = 0; nn < NH; ++nn)
-                        gW2[m][nn] = Mma<K::BF16>::outer(sD + 16 * m, K::SH, sA + 16 * nn, K::SH, lane, gW2[m][nn]);
+                        gW2[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DH, 16 * nn, lane, gW2[m][nn]);
                 wave_lds_sync();
-                float* sD3 = stage;
-                float* sH2 = stage + 16 * K::SD3;
-                for (int r = 0; r < 4; ++r) sD3[j * K::SD3 + 4 * g + r] = d3[0][r];
-                for (int m = 0; m < NH; ++m)
-                    for (int r = 0; r < 4; ++r) sH2[j * K::SH + 16 * m + 4 * g + r] = h2[m][r];
+                ST* sD3 = stage;
+                ST* sH2 = stage + K::region(16);
+                stage_put<K>(sD3, 16, 4 * g, j, d3[0]);
+                for (int m = 0; m < NH; ++m) stage_put<K>(sH2, DH, 16 * m + 4 * g, j, h2[m]);
                 wave_lds_sync();
                 for (int m = 0; m < NH; ++m)
-                    gW3[m] = Mma<K::BF16>::outer(sD3, K::SD3, sH2 + 16 * m, K::SH, lane, gW3[m]);
+                    gW3[m] = stage_outer<K>(sD3, 16, 0, sH2, DH, 16 * m, lane, gW3[m]);
                 wave_lds_sync();
             }
             rec = next;
@@ -450,7 +512,7 @@ k_shade(ShadeArgs a, int rowp) {
 
     // ---- workgroup reduction into LDS (reusing the staging area), one partial per block
     __syncthreads();
-    float* red = smem + K::LDS_BLOB;
+    float* red = smem + P::BLOB_BYTES / 4;
     for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) red[i] = 0.f;
     __syncthreads();
     float* rW1 = red;
@@ -527,22 +589,23 @@ __global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ 
     }
 }
 
-template <class K>
+template <class K, bool FB>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
+    using P = Plan<K, FB>;
     const int rowp = (a.S_tot * K::C + 3) & ~3;
-    const size_t lds = sizeof(float) * (K::LDS_BLOB + kWavesPerBlock * (2 * rowp + K::STAGE));
-    const size_t red = sizeof(float) * (K::LDS_BLOB + K::NPARAM);
+    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
+    const size_t red = P::BLOB_BYTES + sizeof(float) * K::NPARAM;
     const size_t bytes = lds > red ? lds : red;
     if (bytes > 160 * 1024) return hipErrorInvalidValue;
     static size_t opted_in = 65536;
     if (bytes > opted_in) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K, FB>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  static_cast<int>(bytes));
         if (e != hipSuccess) return e;
         opted_in = bytes;
     }
-    k_shade<K><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
+    k_shade<K, FB><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
     return hipGetLastError();
 }
 
@@ -594,8 +657,9 @@ hipError_t launch_shade_blobs(int nl, int c, int dh, bool bf16, const float* w1,
 hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s) {
 #define X(NL, C, DH)                                                                   \
     if (nl == NL && c == C && dh == DH)                                                \
-        return bf16 ? shade_dispatch<Cfg<NL, C, DH, true>>(a, nblocks, s)              \
-                    : shade_dispatch<Cfg<NL, C, DH, false>>(a, nblocks, s);
+        return !bf16        ? shade_dispatch<Cfg<NL, C, DH, false>, false>(a, nblocks, s)  \
+               : a.feat_bf16 ? shade_dispatch<Cfg<NL, C, DH, true>, true>(a, nblocks, s)    \
+                             : shade_dispatch<Cfg<NL, C, DH, true>, false>(a, nblocks, s);
     DS_SHADE_CONFIGS(X)
 #undef X
     return hipErrorInvalidValue;
