@@ -129,15 +129,17 @@ hipError_t launch_edge_grad(const Views& v, int C, const float* color, const flo
                             int T, const int* tri, hipStream_t s);
 
 // Covered pixels grouped by triangle: list[tri_off[t] + rank] =
-// (pixel, bits(b0), bits(b1), t).
+// (pixel, bits(b0), bits(b1), t); with px_list, also the covered pixels in
+// tile order at px_list[tile_px_off[tile] + k].
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
-                                      const float* rast, const int* tri_off, int4* list, hipStream_t s);
+                                      const float* rast, const int* tri_off, int4* list, const int* tile_px_off,
+                                      int* px_list, hipStream_t s);
 
-// Over the triangle-grouped covered-pixel list (int4 (pixel, b0, b1, t)),
-// *n_list entries (read on the device, at most max_n).
+// Over a covered-pixel list (pixel indices), *n_list entries (read on the
+// device, at most max_n).
 hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, const float* grad_color,
                                    const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                   const int* tri, const TriRecord* recs, const int4* list, const int* n_list,
+                                   const int* tri, const TriRecord* recs, const int* px_list, const int* n_list,
                                    long long max_n, hipStream_t s);
 
 hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s);

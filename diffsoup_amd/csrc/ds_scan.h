@@ -81,6 +81,10 @@ size_t scan_tmp_bytes(long long n);
 long long* scan_total_slot(void* tmp, long long n);
 // out[0..n]: exclusive prefix, out[n] = total.
 hipError_t launch_exclusive_scan(const int* in, long long n, int* out, void* tmp, hipStream_t s);
+// Two independent scans of the kind above in one launch when both fit one
+// workgroup (else two scans); tmp0 / tmp1 are separate scan_tmp_bytes areas.
+hipError_t launch_exclusive_scan_pair(const int* in0, long long n0, int* out0, void* tmp0, const int* in1,
+                                      long long n1, int* out1, void* tmp1, hipStream_t s);
 // In-place exclusive prefix of data[0..n).
 hipError_t launch_exclusive_scan_inplace(int* data, long long n, void* tmp, hipStream_t s);
 // out[i] = sum of in[0..i] as int64.

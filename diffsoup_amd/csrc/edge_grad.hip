@@ -313,18 +313,20 @@ __global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ col
     edge_at<false>(v, T, C, color, grad_color, rast, V, pos, grad_pos, tri, nullptr, p0, -1);
 }
 
-// Edge gradients over the covered pixels in the triangle-grouped list that
-// the shading kernel consumes (records (pixel, b0, b1, triangle)); the count
-// is read on the device.  Consecutive entries mostly share a triangle and a
-// view, so a wave groups its lanes by (view, gradient triangle), adds each
-// lane's 12 clip-space components into its group's LDS slot, and flushes one
-// global atomic per (group, component).  A persistent grid strides over the
-// list, so every lane of a wave has a pixel except in the last stride.
+// Edge gradients over the covered pixels listed in tile order (each screen
+// tile's covered pixels together, so a tile's raster, colour and gradient
+// rows are fetched once and its neighbours' rows are near); the count is
+// read on the device.  Consecutive entries mostly share a few triangles of
+// one view, so a wave groups its lanes by (view, gradient triangle), adds
+// each lane's 12 clip-space components into its group's LDS slot, and
+// flushes one global atomic per (group, component).  A persistent grid
+// strides over the list, so every lane of a wave has a pixel except in the
+// last stride.
 __global__ void __launch_bounds__(256)
 k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const float* __restrict__ grad_color,
                    const float4* __restrict__ rast, int V, const float4* __restrict__ pos,
                    float* __restrict__ grad_pos, const int* __restrict__ tri, const TriRecord* __restrict__ recs,
-                   const int4* __restrict__ list, const int* __restrict__ n_list) {
+                   const int* __restrict__ px_list, const int* __restrict__ n_list) {
     __shared__ float slot[4][64][13];
     __shared__ long long sv[4][64][3];
     const int n = *n_list;
@@ -337,7 +339,7 @@ k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const
         int v0[3] = {0, 0, 0};
         int it0 = -1, b = -1;
         bool act = false;
-        if (i < n) act = edge_eval<true>(v, T, C, color, grad_color, rast, V, pos, tri, recs, list[i].x, b, g, v0, it0);
+        if (i < n) act = edge_eval<true>(v, T, C, color, grad_color, rast, V, pos, tri, recs, px_list[i], b, g, v0, it0);
         const long long key = act ? (long long)b * T + it0 : -1;
         int grp = -1, ng = 0;
         unsigned long long todo = __ballot(act);
@@ -371,13 +373,13 @@ k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const
 
 hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, const float* grad_color,
                                    const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                   const int* tri, const TriRecord* recs, const int4* list, const int* n_list,
+                                   const int* tri, const TriRecord* recs, const int* px_list, const int* n_list,
                                    long long max_n, hipStream_t s) {
     if (max_n <= 0) return hipSuccess;
     const long long blocks = std::min<long long>((max_n + 255) / 256, 2048);
     k_edge_grad_sorted<<<static_cast<unsigned>(blocks), 256, 0, s>>>(
         v, T, C, color, grad_color, reinterpret_cast<const float4*>(rast), V, reinterpret_cast<const float4*>(pos),
-        grad_pos, tri, recs, list, n_list);
+        grad_pos, tri, recs, px_list, n_list);
     return hipGetLastError();
 }
 

@@ -298,6 +298,15 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
     unsigned long long* zw = zb[wq];
     cand_setup(cgw, w, wl);
     zw[wl] = static_cast<unsigned long long>(kKeyEmpty);
+    const bool inside = w.px < v.W[w.b] && w.py < v.H[w.b];
+    const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
+    // the background shading's target values are loaded now, so that their
+    // latency overlaps the candidate walk
+    float tg[3] = {0.f, 0.f, 0.f};
+    if constexpr (SHADE) {
+        if (inside)
+            for (int c = 0; c < 3; ++c) tg[c] = bg.target[p * 3 + c];
+    }
 
     float bb0 = 0.f, bb1 = 0.f, bd = 0.f;
     int bt = -1;
@@ -340,8 +349,6 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
         }
     }
 
-    const bool inside = w.px < v.W[w.b] && w.py < v.H[w.b];
-    const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
     if (inside) {
         rast[p] = bt >= 0 ? make_float4(bb0, bb1, bd, static_cast<float>(bt + 1))
                           : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -354,7 +361,6 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
         // pixels of its triangle (one atomic per distinct triangle per wave).
         float l = 0.f;
         if (inside && bt < 0) {
-            const float* tg = bg.target + p * 3;
             float* col = bg.color + p * 3;
             float* gc = bg.grad_color + p * 3;
             float l1 = 0.f;
@@ -445,18 +451,22 @@ __global__ void __launch_bounds__(1024) k_sum_covered(const int* __restrict__ ti
     }
 }
 
-// list[tri_off[t] + rank] = pixel for every covered pixel of every tile.
+// list[tri_off[t] + rank] = pixel for every covered pixel of every tile, and
+// (when px_list is given) the same pixels in tile order:
+// px_list[tile_px_off[tile] + k] = pixel.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_fill_lists(const int2* __restrict__ tile_list,
                                                    const int* __restrict__ tile_ncov,
                                                    const float4* __restrict__ rast,
-                                                   const int* __restrict__ tri_off, int4* __restrict__ list) {
+                                                   const int* __restrict__ tri_off, int4* __restrict__ list,
+                                                   const int* __restrict__ tile_px_off, int* __restrict__ px_list) {
     const int k = threadIdx.x;
     if (k >= tile_ncov[blockIdx.x]) return;
     const int2 e = tile_list[(long long)blockIdx.x * NT + k];
     const float4 r = rast[e.x];
     const int t = static_cast<int>(r.w) - 1;
     list[tri_off[t] + e.y] = make_int4(e.x, __float_as_int(r.x), __float_as_int(r.y), t);
+    if (px_list) px_list[tile_px_off[blockIdx.x] + k] = e.x;
 }
 
 // Score-function gradient of the per-pixel L1 loss w.r.t. fragment opacity,
@@ -659,14 +669,22 @@ hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const
 }
 
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
-                                      const float* rast, const int* tri_off, int4* list, hipStream_t s) {
+                                      const float* rast, const int* tri_off, int4* list, const int* tile_px_off,
+                                      int* px_list, hipStream_t s) {
     const int n_tiles = v.tile_base[v.n];
     if (n_tiles == 0) return hipSuccess;
     const float4* r4 = reinterpret_cast<const float4*>(rast);
     switch (v.tile) {
-        case 8: k_fill_lists<64><<<n_tiles, 64, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list); break;
-        case 16: k_fill_lists<256><<<n_tiles, 256, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list); break;
-        case 32: k_fill_lists<1024><<<n_tiles, 1024, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list); break;
+        case 8:
+            k_fill_lists<64><<<n_tiles, 64, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list, tile_px_off, px_list);
+            break;
+        case 16:
+            k_fill_lists<256><<<n_tiles, 256, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list, tile_px_off, px_list);
+            break;
+        case 32:
+            k_fill_lists<1024><<<n_tiles, 1024, 0, s>>>(tile_list, tile_ncov, r4, tri_off, list, tile_px_off,
+                                                        px_list);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -109,6 +109,20 @@ __global__ void __launch_bounds__(1024) k_scan_one(const int* in, int n, OUT* ou
     }
 }
 
+// Two independent exclusive scans (each at most kOneBlock items), one per
+// workgroup of a single launch.
+__global__ void __launch_bounds__(1024) k_scan_two(const int* in0, int n0, int* out0, long long* total0,
+                                                   const int* in1, int n1, int* out1, long long* total1) {
+    const int* in = blockIdx.x == 0 ? in0 : in1;
+    const int n = blockIdx.x == 0 ? n0 : n1;
+    int* out = blockIdx.x == 0 ? out0 : out1;
+    const long long total = block_scan_1024<int>(in, n, out, false);
+    if (threadIdx.x == 0) {
+        *(blockIdx.x == 0 ? total0 : total1) = total;
+        out[n] = static_cast<int>(total);
+    }
+}
+
 __global__ void k_zero_total(long long* tmp, int* out) {
     tmp[0] = 0;
     if (out) out[0] = 0;
@@ -145,6 +159,18 @@ static hipError_t scan_impl(const int* in, long long n, OUT* out, void* tmp, int
 
 hipError_t launch_exclusive_scan(const int* in, long long n, int* out, void* tmp, hipStream_t s) {
     return scan_impl<int>(in, n, out, tmp, 0, 1, s);
+}
+
+hipError_t launch_exclusive_scan_pair(const int* in0, long long n0, int* out0, void* tmp0, const int* in1,
+                                      long long n1, int* out1, void* tmp1, hipStream_t s) {
+    if (n0 > 0 && n0 <= kOneBlock && n1 > 0 && n1 <= kOneBlock) {
+        k_scan_two<<<2, 1024, 0, s>>>(in0, static_cast<int>(n0), out0, scan_total_slot(tmp0, n0), in1,
+                                      static_cast<int>(n1), out1, scan_total_slot(tmp1, n1));
+        return hipGetLastError();
+    }
+    const hipError_t e = scan_impl<int>(in0, n0, out0, tmp0, 0, 1, s);
+    if (e != hipSuccess) return e;
+    return scan_impl<int>(in1, n1, out1, tmp1, 0, 1, s);
 }
 
 hipError_t launch_exclusive_scan_inplace(int* data, long long n, void* tmp, hipStream_t s) {

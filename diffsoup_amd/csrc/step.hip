@@ -147,6 +147,7 @@ inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct StepLayout {
     size_t raster, gpos, galpha, tile_list, tile_ncov, tri_count, tri_off, list, scan, bgloss, blobs, partials,
+        tile_px_off, px_list, scan2,
         total;
     int n_tiles, nblocks, nparam;
     long long P;
@@ -178,6 +179,9 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.tri_off = o; o += up256(sizeof(int) * ((size_t)d->T + 1));
     L.list = o; o += up256(sizeof(int) * 4 * (size_t)L.P);
     L.scan = o; o += up256(scan_tmp_bytes(d->T));
+    L.tile_px_off = o; o += up256(sizeof(int) * ((size_t)L.n_tiles + 1));
+    L.px_list = o; o += up256(sizeof(int) * (size_t)L.P);
+    L.scan2 = o; o += up256(scan_tmp_bytes(L.n_tiles));
     L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
     L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(nl, d->channels, d->hidden));
     L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
@@ -243,6 +247,9 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     int* tri_off = reinterpret_cast<int*>(ws + L.tri_off);
     int4* list = reinterpret_cast<int4*>(ws + L.list);
     void* scan_tmp = ws + L.scan;
+    int* tile_px_off = reinterpret_cast<int*>(ws + L.tile_px_off);
+    int* px_list = reinterpret_cast<int*>(ws + L.px_list);
+    void* scan_tmp2 = ws + L.scan2;
     float* bgloss = reinterpret_cast<float*>(ws + L.bgloss);
     float* blobs = reinterpret_cast<float*>(ws + L.blobs);
     float* partials = reinterpret_cast<float*>(ws + L.partials);
@@ -289,8 +296,11 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
                                        d->rast, rw, bg, s),
            "resolve");
     DS_TRY(mark(DS_STAGE_RESOLVE, false), "event");
-    DS_TRY(launch_exclusive_scan(tri_count, d->T, tri_off, scan_tmp, s), "triangle scan");
-    DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, tri_off, list, s), "triangle lists");
+    DS_TRY(launch_exclusive_scan_pair(tri_count, d->T, tri_off, scan_tmp, tile_ncov, L.n_tiles, tile_px_off,
+                                      scan_tmp2, s),
+           "triangle / tile scans");
+    DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, tri_off, list, tile_px_off, px_list, s),
+           "triangle lists");
     ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), tri_off, list, d->T, d->feat,
                  reinterpret_cast<const unsigned short*>(d->feat_bf16), d->grad_feat,
                  static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->target, d->color,
@@ -300,7 +310,7 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     DS_TRY(mark(DS_STAGE_SHADE, false), "event");
     DS_TRY(mark(DS_STAGE_EDGE, true), "event");
     DS_TRY(launch_edge_grad_sorted(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri,
-                                   rw.recs, list, tri_off + d->T, L.P, s),
+                                   rw.recs, px_list, tile_px_off + L.n_tiles, L.P, s),
            "edge grad");
     DS_TRY(mark(DS_STAGE_EDGE, false), "event");
     DS_TRY(mark(DS_STAGE_AUX, true), "event");
