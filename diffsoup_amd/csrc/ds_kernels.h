@@ -48,6 +48,17 @@ enum Stat : int {
     kStatCount = 8,
 };
 
+// One covered pixel of the triangle-grouped list that the shading kernel
+// walks: its barycentrics, the pixel, the triangle, and the pixel's target
+// colour (carried along so that shading needs no dependent load per tile).
+struct __align__(16) PixRec {
+    float b0, b1;
+    int pixel;
+    int tri;
+    float tg[3];
+    float pad;
+};
+
 // Workspace carve-up for one raster launch.
 struct RasterWorkspace {
     TriRecord* recs;     // [B * T]
@@ -128,12 +139,12 @@ hipError_t launch_edge_grad(const Views& v, int C, const float* color, const flo
                             const float* rast, int V, const float* pos, float* grad_pos,
                             int T, const int* tri, hipStream_t s);
 
-// Covered pixels grouped by triangle: list[tri_off[t] + rank] =
-// (pixel, bits(b0), bits(b1), t); with px_list, also the covered pixels in
-// tile order at px_list[tile_px_off[tile] + k].
+// Covered pixels grouped by triangle: list[tri_off[t] + rank] = the pixel's
+// PixRec (target colours from target [P,3]); with px_list, also the covered
+// pixels in tile order at px_list[tile_px_off[tile] + k].
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
-                                      const float* rast, const int* tri_off, int4* list, const int* tile_px_off,
-                                      int* px_list, hipStream_t s);
+                                      const float* rast, const float* target, const int* tri_off, PixRec* list,
+                                      const int* tile_px_off, int* px_list, hipStream_t s);
 
 // Over a covered-pixel list (pixel indices), *n_list entries (read on the
 // device, at most max_n).

@@ -6,12 +6,14 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "ds_kernels.h"
+
 namespace ds {
 
 struct ShadeArgs {
     const float4* rast;     // [P] (b0, b1, depth, id + 1)
     const int* tri_off;     // [T + 1] start of each triangle's covered pixels in list
-    const int4* list;       // covered pixels grouped by triangle: (pixel, bits b0, bits b1, t)
+    const PixRec* list;     // covered pixels grouped by triangle
     int T;
     const float* feat;      // [T, S_tot, C]
     const unsigned short* feat_bf16;   // bf16 copy read instead of feat when non-null
@@ -22,7 +24,6 @@ struct ShadeArgs {
     const float* b1;
     const float* b2;
     const float* b3;
-    const float* target;    // [P, 3]
     float* color;           // [P, 3]
     float* grad_color;      // [P, 3]
     float* pix_loss;        // [P]

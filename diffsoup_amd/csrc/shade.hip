@@ -79,6 +79,10 @@ struct Cfg {
     static constexpr int TILE_ELEMS = (BF16 ? 2 : 4) * kWave;
     static constexpr int BLOB_ELEMS = NTILES * TILE_ELEMS;
     static constexpr int NPARAM = DH * DIN + DH * DH + 3 * DH + DH + DH + 3 + 1;
+    // workgroups per CU the register allocation must allow: two (two waves
+    // per SIMD, 256 registers each) up to hidden width 32; the 80->64->64->3
+    // configuration needs one wave's full 512 registers
+    static constexpr int MIN_BLOCKS = DH <= 32 ? 2 : 1;
     // per-wave LDS staging for the weight-gradient transposes: one (delta,
     // activation) pair at a time - (D1, X), (D2, H1), (D3, H2).  The f32
     // products stage floats as [pixel][unit] rows (width + 4 apart); the bf16
@@ -136,6 +140,23 @@ __device__ __forceinline__ void stage_put(typename K::ST* reg, int width, int u0
         for (int r = 0; r < 4; ++r) reg[(u0 + r) * K::SPX + j] = bf16_rne(v[r]);
     } else {
         *reinterpret_cast<float4*>(reg + j * (width + 4) + u0) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+// LDS slot of channels c .. c+3 (c a multiple of 4) of sample s in a
+// gradient row.  The row keeps the table's sample-major layout, but the
+// 4-channel blocks of a sample are permuted by an XOR with bits of s, so
+// that the pixel lanes of one atomic instruction (which add the same channel
+// block to different lattice samples) spread over all 32 LDS banks instead
+// of 32 / C * 4 of them.
+template <int C>
+__device__ __forceinline__ int grad_slot(uint32_t s, int c) {
+    constexpr int NB = C / 4;
+    if constexpr ((NB & (NB - 1)) == 0 && C <= 32) {
+        const int sw = static_cast<int>((s * C) >> 5) & (NB - 1);
+        return static_cast<int>(s) * C + 4 * ((c >> 2) ^ sw);
+    } else {
+        return static_cast<int>(s) * C + c;
     }
 }
 
@@ -219,7 +240,7 @@ __device__ __forceinline__ void dense(const float* blobs, int base, int lane, co
 }
 
 template <class K, bool FB>
-__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+__global__ void __launch_bounds__(kWave * kWavesPerBlock, K::MIN_BLOCKS)
 k_shade(ShadeArgs a, int rowp) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
     using P = Plan<K, FB>;
@@ -311,19 +332,20 @@ k_shade(ShadeArgs a, int rowp) {
         for (int vtx = 0; vtx < 6; ++vtx)
             for (int r = 0; r < 4; ++r) racc[vtx][r] = 0.f;
 
-        int4 rec = j < npx ? a.list[beg + j] : make_int4(0, 0, 0, 0);
+        // lanes of group 0 also take the pixel's target colour from the record
+        const float4* recs4 = reinterpret_cast<const float4*>(a.list);
+        float4 rec = j < npx ? recs4[2 * (beg + j)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 rtg = g == 0 && j < npx ? recs4[2 * (beg + j) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
         for (int c0 = 0; c0 < npx; c0 += 16) {
             const int idx = c0 + j;
             const bool valid = idx < npx;
-            const int p = rec.x;
-            const float b0 = __int_as_float(rec.y), b1 = __int_as_float(rec.z);
-            // next tile's records and this tile's target are in flight while it shades
-            const int4 next = idx + 16 < npx ? a.list[beg + idx + 16] : make_int4(0, 0, 0, 0);
-            float tg0 = 0.f, tg1 = 0.f, tg2 = 0.f;
-            if (g == 0 && valid) {
-                const float* tg = a.target + (long long)p * 3;
-                tg0 = tg[0]; tg1 = tg[1]; tg2 = tg[2];
-            }
+            const int p = __float_as_int(rec.z);
+            const float b0 = rec.x, b1 = rec.y;
+            const float tg0 = rtg.x, tg1 = rtg.y, tg2 = rtg.z;
+            // the next tile's records are in flight while this one shades
+            const bool nv = idx + 16 < npx;
+            const float4 next = nv ? recs4[2 * (beg + idx + 16)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 ntg = g == 0 && nv ? recs4[2 * (beg + idx + 16) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
 
             // ---- gather from the LDS row: lane (g, j) holds features 16 tt + 4 g + r of pixel j
             f32x4 x[NI];
@@ -409,7 +431,7 @@ k_shade(ShadeArgs a, int rowp) {
             // 6 vertices) are shared by nearly all pixels of a tile, so they
             // accumulate per lane in registers over the whole segment and are
             // reduced across the 16 pixel lanes once at its end; finer levels
-            // add into the LDS row with LDS atomics, which rarely collide there.
+            // add into the LDS row with LDS atomics.
             for (int tt = 0; tt < NI; ++tt) {
                 const int f0 = 16 * tt + 4 * g;
                 if (!valid || f0 >= DIN) continue;
@@ -423,9 +445,10 @@ k_shade(ShadeArgs a, int rowp) {
                         for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
                     }
                 } else {
-                    float* gl = grow + loff[lv] * C + c;
-                    for (int k = 0; k < 3; ++k)
-                        for (int r = 0; r < 4; ++r) atomicAdd(gl + cr.idx[k] * C + r, cr.w[k] * dx[tt][r]);
+                    for (int k = 0; k < 3; ++k) {
+                        float* gl = grow + grad_slot<C>(loff[lv] + cr.idx[k], c);
+                        for (int r = 0; r < 4; ++r) atomicAdd(gl + r, cr.w[k] * dx[tt][r]);
+                    }
                 }
             }
 
@@ -465,6 +488,7 @@ k_shade(ShadeArgs a, int rowp) {
                 wave_lds_sync();
             }
             rec = next;
+            rtg = ntg;
         }
 
         // ---- the coarse-level sums: reduce over the 16 pixel lanes of each
@@ -479,10 +503,11 @@ k_shade(ShadeArgs a, int rowp) {
                     racc[vtx][r] = row16_sum(racc[vtx][r]);
             wave_lds_sync();
             if (coarse && j == 0) {
-                float* gl = grow + loff[lv] * C + c;
                 for (int vtx = 0; vtx < 6; ++vtx)
-                    if (vtx < S)
-                        for (int r = 0; r < 4; ++r) gl[vtx * C + r] += racc[vtx][r];
+                    if (vtx < S) {
+                        float* gl = grow + grad_slot<C>(loff[lv] + vtx, c);
+                        for (int r = 0; r < 4; ++r) gl[r] += racc[vtx][r];
+                    }
             }
             wave_lds_sync();
         }
@@ -490,11 +515,12 @@ k_shade(ShadeArgs a, int rowp) {
         // ---- the triangle's feature gradient row
         if (owned) {
             float4* dst = reinterpret_cast<float4*>(a.grad_feat + (long long)t * row);
-            for (int i = lane; i < row / 4; i += kWave) dst[i] = g4[i];
+            for (int i = lane; i < row / 4; i += kWave)
+                dst[i] = *reinterpret_cast<const float4*>(grow + grad_slot<C>(4 * i / C, 4 * i % C));
         } else {
             float* dst = a.grad_feat + (long long)t * row;
             for (int i = lane; i < row; i += kWave) {
-                const float gv = grow[i];
+                const float gv = grow[grad_slot<C>(i / C, i % C & ~3) + (i & 3)];
                 if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
             }
         }

@@ -177,7 +177,7 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.tile_ncov = o; o += up256(sizeof(int) * (size_t)L.n_tiles);
     L.tri_count = o; o += up256(sizeof(int) * (size_t)(d->T > 0 ? d->T : 1));
     L.tri_off = o; o += up256(sizeof(int) * ((size_t)d->T + 1));
-    L.list = o; o += up256(sizeof(int) * 4 * (size_t)L.P);
+    L.list = o; o += up256(sizeof(PixRec) * (size_t)L.P);
     L.scan = o; o += up256(scan_tmp_bytes(d->T));
     L.tile_px_off = o; o += up256(sizeof(int) * ((size_t)L.n_tiles + 1));
     L.px_list = o; o += up256(sizeof(int) * (size_t)L.P);
@@ -245,7 +245,7 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     int* tile_ncov = reinterpret_cast<int*>(ws + L.tile_ncov);
     int* tri_count = reinterpret_cast<int*>(ws + L.tri_count);
     int* tri_off = reinterpret_cast<int*>(ws + L.tri_off);
-    int4* list = reinterpret_cast<int4*>(ws + L.list);
+    PixRec* list = reinterpret_cast<PixRec*>(ws + L.list);
     void* scan_tmp = ws + L.scan;
     int* tile_px_off = reinterpret_cast<int*>(ws + L.tile_px_off);
     int* px_list = reinterpret_cast<int*>(ws + L.px_list);
@@ -299,11 +299,12 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     DS_TRY(launch_exclusive_scan_pair(tri_count, d->T, tri_off, scan_tmp, tile_ncov, L.n_tiles, tile_px_off,
                                       scan_tmp2, s),
            "triangle / tile scans");
-    DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, tri_off, list, tile_px_off, px_list, s),
+    DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, d->target, tri_off, list, tile_px_off,
+                                      px_list, s),
            "triangle lists");
     ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), tri_off, list, d->T, d->feat,
                  reinterpret_cast<const unsigned short*>(d->feat_bf16), d->grad_feat,
-                 static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->target, d->color,
+                 static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->color,
                  d->grad_color, d->pix_loss, 2.0f * inv_n, partials};
     DS_TRY(mark(DS_STAGE_SHADE, true), "event");
     DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade");
