@@ -141,16 +141,17 @@ hipError_t launch_edge_grad(const Views& v, int C, const float* color, const flo
 
 // Covered pixels grouped by triangle: list[tri_off[t] + rank] = the pixel's
 // PixRec (target colours from target [P,3]); with px_list, also the covered
-// pixels in tile order at px_list[tile_px_off[tile] + k].
+// pixels in tile order at px_list[tile_px_off[tile] + k] = (pixel, view,
+// visible triangle, 0).
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
                                       const float* rast, const float* target, const int* tri_off, PixRec* list,
-                                      const int* tile_px_off, int* px_list, hipStream_t s);
+                                      const int* tile_px_off, int4* px_list, hipStream_t s);
 
-// Over a covered-pixel list (pixel indices), *n_list entries (read on the
-// device, at most max_n).
+// Over a covered-pixel list of (pixel, view, visible triangle, 0) entries,
+// *n_list of them (read on the device, at most max_n); C must be 3.
 hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, const float* grad_color,
                                    const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                   const int* tri, const TriRecord* recs, const int* px_list, const int* n_list,
+                                   const int* tri, const TriRecord* recs, const int4* px_list, const int* n_list,
                                    long long max_n, hipStream_t s);
 
 hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s);

@@ -127,20 +127,24 @@ __device__ __forceinline__ bool vertices_ok(const int vi[3], int V) {
 // Per-pixel edge gradient.  With REC the triangle corners come from the
 // raster's per-(view, triangle) clip records (one 48-byte read instead of an
 // index read followed by three vertex reads); b >= 0 gives the pixel's view.
-// All five raster texels and colour / gradient values are loaded before any
-// of them is used, so the dependent-load chain is two levels deep.
-template <bool REC>
+// With KNOWN the caller also gives the visible triangle it0 (a covered
+// pixel) and C is 3: every load then depends on (p0, b, it0) only - the five
+// raster texels, the record and the five pixels' colours and gradients are
+// all in flight together.
+template <bool REC, bool KNOWN = false>
 __device__ __forceinline__ bool edge_eval(const Views& v, int T, int C, const float* __restrict__ color,
                                           const float* __restrict__ grad_color, const float4* __restrict__ rast,
                                           int V, const float4* __restrict__ pos, const int* __restrict__ tri,
                                           const TriRecord* __restrict__ recs, long long p0, int& b, float g[12],
                                           int v0[3], int& it0) {
     const float4 r0 = rast[p0];
-    it0 = static_cast<int>(r0.w) - 1;
-    if (it0 < 0 || it0 >= T) return false;
-    if (b < 0) {
-        b = 0;
-        while (b + 1 < v.n && v.pix_base[b + 1] <= p0) ++b;
+    if constexpr (!KNOWN) {
+        it0 = static_cast<int>(r0.w) - 1;
+        if (it0 < 0 || it0 >= T) return false;
+        if (b < 0) {
+            b = 0;
+            while (b + 1 < v.n && v.pix_base[b + 1] <= p0) ++b;
+        }
     }
     const int W = v.W[b], H = v.H[b];
     const int local = static_cast<int>(p0 - v.pix_base[b]);
@@ -175,6 +179,25 @@ __device__ __forceinline__ bool edge_eval(const Views& v, int T, int C, const fl
     }
     const float* c0 = color + p0 * C;
     const float* g0 = grad_color + p0 * C;
+    // KNOWN: the colour adjoints of the four edges from preloaded values
+    // (the same sums, in the same order, as the generic loop below)
+    float adjv[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (KNOWN) {
+        float cq[5][3], gq[5][3];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            const long long q = e == 0 ? p0 : pn[e - 1];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                cq[e][i] = color[q * 3 + i];
+                gq[e][i] = grad_color[q * 3 + i];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) adjv[e] += 0.5f * (gq[0][i] + gq[e + 1][i]) * (cq[0][i] - cq[e + 1][i]);
+    }
 
     for (int k = 0; k < 12; ++k) g[k] = 0.f;
     bool touched = false;
@@ -188,10 +211,14 @@ __device__ __forceinline__ bool edge_eval(const Views& v, int T, int C, const fl
         const float depth1 = r1.z;
         int it1 = static_cast<int>(r1.w) - 1;
         if (it1 >= T) it1 = -1;
-        const float* c1 = color + p1 * C;
-        const float* g1 = grad_color + p1 * C;
         float adj = 0.f;
-        for (int i = 0; i < C; ++i) adj += 0.5f * (g0[i] + g1[i]) * (c0[i] - c1[i]);
+        if constexpr (KNOWN) {
+            adj = adjv[e];
+        } else {
+            const float* c1 = color + p1 * C;
+            const float* g1 = grad_color + p1 * C;
+            for (int i = 0; i < C; ++i) adj += 0.5f * (g0[i] + g1[i]) * (c0[i] - c1[i]);
+        }
         if (adj == 0.f) continue;
 
         const float gx = adj * static_cast<float>(DX[e]) * static_cast<float>(W) / 2.f;
@@ -315,8 +342,8 @@ __global__ void k_edge_grad(Views v, int T, int C, const float* __restrict__ col
 
 // Edge gradients over the covered pixels listed in tile order (each screen
 // tile's covered pixels together, so a tile's raster, colour and gradient
-// rows are fetched once and its neighbours' rows are near); the count is
-// read on the device.  Consecutive entries mostly share a few triangles of
+// rows are fetched once and its neighbours' rows are near), entries
+// (pixel, view, visible triangle, 0); the count is read on the device.  Consecutive entries mostly share a few triangles of
 // one view, so a wave groups its lanes by (view, gradient triangle), adds
 // each lane's 12 clip-space components into its group's LDS slot, and
 // flushes one global atomic per (group, component).  A persistent grid
@@ -326,7 +353,7 @@ __global__ void __launch_bounds__(256)
 k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const float* __restrict__ grad_color,
                    const float4* __restrict__ rast, int V, const float4* __restrict__ pos,
                    float* __restrict__ grad_pos, const int* __restrict__ tri, const TriRecord* __restrict__ recs,
-                   const int* __restrict__ px_list, const int* __restrict__ n_list) {
+                   const int4* __restrict__ px_list, const int* __restrict__ n_list) {
     __shared__ float slot[4][64][13];
     __shared__ long long sv[4][64][3];
     const int n = *n_list;
@@ -339,7 +366,12 @@ k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const
         int v0[3] = {0, 0, 0};
         int it0 = -1, b = -1;
         bool act = false;
-        if (i < n) act = edge_eval<true>(v, T, C, color, grad_color, rast, V, pos, tri, recs, px_list[i], b, g, v0, it0);
+        if (i < n) {
+            const int4 e = px_list[i];
+            b = e.y;
+            it0 = e.z;
+            act = edge_eval<true, true>(v, T, 3, color, grad_color, rast, V, pos, tri, recs, e.x, b, g, v0, it0);
+        }
         const long long key = act ? (long long)b * T + it0 : -1;
         int grp = -1, ng = 0;
         unsigned long long todo = __ballot(act);
@@ -373,9 +405,10 @@ k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const
 
 hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, const float* grad_color,
                                    const float* rast, int V, const float* pos, float* grad_pos, int T,
-                                   const int* tri, const TriRecord* recs, const int* px_list, const int* n_list,
+                                   const int* tri, const TriRecord* recs, const int4* px_list, const int* n_list,
                                    long long max_n, hipStream_t s) {
     if (max_n <= 0) return hipSuccess;
+    if (C != 3) return hipErrorInvalidValue;
     const long long blocks = std::min<long long>((max_n + 255) / 256, 2048);
     k_edge_grad_sorted<<<static_cast<unsigned>(blocks), 256, 0, s>>>(
         v, T, C, color, grad_color, reinterpret_cast<const float4*>(rast), V, reinterpret_cast<const float4*>(pos),

@@ -455,12 +455,12 @@ __global__ void __launch_bounds__(1024) k_sum_covered(const int* __restrict__ ti
 // (when px_list is given) the same pixels in tile order:
 // px_list[tile_px_off[tile] + k] = pixel.
 template <int NT>
-__global__ void __launch_bounds__(NT) k_fill_lists(const int2* __restrict__ tile_list,
+__global__ void __launch_bounds__(NT) k_fill_lists(Views v, const int2* __restrict__ tile_list,
                                                    const int* __restrict__ tile_ncov,
                                                    const float4* __restrict__ rast,
                                                    const float* __restrict__ target,
                                                    const int* __restrict__ tri_off, PixRec* __restrict__ list,
-                                                   const int* __restrict__ tile_px_off, int* __restrict__ px_list) {
+                                                   const int* __restrict__ tile_px_off, int4* __restrict__ px_list) {
     const int k = threadIdx.x;
     if (k >= tile_ncov[blockIdx.x]) return;
     const int2 e = tile_list[(long long)blockIdx.x * NT + k];
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(NT) k_fill_lists(const int2* __restrict__ tile
     float4* dst = reinterpret_cast<float4*>(list + tri_off[t] + e.y);
     dst[0] = make_float4(r.x, r.y, __int_as_float(e.x), __int_as_float(t));
     dst[1] = make_float4(tg[0], tg[1], tg[2], 0.f);
-    if (px_list) px_list[tile_px_off[blockIdx.x] + k] = e.x;
+    if (px_list) px_list[tile_px_off[blockIdx.x] + k] = make_int4(e.x, view_of_tile(v, blockIdx.x), t, 0);
 }
 
 // Score-function gradient of the per-pixel L1 loss w.r.t. fragment opacity,
@@ -674,21 +674,21 @@ hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const
 
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
                                       const float* rast, const float* target, const int* tri_off, PixRec* list,
-                                      const int* tile_px_off, int* px_list, hipStream_t s) {
+                                      const int* tile_px_off, int4* px_list, hipStream_t s) {
     const int n_tiles = v.tile_base[v.n];
     if (n_tiles == 0) return hipSuccess;
     const float4* r4 = reinterpret_cast<const float4*>(rast);
     switch (v.tile) {
         case 8:
-            k_fill_lists<64><<<n_tiles, 64, 0, s>>>(tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off,
+            k_fill_lists<64><<<n_tiles, 64, 0, s>>>(v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off,
                                                     px_list);
             break;
         case 16:
-            k_fill_lists<256><<<n_tiles, 256, 0, s>>>(tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off,
+            k_fill_lists<256><<<n_tiles, 256, 0, s>>>(v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off,
                                                       px_list);
             break;
         case 32:
-            k_fill_lists<1024><<<n_tiles, 1024, 0, s>>>(tile_list, tile_ncov, r4, target, tri_off, list,
+            k_fill_lists<1024><<<n_tiles, 1024, 0, s>>>(v, tile_list, tile_ncov, r4, target, tri_off, list,
                                                         tile_px_off, px_list);
             break;
         default: return hipErrorInvalidValue;

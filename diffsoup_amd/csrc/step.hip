@@ -180,7 +180,7 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.list = o; o += up256(sizeof(PixRec) * (size_t)L.P);
     L.scan = o; o += up256(scan_tmp_bytes(d->T));
     L.tile_px_off = o; o += up256(sizeof(int) * ((size_t)L.n_tiles + 1));
-    L.px_list = o; o += up256(sizeof(int) * (size_t)L.P);
+    L.px_list = o; o += up256(sizeof(int4) * (size_t)L.P);
     L.scan2 = o; o += up256(scan_tmp_bytes(L.n_tiles));
     L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
     L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(nl, d->channels, d->hidden));
@@ -248,7 +248,7 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     PixRec* list = reinterpret_cast<PixRec*>(ws + L.list);
     void* scan_tmp = ws + L.scan;
     int* tile_px_off = reinterpret_cast<int*>(ws + L.tile_px_off);
-    int* px_list = reinterpret_cast<int*>(ws + L.px_list);
+    int4* px_list = reinterpret_cast<int4*>(ws + L.px_list);
     void* scan_tmp2 = ws + L.scan2;
     float* bgloss = reinterpret_cast<float*>(ws + L.bgloss);
     float* blobs = reinterpret_cast<float*>(ws + L.blobs);
