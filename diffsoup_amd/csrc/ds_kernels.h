@@ -49,14 +49,16 @@ enum Stat : int {
 };
 
 // One covered pixel of the triangle-grouped list that the shading kernel
-// walks: its barycentrics, the pixel, the triangle, and the pixel's target
-// colour (carried along so that shading needs no dependent load per tile).
+// walks: its barycentrics, the pixel, the triangle, the pixel's target
+// colour and the end of the triangle's run in the list (carried along so
+// that shading walks from one triangle's run to the next, and from tile to
+// tile, without dependent loads of offsets or targets).
 struct __align__(16) PixRec {
     float b0, b1;
     int pixel;
     int tri;
     float tg[3];
-    float pad;
+    int seg_end;
 };
 
 // Workspace carve-up for one raster launch.

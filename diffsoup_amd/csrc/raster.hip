@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(NT) k_fill_lists(Views v, const int2* __restri
     const int t = static_cast<int>(r.w) - 1;
     float4* dst = reinterpret_cast<float4*>(list + tri_off[t] + e.y);
     dst[0] = make_float4(r.x, r.y, __int_as_float(e.x), __int_as_float(t));
-    dst[1] = make_float4(tg[0], tg[1], tg[2], 0.f);
+    dst[1] = make_float4(tg[0], tg[1], tg[2], __int_as_float(tri_off[t + 1]));
     if (px_list) px_list[tile_px_off[blockIdx.x] + k] = make_int4(e.x, view_of_tile(v, blockIdx.x), t, 0);
 }
 

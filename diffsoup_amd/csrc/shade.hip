@@ -284,26 +284,30 @@ k_shade(ShadeArgs a, int rowp) {
     // list and walks the triangle segments inside it.  A triangle whose
     // pixels all fall in one wave's range gets its gradient row stored; a
     // triangle split between waves is added with atomics into the zeroed table.
+    // A segment's triangle and end come from its first record (wave-uniform
+    // loads), so the walk never reads the offset table.
     const int n_waves = gridDim.x * kWavesPerBlock;
     const int wid = blockIdx.x * kWavesPerBlock + wave;
     const int total = a.tri_off[a.T];
     const int per = (total + n_waves - 1) / n_waves;
     const int r0 = min(total, wid * per), r1 = min(total, r0 + per);
-    int t = 0;
-    {   // first triangle whose segment ends after r0
-        int lo = 0, hi = a.T;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (a.tri_off[mid + 1] <= r0) lo = mid + 1; else hi = mid;
-        }
-        t = lo;
+    int t = 0, te = 0;
+    bool first_whole = true;   // the first segment starts at r0
+    if (r0 < r1) {
+        t = a.list[r0].tri;
+        te = a.list[r0].seg_end;
+        first_whole = r0 == 0 || a.list[r0 - 1].tri != t;
     }
-    for (int pos = r0; pos < r1; ++t) {
-        const int tb = a.tri_off[t], te = a.tri_off[t + 1];
-        if (te <= pos) continue;
+    for (int pos = r0; pos < r1;) {
         const int beg = pos, end = min(te, r1);
         const int npx = end - beg;
-        const bool owned = tb >= r0 && te <= r1;
+        const bool owned = (beg > r0 || first_whole) && te <= r1;
+        // the next segment's triangle and end
+        int t_next = 0, te_next = 0;
+        if (te < r1) {
+            t_next = a.list[te].tri;
+            te_next = a.list[te].seg_end;
+        }
         pos = end;
         float4* g4 = reinterpret_cast<float4*>(grow);
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -525,6 +529,8 @@ k_shade(ShadeArgs a, int rowp) {
             }
         }
         wave_lds_sync();
+        t = t_next;
+        te = te_next;
     }
 
     // ---- reduce the bias gradients over the 16 pixels of each lane group
