@@ -38,7 +38,9 @@ def stage_kernel(stage, step):
     summary (template arguments: tile edge or threads per tile; shading
     levels, channels, hidden width and precision)."""
     tile = step.tile
-    shade = f"k_shade<{step.nl},{step.C},{step.hidden},{'true' if step.bf16 else 'false'}>"
+    # the bf16-MLP kernel that reads bf16 feature rows is a separate instance
+    rows = ",bf16rows" if step.bf16 and step.bf16_features else ""
+    shade = f"k_shade<{step.nl},{step.C},{step.hidden},{'true' if step.bf16 else 'false'}{rows}>"
     return {"resolve": f"k_resolve<{tile},true>", "shade": shade,
             "edge": "k_edge_grad_sorted", "aux": f"k_opacity_aux<{tile}>"}[stage]
 
@@ -350,7 +352,8 @@ def _kernel_key(name):
     if m:
         k = m.group(1)
         t = re.search(r"Cfg<([^>]*)>", name) or re.search(r"k_\w+<([^>(]*)>", name)
-        return k + ("<" + t.group(1).replace(" ", "") + ">" if t else "")
+        rows = ",bf16rows" if re.search(r"Cfg<[^>]*>,\s*true>", name) else ""
+        return k + ("<" + t.group(1).replace(" ", "") + rows + ">" if t else "")
     if "neg" in name:
         return "calibration torch.neg 256 MiB"
     return re.sub(r"\(.*", "", name)[:50]
@@ -384,8 +387,9 @@ def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
                "gpurun_out/pmc_FETCH_SIZE/run_counter_collection.csv "
                "gpurun_out/pmc_WRITE_SIZE/run_counter_collection.csv " + os.path.relpath(PROFILE_MD, ROOT))
     out = ["# Fused training step: sphere+torus soup, 15K triangles, 8 views 800x800\n",
-           "bench.py runs the fp32-MLP step (`k_shade<4,8,32,false>`) and the bf16-MLP step "
-           "(`k_shade<4,8,32,true>`); every other kernel is shared.  `$REPO` is the repository root.\n",
+           "bench.py runs the fp32-MLP step (`k_shade<4,8,32,false>`), the bf16-MLP step with bf16 "
+           "feature rows (`k_shade<4,8,32,true,bf16rows>`) and with fp32 features (`k_shade<4,8,32,true>`); "
+           "every other kernel is shared.  `$REPO` is the repository root.\n",
            "Kernel trace:\n", "```\n" + cmd_stats + "\n```\n",
            "| kernel | calls | avg us | min us | max us | % time |", "|---|---:|---:|---:|---:|---:|"]
     for r in rows[:24]:
