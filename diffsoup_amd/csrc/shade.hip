@@ -118,8 +118,13 @@ template <class K, bool FB>
 struct Plan {
     using FT = typename std::conditional<FB, unsigned short, float>::type;
     __host__ __device__ static constexpr int frow_bytes(int rowp) { return up16(rowp * static_cast<int>(sizeof(FT))); }
+    __host__ __device__ static constexpr int stage_bytes() {
+        return up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)));
+    }
+    // feature row, gradient row, transpose stage, and the claim table of the
+    // gradient row's 4-channel blocks (one int each: rowp bytes)
     __host__ __device__ static constexpr int wave_bytes(int rowp) {
-        return frow_bytes(rowp) + 4 * rowp + up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)));
+        return frow_bytes(rowp) + 4 * rowp + stage_bytes() + rowp;
     }
     static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::C) + 3) & ~3;
     static constexpr bool LB = 4 * K::BLOB_ELEMS + kWavesPerBlock * wave_bytes(ROW0) <= 160 * 1024;
@@ -254,6 +259,7 @@ k_shade(ShadeArgs a, int rowp) {
     FT* frow = reinterpret_cast<FT*>(wbase);
     float* grow = reinterpret_cast<float*>(wbase + P::frow_bytes(rowp));
     ST* stage = reinterpret_cast<ST*>(wbase + P::frow_bytes(rowp) + 4 * rowp);
+    int* claim = reinterpret_cast<int*>(wbase + P::frow_bytes(rowp) + 4 * rowp + P::stage_bytes());
     const int row = a.S_tot * C;
 
     // stage the weight blobs
@@ -434,26 +440,56 @@ k_shade(ShadeArgs a, int rowp) {
             // that level's lattice corners.  The two coarsest lattices (3 and
             // 6 vertices) are shared by nearly all pixels of a tile, so they
             // accumulate per lane in registers over the whole segment and are
-            // reduced across the 16 pixel lanes once at its end; finer levels
-            // add into the LDS row with LDS atomics.
+            // reduced across the 16 pixel lanes once at its end.  Finer
+            // levels add into the LDS row.  LDS float atomics are slow there
+            // (the LDS stays busy ~2 cycles per lane), and at those levels the
+            // pixels of a tile rarely share a lattice sample, so each lane
+            // first claims its 4-channel block (it writes its lane id into
+            // the block's claim word and reads it back); the one lane that
+            // reads back its own id owns the block and adds with a plain
+            // 16-byte read-modify-write, and the rare losers add with
+            // atomics afterwards.
             for (int tt = 0; tt < NI; ++tt) {
                 const int f0 = 16 * tt + 4 * g;
-                if (!valid || f0 >= DIN) continue;
-                const int lv = f0 / C, c = f0 % C;
+                const bool in = valid && f0 < DIN;
+                const int lv = in ? f0 / C : 0, c = f0 % C;
                 const Corners& cr = crs[tt];
-                if (tt == 0 && a.lo + lv <= 1) {
+                const bool coarse = tt == 0 && a.lo + lv <= 1;
+                if (in && coarse) {
                     for (int vtx = 0; vtx < 6; ++vtx) {
                         const float wv = (cr.idx[0] == (uint32_t)vtx ? cr.w[0] : 0.f) +
                                          (cr.idx[1] == (uint32_t)vtx ? cr.w[1] : 0.f) +
                                          (cr.idx[2] == (uint32_t)vtx ? cr.w[2] : 0.f);
                         for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
                     }
-                } else {
-                    for (int k = 0; k < 3; ++k) {
-                        float* gl = grow + grad_slot<C>(loff[lv] + cr.idx[k], c);
-                        for (int r = 0; r < 4; ++r) atomicAdd(gl + r, cr.w[k] * dx[tt][r]);
-                    }
                 }
+                const bool fine = in && !coarse;
+                if (__ballot(fine) == 0ull) continue;   // wave-uniform
+                int slot[3];
+                for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<C>(loff[lv] + cr.idx[k], c) : 0;
+                if (fine)
+                    for (int k = 0; k < 3; ++k)
+                        __hip_atomic_store(claim + (slot[k] >> 2), lane, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+                wave_lds_sync();
+                bool own[3];
+                for (int k = 0; k < 3; ++k)
+                    own[k] = fine && __hip_atomic_load(claim + (slot[k] >> 2), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WAVEFRONT) == lane;
+                for (int k = 0; k < 3; ++k)
+                    if (own[k]) {
+                        float4* q = reinterpret_cast<float4*>(grow + slot[k]);
+                        float4 v = *q;
+                        v.x += cr.w[k] * dx[tt][0];
+                        v.y += cr.w[k] * dx[tt][1];
+                        v.z += cr.w[k] * dx[tt][2];
+                        v.w += cr.w[k] * dx[tt][3];
+                        *q = v;
+                    }
+                wave_lds_sync();
+                for (int k = 0; k < 3; ++k)
+                    if (fine && !own[k])
+                        for (int r = 0; r < 4; ++r) atomicAdd(grow + slot[k] + r, cr.w[k] * dx[tt][r]);
             }
 
             // ---- bias gradients stay per lane until the end
