@@ -121,14 +121,22 @@ struct Plan {
     __host__ __device__ static constexpr int stage_bytes() {
         return up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)));
     }
-    // feature row, gradient row, transpose stage, and the claim table of the
-    // gradient row's 4-channel blocks (one int each: rowp bytes)
-    __host__ __device__ static constexpr int wave_bytes(int rowp) {
-        return frow_bytes(rowp) + 4 * rowp + stage_bytes() + rowp;
+    // feature row, gradient row, transpose stage, and (CLAIM) the claim table
+    // of the gradient row's 4-channel blocks (one int each: rowp bytes)
+    __host__ __device__ static constexpr int base_bytes(int rowp) {
+        return frow_bytes(rowp) + 4 * rowp + stage_bytes();
     }
     static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::C) + 3) & ~3;
-    static constexpr bool LB = 4 * K::BLOB_ELEMS + kWavesPerBlock * wave_bytes(ROW0) <= 160 * 1024;
-    static constexpr int BLOB_BYTES = LB ? 4 * K::BLOB_ELEMS : 0;
+    static constexpr int LDS_MAX = 160 * 1024, BLOB = 4 * K::BLOB_ELEMS;
+    // preference, for rows starting at level 0: operands and claim table in
+    // LDS; claim table only; operands only; neither
+    static constexpr bool FIT_BOTH = BLOB + kWavesPerBlock * (base_bytes(ROW0) + ROW0) <= LDS_MAX;
+    static constexpr bool FIT_CLAIM = kWavesPerBlock * (base_bytes(ROW0) + ROW0) <= LDS_MAX;
+    static constexpr bool FIT_BLOB = BLOB + kWavesPerBlock * base_bytes(ROW0) <= LDS_MAX;
+    static constexpr bool CLAIM = FIT_BOTH || FIT_CLAIM;
+    static constexpr bool LB = FIT_BOTH || (!FIT_CLAIM && FIT_BLOB);
+    static constexpr int BLOB_BYTES = LB ? BLOB : 0;
+    __host__ __device__ static constexpr int wave_bytes(int rowp) { return base_bytes(rowp) + (CLAIM ? rowp : 0); }
 };
 
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -259,7 +267,7 @@ k_shade(ShadeArgs a, int rowp) {
     FT* frow = reinterpret_cast<FT*>(wbase);
     float* grow = reinterpret_cast<float*>(wbase + P::frow_bytes(rowp));
     ST* stage = reinterpret_cast<ST*>(wbase + P::frow_bytes(rowp) + 4 * rowp);
-    int* claim = reinterpret_cast<int*>(wbase + P::frow_bytes(rowp) + 4 * rowp + P::stage_bytes());
+    int* claim = reinterpret_cast<int*>(wbase + P::base_bytes(rowp));
     const int row = a.S_tot * C;
 
     // stage the weight blobs
@@ -464,6 +472,15 @@ k_shade(ShadeArgs a, int rowp) {
                     }
                 }
                 const bool fine = in && !coarse;
+                if constexpr (!P::CLAIM) {
+                    // no room for the claim table: LDS atomics
+                    if (fine)
+                        for (int k = 0; k < 3; ++k) {
+                            float* gl = grow + grad_slot<C>(loff[lv] + cr.idx[k], c);
+                            for (int r = 0; r < 4; ++r) atomicAdd(gl + r, cr.w[k] * dx[tt][r]);
+                        }
+                    continue;
+                }
                 if (__ballot(fine) == 0ull) continue;   // wave-uniform
                 int slot[3];
                 for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<C>(loff[lv] + cr.idx[k], c) : 0;
@@ -578,11 +595,12 @@ k_shade(ShadeArgs a, int rowp) {
     for (int r = 0; r < 4; ++r) gb3[r] = row16_sum(gb3[r]);
     for (int d = 32; d > 0; d >>= 1) loss += __shfl_xor(loss, d, kWave);
 
-    // ---- workgroup reduction into LDS (reusing the staging area), one partial per block
+    // ---- workgroup reduction, one partial per block: every wave stores its
+    // accumulators into its own LDS copy of the parameter vector (each entry
+    // is held by exactly one lane, so plain stores; the whole LDS is free by
+    // now), then the block sums the four copies
     __syncthreads();
-    float* red = smem + P::BLOB_BYTES / 4;
-    for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) red[i] = 0.f;
-    __syncthreads();
+    float* red = smem + wave * K::NPARAM;
     float* rW1 = red;
     float* rW2 = rW1 + DH * DIN;
     float* rW3 = rW2 + DH * DH;
@@ -595,24 +613,28 @@ k_shade(ShadeArgs a, int rowp) {
             const int u = 16 * m + 4 * g + r;   // accumulator row
             for (int nn = 0; nn < NI; ++nn) {
                 const int f = 16 * nn + j;
-                if (f < DIN) atomicAdd(&rW1[u * DIN + f], gW1[m][nn][r]);
+                if (f < DIN) rW1[u * DIN + f] = gW1[m][nn][r];
             }
-            for (int nn = 0; nn < NH; ++nn) atomicAdd(&rW2[u * DH + 16 * nn + j], gW2[m][nn][r]);
+            for (int nn = 0; nn < NH; ++nn) rW2[u * DH + 16 * nn + j] = gW2[m][nn][r];
             if (j == 0) {
-                atomicAdd(&rb1[u], gb1[m][r]);
-                atomicAdd(&rb2[u], gb2[m][r]);
+                rb1[u] = gb1[m][r];
+                rb2[u] = gb2[m][r];
             }
         }
         // gW3 rows are output channels 4 g + r (only 0..2 exist), columns hidden units
         if (g == 0)
-            for (int r = 0; r < 3; ++r) atomicAdd(&rW3[r * DH + 16 * m + j], gW3[m][r]);
+            for (int r = 0; r < 3; ++r) rW3[r * DH + 16 * m + j] = gW3[m][r];
     }
     if (g == 0 && j == 0)
-        for (int r = 0; r < 3; ++r) atomicAdd(&rb3[r], gb3[r]);
-    if (lane == 0) atomicAdd(rl, loss);
+        for (int r = 0; r < 3; ++r) rb3[r] = gb3[r];
+    if (lane == 0) *rl = loss;
     __syncthreads();
     float* out = a.partials + (long long)blockIdx.x * K::NPARAM;
-    for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) out[i] = red[i];
+    for (int i = threadIdx.x; i < K::NPARAM; i += blockDim.x) {
+        float v = smem[i];
+        for (int w = 1; w < kWavesPerBlock; ++w) v += smem[w * K::NPARAM + i];
+        out[i] = v;
+    }
 }
 
 // Per-lane weight blobs for one configuration, from the fp32 master weights
@@ -662,7 +684,7 @@ hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     using P = Plan<K, FB>;
     const int rowp = (a.S_tot * K::C + 3) & ~3;
     const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
-    const size_t red = P::BLOB_BYTES + sizeof(float) * K::NPARAM;
+    const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
     const size_t bytes = lds > red ? lds : red;
     if (bytes > 160 * 1024) return hipErrorInvalidValue;
     static size_t opted_in = 65536;

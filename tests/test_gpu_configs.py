@@ -1,7 +1,7 @@
 """The larger workloads on the fused step: batched uneven views (50K-triangle
 unbounded scene, mixed resolutions), bf16 features with the 80->64->64->3
 MLP (100K triangles, 5 levels x 16 channels), and the stress scene (300K
-triangles, 40% stacked depth layers).
+triangles, 40% stacked depth layers), plus the 80->64->64->3 MLP in fp32.
 
 Each runs scaled down against the oracle - raster bit-exact on every view,
 colour and gradients to the fused-step tolerances - and the batched and bf16
@@ -46,6 +46,9 @@ SMALL = {
     "bf16_large": (lambda: scenes.bf16_large_soup(2, T=5000, views=2, resolutions=[(216, 384)], edge_scale=5.0),
                    True),
     "stress": (lambda: scenes.stress_soup(3, T=9000, views=4, res_scale=0.2, edge_scale=5.0), True),
+    # the 80->64->64->3 MLP in fp32: weight operands from global memory, LDS atomics in the scatter
+    "large_fp32": (lambda: scenes.bf16_large_soup(4, T=3000, views=2, resolutions=[(216, 384)], edge_scale=5.0),
+                   False),
 }
 
 
