@@ -294,17 +294,53 @@ k_shade(ShadeArgs a, int rowp) {
         for (int k = 0; k < K::NL; ++k) { loff[k] = o; o += feats_at_level(a.lo + k); }
     }
 
-    // Each wave owns an equal, contiguous range of the triangle-grouped pixel
-    // list and walks the triangle segments inside it.  A triangle whose
-    // pixels all fall in one wave's range gets its gradient row stored; a
-    // triangle split between waves is added with atomics into the zeroed table.
-    // A segment's triangle and end come from its first record (wave-uniform
+    // Each wave owns a contiguous range of the triangle-grouped pixel list
+    // and walks the triangle segments inside it.  A triangle whose pixels all
+    // fall in one wave's range gets its gradient row stored; a triangle split
+    // between waves is added with atomics into the zeroed table.  A
+    // segment's triangle and end come from its first record (wave-uniform
     // loads), so the walk never reads the offset table.
+    //
+    // The ranges split an estimated cost evenly, not the pixel count: a pixel
+    // costs 1 and each triangle's segment kSegCost more (its row staging,
+    // coarse-level reduction and row store, and on average half a tile of
+    // idle lanes), with the triangle index standing in for the number of
+    // non-empty segments before it.  Cost(t) = tri_off[t] + kSegCost t is
+    // monotone, so a 64-way search over triangles (one vector load per
+    // level) finds the segment holding each range boundary.
+    constexpr long long kSegCost = 16;
     const int n_waves = gridDim.x * kWavesPerBlock;
     const int wid = blockIdx.x * kWavesPerBlock + wave;
     const int total = a.tri_off[a.T];
-    const int per = (total + n_waves - 1) / n_waves;
-    const int r0 = min(total, wid * per), r1 = min(total, r0 + per);
+    const long long ctot = total + kSegCost * a.T;
+    int r0, r1;
+    {
+        const long long q0 = ctot * wid / n_waves, q1 = ctot * (wid + 1) / n_waves;
+        // largest t in [0, T] with Cost(t) <= q, for both boundaries at once
+        int lo0 = 0, hi0 = a.T + 1, lo1 = 0, hi1 = a.T + 1;
+        while (hi0 - lo0 > 1 || hi1 - lo1 > 1) {
+            const int st0 = (hi0 - lo0 + kWave - 1) / kWave, st1 = (hi1 - lo1 + kWave - 1) / kWave;
+            const int p0 = lo0 + lane * st0, p1 = lo1 + lane * st1;
+            const bool ok0 = p0 < hi0 && a.tri_off[p0] + kSegCost * p0 <= q0;
+            const bool ok1 = p1 < hi1 && a.tri_off[p1] + kSegCost * p1 <= q1;
+            // lane 0 probes lo, which always qualifies
+            const int k0 = 63 - __builtin_clzll(__ballot(ok0));
+            const int k1 = 63 - __builtin_clzll(__ballot(ok1));
+            lo0 += k0 * st0;
+            hi0 = min(hi0, lo0 + st0);
+            lo1 += k1 * st1;
+            hi1 = min(hi1, lo1 + st1);
+        }
+        // the boundary's pixel inside that triangle's segment
+        auto boundary = [&](long long q, int tq) -> int {
+            if (tq >= a.T) return total;
+            const int tb = a.tri_off[tq], tn = a.tri_off[tq + 1];
+            const long long over = q - (tb + kSegCost * tq) - kSegCost;
+            return over <= 0 ? tb : static_cast<int>(min<long long>(tn, tb + over));
+        };
+        r0 = wid == 0 ? 0 : boundary(q0, lo0);
+        r1 = wid == n_waves - 1 ? total : boundary(q1, lo1);
+    }
     int t = 0, te = 0;
     bool first_whole = true;   // the first segment starts at r0
     if (r0 < r1) {
