@@ -22,6 +22,15 @@ from .multires import (
     rasterize_views,
 )
 from .rasterize import count_triangle_ids, edge_grad, encode_view_dir_sh2, opacity_aux_loss
+from .remesh import (
+    expand_by_index,
+    remove_unreferenced_vertices_from_soup,
+    split_edges_from_training_views,
+    split_triangle_soup,
+    split_triangle_soup_clip,
+    split_triangle_soup_clip_until,
+    split_triangle_soup_until,
+)
 from .visibility import build_keep_map, count_visible_triangles
 
 

@@ -38,7 +38,9 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     for (long long i = i0; i < n_alpha; i += stride) {
-        alpha[i] = 1.0f / (1.0f + expf(-logits[i]));
+        // evaluated in double and rounded once, so the opacity is the
+        // correctly rounded sigmoid (the last bit decides mask ties)
+        alpha[i] = static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(logits[i]))));
         galpha[i] = 0.f;
     }
     float4* gf4 = reinterpret_cast<float4*>(gfeat);

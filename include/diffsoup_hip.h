@@ -337,6 +337,38 @@ int ds_count_triangle_ids(int64_t P, const float* rast, int T, int64_t* counts, 
 /* Fill n floats with value (a kernel, so it is safe inside graph capture). */
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Soup splitting (host).  Longest-edge bisection between training phases;
+ * the outputs are bit-identical to the reference's splitters, in the same
+ * order.  Unlike every other entry point these run on the host, take host
+ * pointers and allocate: the result lives in an opaque ds_soup_split until
+ * ds_split_soup_free.  Sizes are read with ds_split_soup_counts and the
+ * arrays copied out with ds_split_soup_export (any output may be NULL):
+ * verts [nv,3] f32, faces [nt,3] i32, face_map [nt] (source face of each
+ * face), same_as_original [nt] (1 = the source face unchanged).
+ *
+ * ds_split_soup: world-space edge lengths.  Replaces split_triangle_soup
+ * (src/main.cpp:308-345, src/remesh.cpp:118-156): at most num_splits
+ * bisections (num_splits < 0: no cap, then tau must be > 0), stopping once
+ * the longest edge is shorter than tau (tau <= 0: no threshold).
+ *
+ * ds_split_soup_clip: lengths in NDC through one row-major mvp [4,4], x
+ * scaled by aspect_wh = W/H, threshold tau_ratio in image heights; edges
+ * with an end outside the NDC cube are never split; valid_faces [nt] (NULL:
+ * all) selects the faces whose edges are considered.  Replaces
+ * split_triangle_soup_clip (src/main.cpp:347-385, src/remesh_clip.cpp:200-244).
+ * ------------------------------------------------------------------------ */
+typedef struct ds_soup_split ds_soup_split;
+
+int ds_split_soup(const float* verts, int nv, const int* faces, int nt, int num_splits, float tau,
+                  ds_soup_split** out);
+int ds_split_soup_clip(const float* mvp, const float* verts, int nv, const int* faces, int nt,
+                       const int* valid_faces, int num_splits, float tau_ratio, float aspect_wh,
+                       ds_soup_split** out);
+int ds_split_soup_counts(const ds_soup_split* s, int* nv, int* nt);
+int ds_split_soup_export(const ds_soup_split* s, float* verts, int* faces, int* face_map, int* same_as_original);
+void ds_split_soup_free(ds_soup_split* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,9 +41,70 @@ _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _c = ctypes
 
 
+REF_ROOT = "/root/reference"
+_REF_REMESH = os.path.join(_HERE, "_ref", "libref_remesh.so")
+_ref_remesh = None
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.isdir(os.path.join(REF_ROOT, "src")):
+        # the reference's own soup splitters, where its sources are present
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", f"REF={REF_ROOT}"], check=True)
     return _LIB_PATH
+
+
+def ref_remesh_available() -> bool:
+    return os.path.exists(_REF_REMESH)
+
+
+def _ref_remesh_lib():
+    global _ref_remesh
+    if _ref_remesh is None:
+        if not ref_remesh_available():
+            raise FileNotFoundError(f"{_REF_REMESH} is not built (oracle.build() where the reference is present)")
+        L = ctypes.CDLL(_REF_REMESH)
+        ip = ctypes.POINTER(ctypes.c_int)
+        tail = [_c.c_int, _c.c_int, _f32p, _i32p, _i32p, _i32p, ip, ip]
+        L.ref_split_soup.argtypes = [_f32p, _c.c_int, _i32p, _c.c_int, _c.c_int, _c.c_float] + tail
+        L.ref_split_soup_clip.argtypes = ([_f32p, _f32p, _c.c_int, _i32p, _c.c_int, _i32p, _c.c_int, _c.c_float,
+                                           _c.c_float] + tail)
+        _ref_remesh = L
+    return _ref_remesh
+
+
+def _ref_split(call, nv, nt, budget):
+    """Run one reference splitter through the driver, growing the output
+    buffers until they hold the result."""
+    cap_v, cap_t = nv + 3 * max(budget, 0) + 3, nt + max(budget, 0) + 1
+    while True:
+        ov, of = np.zeros((cap_v, 3), np.float32), np.zeros((cap_t, 3), np.int32)
+        om, ofl = np.zeros(cap_t, np.int32), np.zeros(cap_t, np.int32)
+        got_v, got_t = ctypes.c_int(), ctypes.c_int()
+        rc = call(cap_v, cap_t, ov, of, om, ofl, ctypes.byref(got_v), ctypes.byref(got_t))
+        if rc == 0:
+            n, m = got_v.value, got_t.value
+            return ov[:n].copy(), of[:m].copy(), om[:m].copy(), ofl[:m].copy()
+        cap_v, cap_t = got_v.value, got_t.value
+
+
+def ref_split_soup(verts, faces, num_splits, tau=0.0):
+    """The reference's TriangleSoupSplitter (src/remesh.cpp), compiled from its
+    own sources: (verts, faces, face_map, same_as_original) as numpy arrays."""
+    L = _ref_remesh_lib()
+    v, f = _f32(verts).reshape(-1, 3), _i32(faces).reshape(-1, 3)
+    return _ref_split(lambda *o: L.ref_split_soup(v, v.shape[0], f, f.shape[0], int(num_splits), float(tau), *o),
+                      v.shape[0], f.shape[0], int(num_splits))
+
+
+def ref_split_soup_clip(mvp, verts, faces, valid, num_splits, tau_ratio, aspect_wh):
+    """The reference's TriangleSoupSplitterClip (src/remesh_clip.cpp)."""
+    L = _ref_remesh_lib()
+    m, v, f = _f32(mvp).reshape(4, 4), _f32(verts).reshape(-1, 3), _i32(faces).reshape(-1, 3)
+    ok = _i32(valid).reshape(-1)
+    return _ref_split(lambda *o: L.ref_split_soup_clip(m, v, v.shape[0], f, f.shape[0], ok, int(num_splits),
+                                                       float(tau_ratio), float(aspect_wh), *o),
+                      v.shape[0], f.shape[0], int(num_splits))
 
 
 def lib():
