@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdiffsoup_hip.so")
 SOURCES = ["capi.hip", "raster.hip", "multires.hip", "encoding.hip", "edge_grad.hip", "shade.hip", "step.hip", "plain.hip",
-           "scan.hip", "optim.hip", "visibility.hip", "remesh.cpp"]
+           "scan.hip", "optim.hip", "visibility.hip", "ssim.hip", "remesh.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -87,6 +87,7 @@ int plain_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 int optim_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 int visibility_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 int remesh_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+int ssim_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 }  // namespace ds
 
 extern "C" {

@@ -338,6 +338,32 @@ int ds_count_triangle_ids(int64_t P, const float* rast, int T, int64_t* counts, 
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Structural similarity of pytorch_msssim.ssim (Gaussian window of win_size
+ * odd taps <= 11 given in win, "valid" filtering, a side shorter than the
+ * window left unfiltered, c1 = (K1 R)^2, c2 = (K2 R)^2), the term of the
+ * reference's loss 0.8 L1 + 0.2 (1 - SSIM) / 2 (examples/02_synthetic.py:
+ * 376-380).  X (reference image) and Y are [P, C] packed over B views like
+ * rast, 1 <= C <= 4.
+ *
+ * ds_ssim_tiles: the number of window tiles (the length of the partial-sum
+ * array, times C); tile_base (host, B + 1 entries, may be NULL) receives
+ * where each view's tiles start.
+ * ds_ssim_fwd: per-tile, per-channel sums of the SSIM map into partial
+ * [tiles, C], and the three derivative maps into maps [3, P, C] (entries
+ * outside the windows are not written).
+ * ds_ssim_bwd: grad [P, C] = scale[b, c] * dscale[b, c] * dSSIM_sum/dY +
+ * l1_scale * dl1 * sign(Y - X), stored or added (accumulate).  scale is a
+ * host array [B, C]; dscale (device [B, C]) and dl1 (device scalar) may be
+ * NULL (= 1).  For the mean SSIM of one view, scale = 1 / (C H' W').
+ * ------------------------------------------------------------------------ */
+int64_t ds_ssim_tiles(int B, const int* H, const int* W, int win_size, int64_t* tile_base);
+int ds_ssim_fwd(int B, const int* H, const int* W, int C, const float* X, const float* Y, int win_size,
+                const float* win, float c1, float c2, float* maps, float* partial, void* stream);
+int ds_ssim_bwd(int B, const int* H, const int* W, int C, const float* X, const float* Y, int win_size,
+                const float* win, const float* maps, const float* scale, const float* dscale, float l1_scale,
+                const float* dl1, float* grad, int accumulate, void* stream);
+
+/* ------------------------------------------------------------------------
  * Soup splitting (host).  Longest-edge bisection between training phases;
  * the outputs are bit-identical to the reference's splitters, in the same
  * order.  Unlike every other entry point these run on the host, take host

@@ -416,3 +416,70 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
     return dict(rast=rast, color=color.detach().numpy(), grad_color=grad_color32, loss=float(loss.detach()),
                 grad_feat=grad_feat, grad_verts=gverts, grad_alpha_logits=galog, grad_mlp=gmlp,
                 covered=covered, pos=pos, alpha=alpha)
+
+
+# ---------------------------------------------------------------------------
+# Photometric loss (examples/02_synthetic.py:376-380).  SSIM comes from the
+# reference's dependency pytorch_msssim (unpinned in requirements.txt; the
+# algorithm below restates the published pytorch_msssim 1.0.0 functions
+# _fspecial_gauss_1d, gaussian_filter, _ssim and ssim), evaluated in float64
+# with torch on the CPU; gradients come from torch autograd.
+# ---------------------------------------------------------------------------
+
+def gaussian_window_1d(size=11, sigma=1.5):
+    """_fspecial_gauss_1d: exp(-c^2 / (2 sigma^2)) over c = -size//2 .. size//2,
+    normalised, in float32 (as there)."""
+    import torch
+    coords = torch.arange(size, dtype=torch.float32) - size // 2
+    g = torch.exp(-(coords ** 2) / (2 * sigma ** 2))
+    return g / g.sum()
+
+
+def gaussian_filter_valid(x, win):
+    """gaussian_filter: the 1-D window along H then W with no padding (each
+    output is one full window), skipping a side shorter than the window;
+    x (N, C, H, W) float64."""
+    import torch.nn.functional as F
+    C = x.shape[1]
+    w = win.to(x.dtype).reshape(1, 1, 1, -1).repeat(C, 1, 1, 1)
+    out = x
+    if x.shape[2] >= w.shape[-1]:
+        out = F.conv2d(out, w.transpose(2, 3), groups=C)
+    if x.shape[3] >= w.shape[-1]:
+        out = F.conv2d(out, w, groups=C)
+    return out
+
+
+def ssim_map_fp64(X, Y, data_range=1.0, win=None, K=(0.01, 0.03)):
+    """_ssim's per-window map (N, C, H', W') in float64."""
+    import torch
+    win = gaussian_window_1d() if win is None else win
+    X, Y = X.to(torch.float64), Y.to(torch.float64)
+    c1, c2 = (K[0] * data_range) ** 2, (K[1] * data_range) ** 2
+    mu1, mu2 = gaussian_filter_valid(X, win), gaussian_filter_valid(Y, win)
+    s11 = gaussian_filter_valid(X * X, win) - mu1 * mu1
+    s22 = gaussian_filter_valid(Y * Y, win) - mu2 * mu2
+    s12 = gaussian_filter_valid(X * Y, win) - mu1 * mu2
+    cs = (2 * s12 + c2) / (s11 + s22 + c2)
+    return ((2 * mu1 * mu2 + c1) / (mu1 * mu1 + mu2 * mu2 + c1)) * cs
+
+
+def ssim_fp64(X, Y, data_range=1.0, size_average=True, win=None, K=(0.01, 0.03), nonnegative_ssim=False):
+    """pytorch_msssim.ssim on (N, C, H, W) images, float64."""
+    import torch
+    per_channel = torch.flatten(ssim_map_fp64(X, Y, data_range, win, K), 2).mean(-1)
+    if nonnegative_ssim:
+        per_channel = torch.relu(per_channel)
+    return per_channel.mean() if size_average else per_channel.mean(1)
+
+
+def l1_ssim_loss_fp64(color, target, l1_weight=0.8, ssim_weight=0.2):
+    """The reference's image loss on [B, H, W, C] images and its gradient with
+    respect to color (float64, torch autograd): (loss, grad)."""
+    import torch
+    c = torch.as_tensor(np.asarray(color), dtype=torch.float64).clone().requires_grad_(True)
+    t = torch.as_tensor(np.asarray(target), dtype=torch.float64)
+    loss = l1_weight * (t - c).abs().mean() + ssim_weight * 0.5 * (
+        1 - ssim_fp64(t.permute(0, 3, 1, 2), c.permute(0, 3, 1, 2)))
+    loss.backward()
+    return float(loss), c.grad.numpy()

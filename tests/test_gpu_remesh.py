@@ -19,7 +19,7 @@ def test_split_edges_from_training_views(dev):
     H, W = 120, 160
     s = scenes.sphere_torus_soup(4, n_surface=2000, n_float=500, views=5, res=(H, W))
     alpha = (1 / (1 + np.exp(-s.alpha_logits.astype(np.float64)))).astype(np.float32)
-    cap, tau_ratio, gseed = 3, 0.04, 3
+    cap, tau_ratio, gseed = 3, 0.008, 3
     V, F, fmap = ds.split_edges_from_training_views(
         (H, W), torch.from_numpy(s.mvp).to(dev), torch.from_numpy(s.verts).to(dev), torch.from_numpy(s.tri).to(dev),
         s.alpha_level, torch.from_numpy(alpha).to(dev), tau_ratio, cap, generator=torch.Generator().manual_seed(gseed))
@@ -41,7 +41,7 @@ def test_split_edges_from_training_views(dev):
         rv, rf, nxt, _ = ds.split_triangle_soup_clip_until((H, W), torch.from_numpy(chosen[i]), rv, rf,
                                                            torch.from_numpy(valid), tau_ratio)
         rmap = rmap[nxt.numpy()]
-    assert total_valid > 500 and rf.shape[0] > s.T
+    assert total_valid > 300 and rf.shape[0] > s.T + 50   # the views saw faces and split some
     np.testing.assert_array_equal(V.cpu().numpy().view(np.uint32), rv.numpy().view(np.uint32))
     np.testing.assert_array_equal(F.cpu().numpy(), rf.numpy())
     np.testing.assert_array_equal(fmap.cpu().numpy(), rmap)
