@@ -159,4 +159,10 @@ hipError_t launch_edge_grad_sorted(const Views& v, int C, const float* color, co
 hipError_t launch_fill_i32(int* p, long long n, int value, hipStream_t s);
 hipError_t launch_fill_f32(float* p, long long n, float value, hipStream_t s);
 
+// The training step's L1 + SSIM loss (ssim.hip): scratch floats needed, and
+// the launch that writes dLoss/dcolour for every pixel and the loss.
+size_t photo_loss_floats(const Views& views, int C);
+hipError_t launch_photo_loss(const Views& views, int C, const float* target, const float* color, float* scratch,
+                             float w_l1, float w_ssim, float* grad, float* loss, hipStream_t s);
+
 }  // namespace ds

@@ -29,6 +29,12 @@ struct ShadeArgs {
     float* pix_loss;        // [P]
     float grad_scale;       // 2 / (3 P)
     float* partials;        // [nblocks, nparam]
+    // Two-pass use for a loss that couples pixels (L1 + SSIM): with fwd_only
+    // the kernel writes colour, the MSE gradient and the per-pixel L1 and
+    // skips the adjoint; with ext_grad [P, 3] it reads dLoss/dcolour from
+    // there (and writes no colour, gradient, L1 or loss) and runs the adjoint.
+    const float* ext_grad;
+    int fwd_only;
 };
 
 bool shade_supported(int nl, int c, int dh);

@@ -451,18 +451,28 @@ k_shade(ShadeArgs a, int rowp) {
                     const float sg = sigmoidf(y[0][c]);
                     const float rgb = x[0][c];
                     const float out = (1.0f - res) * rgb + res * sg;
-                    const float d = out - tg[c];
-                    loss += d * d;
-                    l1 += fabsf(d);
-                    const float go = a.grad_scale * d;
-                    col[c] = out;
-                    gc[c] = go;
+                    float go;
+                    if (a.ext_grad) {
+                        go = a.ext_grad[(long long)p * 3 + c];
+                    } else {
+                        const float d = out - tg[c];
+                        loss += d * d;
+                        l1 += fabsf(d);
+                        go = a.grad_scale * d;
+                        col[c] = out;
+                        gc[c] = go;
+                    }
                     dxr[c] = go * (1.0f - res);
                     dres += go * (sg - rgb);
                     d3[0][c] = go * res * sg * (1.0f - sg);
                 }
                 dxr[3] = dres;
-                a.pix_loss[p] = l1;
+                if (!a.ext_grad) a.pix_loss[p] = l1;
+            }
+            if (a.fwd_only) {   // wave-uniform: colour only, no adjoint
+                rec = next;
+                rtg = ntg;
+                continue;
             }
 
             // ---- backward through the MLP
@@ -582,6 +592,13 @@ k_shade(ShadeArgs a, int rowp) {
             }
             rec = next;
             rtg = ntg;
+        }
+
+        if (a.fwd_only) {
+            wave_lds_sync();
+            t = t_next;
+            te = te_next;
+            continue;
         }
 
         // ---- the coarse-level sums: reduce over the 16 pixel lanes of each

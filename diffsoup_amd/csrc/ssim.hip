@@ -161,9 +161,11 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
                                                         const float* __restrict__ mb, const float* __restrict__ mc,
                                                         const float* __restrict__ dscale, float l1,
                                                         const float* __restrict__ dl1, float* __restrict__ grad,
-                                                        int accumulate) {
+                                                        int accumulate, float* __restrict__ l1_partial) {
     __shared__ float sm[3][kLH][kLW];
     __shared__ float hs[3][kLH][kTW];
+    __shared__ float red[kThreads / 64];
+    float l1_acc = 0.f;   // sum of |y - x| over this tile's pixels and channels
     const int tile = blockIdx.x;
     const int b = view_of_tile(v, tile);
     const int local = tile - v.tile_base[b];
@@ -225,8 +227,55 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
             const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
             const float gval = sc * (a + 2.f * yv * bb + xv * cc) + l1s * sg;
             grad[o] = accumulate ? grad[o] + gval : gval;
+            l1_acc += fabsf(d);
         }
         __syncthreads();
+    }
+    if (l1_partial) {
+        l1_acc = wave_sum(l1_acc);
+        if ((tid & 63) == 0) red[tid >> 6] = l1_acc;
+        __syncthreads();
+        if (tid == 0) {
+            float t = 0.f;
+            for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+            l1_partial[tile] = t;
+        }
+    }
+}
+
+// The training step's photometric loss from the partial sums, one workgroup:
+// w_l1 * sum|y - x| / (C P) + w_ssim * (1 - SSIM) / 2, SSIM being the mean
+// over views of each view's mean over channels and windows.
+__global__ __launch_bounds__(1024) void k_photo_loss(SsimTiles v, const float* __restrict__ ssim_partial,
+                                                     const float* __restrict__ l1_partial, int n_pix_tiles,
+                                                     float w_l1, float w_ssim, float* __restrict__ loss) {
+    __shared__ double rs[1024 / 64], rl[1024 / 64];
+    const int C = v.C;
+    double s = 0.0, l = 0.0;
+    for (int i = threadIdx.x; i < v.tile_base[v.n] * C; i += blockDim.x) {
+        const int b = view_of_tile(v, i / C);
+        const int kh = v.kh[b], kw = v.kw[b];
+        const double windows = (double)(v.H[b] - kh + 1) * (v.W[b] - kw + 1);
+        s += ssim_partial[i] / windows;
+    }
+    for (int i = threadIdx.x; i < n_pix_tiles; i += blockDim.x) l += l1_partial[i];
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        l += __shfl_xor(l, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        rs[threadIdx.x >> 6] = s;
+        rl[threadIdx.x >> 6] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double st = 0.0, lt = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            st += rs[w];
+            lt += rl[w];
+        }
+        const double ssim = st / ((double)v.n * C);
+        loss[0] = static_cast<float>(w_l1 * lt / ((double)C * v.pix_base[v.n]) + w_ssim * 0.5 * (1.0 - ssim));
     }
 }
 
@@ -284,7 +333,56 @@ hipError_t launch_ssim_grad(int B, const int* H, const int* W, int C, const floa
     if (tiles == 0) return hipSuccess;
     const long long n = t.pix_base[B] * C;
     k_ssim_grad<<<static_cast<unsigned>(tiles), kThreads, 0, s>>>(t, X, Y, maps, maps + n, maps + 2 * n, dscale, l1,
-                                                                    dl1, grad, accumulate);
+                                                                    dl1, grad, accumulate, nullptr);
+    return hipGetLastError();
+}
+
+// ---- the training step's L1 + SSIM loss (views of any resolution)
+
+size_t photo_loss_floats(const Views& views, int C) {
+    SsimTiles w, p;
+    const float ones[kMaxTaps] = {};
+    const long long tw = make_tiles(views.n, views.H, views.W, C, kMaxTaps, ones, true, w);
+    const long long tp = make_tiles(views.n, views.H, views.W, C, kMaxTaps, ones, false, p);
+    if (tw < 0 || tp < 0) return 0;
+    // maps [3, P, C], window partials [tw, C], pixel-tile L1 partials [tp]
+    return 3 * (size_t)w.pix_base[views.n] * C + (size_t)tw * C + (size_t)tp;
+}
+
+hipError_t launch_photo_loss(const Views& views, int C, const float* target, const float* color, float* scratch,
+                             float w_l1, float w_ssim, float* grad, float* loss, hipStream_t s) {
+    // the 11-tap Gaussian of sigma 1.5 in float32, normalised as pytorch_msssim does
+    float win[kMaxTaps];
+    float sum = 0.f;
+    for (int k = 0; k < kMaxTaps; ++k) {
+        const float c = static_cast<float>(k - kMaxTaps / 2);
+        win[k] = expf(-(c * c) / (2.f * 1.5f * 1.5f));
+        sum += win[k];
+    }
+    for (int k = 0; k < kMaxTaps; ++k) win[k] /= sum;
+    SsimTiles w, p;
+    const long long tw = make_tiles(views.n, views.H, views.W, C, kMaxTaps, win, true, w);
+    const long long tp = make_tiles(views.n, views.H, views.W, C, kMaxTaps, win, false, p);
+    if (tw < 1 || tp < 1) return hipErrorInvalidValue;
+    const long long n = w.pix_base[views.n] * C;
+    float* maps = scratch;
+    float* spart = maps + 3 * n;
+    float* lpart = spart + tw * C;
+    const float c1 = 0.01f * 0.01f, c2 = 0.03f * 0.03f;
+    k_ssim_maps<<<static_cast<unsigned>(tw), kThreads, 0, s>>>(w, target, color, c1, c2, maps, maps + n,
+                                                                 maps + 2 * n, spart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int b = 0; b < views.n; ++b) {
+        const double windows = (double)(views.H[b] - p.kh[b] + 1) * (views.W[b] - p.kw[b] + 1);
+        for (int c = 0; c < C; ++c)
+            p.scale[b * C + c] = static_cast<float>(-0.5 * w_ssim / ((double)views.n * C * windows));
+    }
+    const float l1s = static_cast<float>(w_l1 / ((double)C * w.pix_base[views.n]));
+    k_ssim_grad<<<static_cast<unsigned>(tp), kThreads, 0, s>>>(p, target, color, maps, maps + n, maps + 2 * n,
+                                                                 nullptr, l1s, nullptr, grad, 0, lpart);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_photo_loss<<<1, 1024, 0, s>>>(w, spart, lpart, static_cast<int>(tp), w_l1, w_ssim, loss);
     return hipGetLastError();
 }
 

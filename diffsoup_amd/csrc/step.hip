@@ -5,7 +5,8 @@
 //   k_blobs         per-lane MLP weight operands (shade.hip)
 //   raster bin      setup / scan / emit (raster.hip)
 //   resolve+shade   depth test, background shading, covered-pixel list
-//   k_shade         gather -> MLP -> blend -> MSE, MLP adjoint, feature scatter
+//   k_shade         gather -> MLP -> blend -> loss gradient, MLP adjoint,
+//                   feature scatter
 //   edge grad       silhouette / intersection gradients into clip positions
 //   opacity aux     score-function opacity gradient (same bins, no re-binning)
 //   k_epilogue      MLP weight-gradient reduction, the loss, clip -> world
@@ -13,8 +14,11 @@
 //
 // No allocation, no host synchronisation: everything lives in the caller's
 // workspace and output buffers, so the sequence can be captured in a graph.
-// Composition follows examples/02_synthetic.py:354-388 with MSE in place of
-// L1 + SSIM and the features concatenated per level.
+// Composition follows examples/02_synthetic.py:354-388 with the features
+// concatenated per level; the image loss is MSE (one shading pass that forms
+// the loss gradient per pixel) or the reference's 0.8 L1 + 0.2 (1 - SSIM) / 2
+// (a colour pass, the loss kernels of ssim.hip over the whole image, then the
+// adjoint pass on their gradient).
 #include "diffsoup_hip.h"
 #include "ds_kernels.h"
 #include "ds_math.h"
@@ -104,7 +108,7 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
         for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
         if (tx == 0) acc[0][ty] = sum;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && loss) {   // null when another kernel forms the loss
             float t = 0.f;
             for (int q = 0; q < 16; ++q) t += acc[0][q];
             *loss = t * scale;
@@ -149,7 +153,7 @@ inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct StepLayout {
     size_t raster, gpos, galpha, tile_list, tile_ncov, tri_count, tri_off, list, scan, bgloss, blobs, partials,
-        tile_px_off, px_list, scan2,
+        tile_px_off, px_list, scan2, photo,
         total;
     int n_tiles, nblocks, nparam;
     long long P;
@@ -187,6 +191,12 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
     L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(nl, d->channels, d->hidden));
     L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
+    L.photo = o;
+    if (d->loss_kind == DS_LOSS_L1_SSIM) {
+        const size_t nf = photo_loss_floats(v, 3);
+        if (nf == 0) return false;
+        o += up256(sizeof(float) * nf);
+    }
     L.total = o;
     return true;
 }
@@ -223,6 +233,8 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     if (d->feat_lo < 0 || nl < 1 || d->feat_hi > 10 || d->alpha_level < 0 || d->alpha_level > 10)
         return step_fail(DS_ERR_INVALID, "level range invalid");
     if (d->T < 0 || d->V < 0 || d->T > (1 << 24) - 1) return step_fail(DS_ERR_INVALID, "bad T or V");
+    if (d->loss_kind != DS_LOSS_MSE && d->loss_kind != DS_LOSS_L1_SSIM)
+        return step_fail(DS_ERR_INVALID, "loss_kind must be DS_LOSS_MSE or DS_LOSS_L1_SSIM");
     StepLayout L;
     if (!layout(d, v, L)) return step_fail(DS_ERR_UNSUPPORTED, "no compiled shading kernel for this (levels, channels, hidden)");
     if (!d->workspace || d->workspace_bytes < L.total)
@@ -308,7 +320,19 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
                  reinterpret_cast<const unsigned short*>(d->feat_bf16), d->grad_feat,
                  static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->color,
                  d->grad_color, d->pix_loss, 2.0f * inv_n, partials};
+    const bool photo = d->loss_kind == DS_LOSS_L1_SSIM;
     DS_TRY(mark(DS_STAGE_SHADE, true), "event");
+    if (photo) {
+        // the loss couples neighbouring pixels: colour first, then the
+        // L1 + SSIM gradient of the whole image, then the adjoint on it
+        sa.fwd_only = 1;
+        DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade (colour)");
+        DS_TRY(launch_photo_loss(v, 3, d->target, d->color, reinterpret_cast<float*>(ws + L.photo), 0.8f, 0.2f,
+                                 d->grad_color, d->loss, s),
+               "L1 + SSIM loss");
+        sa.fwd_only = 0;
+        sa.ext_grad = d->grad_color;
+    }
     DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade");
     DS_TRY(mark(DS_STAGE_SHADE, false), "event");
     DS_TRY(mark(DS_STAGE_EDGE, true), "event");
@@ -324,7 +348,8 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
         const long long nf = std::max<long long>(d->V, n_alpha);
         const unsigned R = static_cast<unsigned>((L.nparam + 63) / 64);
         k_epilogue<<<R + 2 + blocks_for(nf, 1024), 1024, 0, s>>>(
-            partials, L.nblocks, L.nparam, d->grad_mlp, bgloss, L.n_tiles, inv_n, d->loss, d->B, d->V, d->mvp,
+            partials, L.nblocks, L.nparam, d->grad_mlp, bgloss, L.n_tiles, inv_n, photo ? nullptr : d->loss, d->B,
+            d->V, d->mvp,
             reinterpret_cast<const float4*>(gpos), d->grad_verts, n_alpha, d->alpha, galpha, d->grad_alpha_logits,
             rw.tile_cov, L.n_tiles, rw.stats);
         DS_TRY(hipGetLastError(), "epilogue");

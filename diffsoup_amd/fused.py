@@ -2,9 +2,10 @@
 
 The step reproduces the reference's training iteration
 (examples/02_synthetic.py:354-388) for a triangle soup with per-level
-concatenated multires features and an MSE loss, in HIP kernels:
-projection, stochastic-opacity rasterization, background shading,
-gather + pixel MLP + blend (fp32 or bf16 MFMA), MSE and its gradient, the MLP
+concatenated multires features, in HIP kernels: projection,
+stochastic-opacity rasterization, background shading, gather + pixel MLP +
+blend (fp32 or bf16 MFMA), the image loss and its gradient (MSE, or with
+``loss="l1_ssim"`` the reference's 0.8 L1 + 0.2 (1 - SSIM) / 2), the MLP
 adjoint and feature scatter, edge gradients, the opacity auxiliary gradient
 and the world-space vertex gradient.  Views may differ in resolution.
 
@@ -45,7 +46,11 @@ class StepDesc(ctypes.Structure):
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("capacity", ctypes.c_int64),
         ("timed_stage", ctypes.c_int), ("ev_begin", ctypes.c_void_p), ("ev_end", ctypes.c_void_p),
         ("seed_device", ctypes.c_void_p), ("feat_bf16", ctypes.c_void_p),
+        ("loss_kind", ctypes.c_int),
     ]
+
+
+LOSS_KINDS = {"mse": 0, "l1_ssim": 1}
 
 
 STAGES = {"prep": 1, "blobs": 2, "project": 3, "bin": 4, "resolve": 5, "shade": 6, "edge": 7, "aux": 8,
@@ -95,7 +100,8 @@ class FusedStep:
     """Pre-allocated buffers and workspace for ds_train_step on one set of views."""
 
     def __init__(self, resolutions, T, V, alpha_level, levels, channels, hidden, *, bf16=False,
-                 bf16_features=False, stochastic=True, tile=16, background=1.0, device="cuda", capacity=None):
+                 bf16_features=False, stochastic=True, tile=16, background=1.0, device="cuda", capacity=None,
+                 loss="mse"):
         self.dev = torch.device(device)
         self.res = [(int(h), int(w)) for h, w in resolutions]
         self.B = len(self.res)
@@ -106,6 +112,9 @@ class FusedStep:
         self.C, self.hidden = channels, hidden
         self.bf16, self.stochastic, self.tile, self.background = bf16, stochastic, tile, background
         self.bf16_features = bool(bf16_features)
+        if loss not in LOSS_KINDS:
+            raise ValueError(f"loss must be one of {sorted(LOSS_KINDS)}")
+        self.loss_kind = loss
         self.Sa = feats_at_level(alpha_level)
         self.S_tot = feats_in_levels(self.lo, self.hi)
         self.nl = self.hi - self.lo + 1
@@ -144,6 +153,7 @@ class FusedStep:
         d.feat_lo, d.feat_hi, d.channels = self.lo, self.hi, self.C
         d.hidden, d.bf16 = self.hidden, int(bool(self.bf16))
         d.background = self.background
+        d.loss_kind = LOSS_KINDS[self.loss_kind]
         for name in ("pos", "alpha", "rast", "color", "grad_color", "pix_loss", "grad_verts",
                      "grad_alpha_logits", "grad_feat", "grad_mlp", "loss"):
             setattr(d, name, getattr(self, name).data_ptr())

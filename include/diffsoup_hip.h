@@ -189,7 +189,7 @@ typedef struct ds_step_desc {
     float* grad_alpha_logits; /* out [T,S_a]                                 */
     float* grad_feat;         /* out [T,S_tot,channels]                      */
     float* grad_mlp;          /* out [ds_step_mlp_params]: dW1,dW2,dW3,db1,db2,db3,loss_sum */
-    float* loss;              /* out [1] mean squared error                  */
+    float* loss;              /* out [1] the loss (MSE unless loss_kind)     */
     void* workspace;          /* 256-byte aligned                            */
     size_t workspace_bytes;
     int64_t capacity;         /* (triangle, tile) bin capacity               */
@@ -204,7 +204,17 @@ typedef struct ds_step_desc {
                                     bf16 copy [T, S_tot, channels] (round to
                                     nearest even of the fp32 master) instead of
                                     feat; grad_feat stays fp32 */
+    int loss_kind;               /* DS_LOSS_MSE (0): mean squared error; or
+                                    DS_LOSS_L1_SSIM: the reference's
+                                    0.8 L1 + 0.2 (1 - SSIM) / 2 (examples/
+                                    02_synthetic.py:376-380), SSIM the mean over
+                                    views of pytorch_msssim.ssim per view; loss
+                                    then holds that value and grad_color its
+                                    gradient (pix_loss stays the per-pixel L1) */
 } ds_step_desc;
+
+#define DS_LOSS_MSE 0
+#define DS_LOSS_L1_SSIM 1
 
 /* Stages of ds_train_step that timed_stage can select. */
 #define DS_STAGE_PREP 1     /* sigmoid, zero accumulators                    */

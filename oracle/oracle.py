@@ -354,13 +354,15 @@ def project(verts, mvp):
     return out
 
 
-def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, grad_color_in=None):
+def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, grad_color_in=None, loss="mse"):
     """Full forward + backward of the fused step (views of any resolution).
 
     Returns a dict with rast, color, grad_color, loss and the gradients of the
     MLP (torch layout), features, world vertices and opacity logits.  With
     color_in / grad_color_in given, the edge and opacity gradients use those
-    images (to isolate kernels from upstream rounding)."""
+    images (to isolate kernels from upstream rounding).  loss is "mse" or
+    "l1_ssim": 0.8 mean|target - color| + 0.2 (1 - SSIM) / 2 with SSIM the
+    mean over views of ssim_fp64 per view (examples/02_synthetic.py:376-380)."""
     import torch
 
     dtype = dtype or torch.float64
@@ -391,13 +393,24 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
     blend = (1 - r3) * Xt[:, :3] + r3 * h
     tgt = torch.tensor(s.target, dtype=dtype)
     color = torch.ones(P, 3, dtype=dtype).index_put((idx,), blend)
-    loss = ((color - tgt) ** 2).mean()
+    gcol = None
+    if loss == "mse":
+        loss = ((color - tgt) ** 2).mean()
+    else:
+        views = [ssim_fp64(tgt[off[b]:off[b + 1]].reshape(1, H, W, 3).permute(0, 3, 1, 2),
+                           color[off[b]:off[b + 1]].reshape(1, H, W, 3).permute(0, 3, 1, 2))
+                 for b, (H, W) in enumerate(res)]
+        loss = 0.8 * (tgt - color).abs().mean() + 0.2 * 0.5 * (1 - torch.stack(views).mean())
+        gcol = torch.autograd.grad(loss, color, retain_graph=True)[0]
     loss.backward()
     dX = np.zeros(X.shape, np.float32)
     dX[covered] = Xt.grad.numpy()
     grad_feat = multires_color_bwd(rast, lo, hi, s.feat.shape, dX, True)
     color32 = color.detach().numpy().astype(np.float32)
-    grad_color32 = (2.0 * (color.detach() - tgt) / color.numel()).numpy().astype(np.float32)
+    if gcol is not None:
+        grad_color32 = gcol.numpy().astype(np.float32)
+    else:
+        grad_color32 = (2.0 * (color.detach() - tgt) / color.numel()).numpy().astype(np.float32)
     ci = color32 if color_in is None else color_in
     gi = grad_color32 if grad_color_in is None else grad_color_in
     gpos = np.zeros((B, pos.shape[1], 4), np.float32)

@@ -26,14 +26,14 @@ def _close(got, ref, rel, what):
     assert err <= rel * scale, f"{what}: max err {err:.3e} > {rel:.1e} x {scale:.3e}"
 
 
-def _run(dev, s, bf16, seed=5, bf16_features=False):
+def _run(dev, s, bf16, seed=5, bf16_features=False, loss="mse"):
     from diffsoup_amd import fused
     prm = fused.SoupParams.from_soup(s, dev)
     if bf16_features:
         prm.with_bf16_features()
     nl = s.levels[1] - s.levels[0] + 1
     st = fused.FusedStep(s.resolutions, s.T, s.verts.shape[0], s.alpha_level, s.levels, s.feat.shape[2],
-                         s.mlp_dims[1], bf16=bf16, bf16_features=bf16_features, device=dev)
+                         s.mlp_dims[1], bf16=bf16, bf16_features=bf16_features, device=dev, loss=loss)
     loss = st.run(prm, torch.from_numpy(s.mvp).to(dev), torch.from_numpy(s.target).to(dev), seed)
     torch.cuda.synchronize()
     out = {k: getattr(st, k).cpu().numpy() for k in ("pos", "alpha", "rast", "color", "grad_color",
@@ -155,3 +155,33 @@ def test_fused_fd_continuous_params(dev, small):
         assert abs(dW1[i, j] - fd) <= 2e-3 * abs(fd) + 1e-8, (i, j, dW1[i, j], fd)
         checked += 1
     assert checked == 14
+
+
+@pytest.mark.parametrize("which", ["small", "uneven", "medium"])
+def test_fused_step_l1_ssim_loss(dev, small, medium, which):
+    """The reference's image loss 0.8 L1 + 0.2 (1 - SSIM) / 2 inside the fused
+    step (colour pass, SSIM kernels, adjoint pass) against the oracle step
+    with the loss and its colour gradient in float64 (SSIM per view, averaged
+    over views).  Tolerances: colour 1e-5 and loss 1e-5 relative (fp32), the
+    colour gradient 1e-3 of its largest magnitude (fp32 SSIM moments), the
+    parameter gradients 2e-3 (they inherit it); edge and opacity gradients
+    1e-4 on the step's own colour and colour gradient."""
+    s = {"small": small, "medium": medium,
+         "uneven": scenes.batched_uneven_soup(5, T=4000, views=5, res_scale=0.2, edge_scale=5.0)}[which]
+    print(f"fused step with L1 + SSIM: {s.name} views {s.resolutions}", flush=True)
+    g = _run(dev, s, False, seed=7, loss="l1_ssim")
+    rast = np.concatenate([O.rasterize((h, w), g["pos"][b:b + 1], s.tri, s.alpha_level, g["alpha"], True, 7,
+                                       view_base=b).reshape(-1, 4) for b, (h, w) in enumerate(s.resolutions)])
+    np.testing.assert_array_equal(g["rast"].view(np.uint32), rast.view(np.uint32))
+    ref = O.reference_step(s, 7, pos=g["pos"], alpha=g["alpha"], color_in=g["color"],
+                           grad_color_in=g["grad_color"], loss="l1_ssim")
+    err = np.abs(g["color"] - ref["color"]).max()
+    print(f"  colour err {err:.3e}; loss {g['loss']:.7f} vs {ref['loss']:.7f}", flush=True)
+    assert err <= 1e-5
+    assert abs(g["loss"] - ref["loss"]) <= 1e-5 * ref["loss"]
+    np.testing.assert_allclose(g["pix_loss"], np.abs(g["color"] - s.target).sum(1), rtol=1e-6, atol=1e-6)
+    _close(g["grad_color"], ref["grad_color"], 1e-3, "colour grads")
+    _close(g["grad_mlp"][:-1], ref["grad_mlp"], 2e-3, "MLP grads")
+    _close(g["grad_feat"], ref["grad_feat"], 2e-3, "feature grads")
+    _close(g["grad_verts"], ref["grad_verts"], 1e-4, "vertex grads")
+    _close(g["grad_alpha_logits"], ref["grad_alpha_logits"], 1e-4, "opacity logit grads")
