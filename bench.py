@@ -93,13 +93,15 @@ def main():
     dims = dict(T=soup.T, V=soup.verts.shape[0], alpha_level=soup.alpha_level, levels=soup.levels,
                 channels=soup.feat.shape[2], hidden=soup.mlp_dims[1])
 
-    def make(bf16, bf16_features=None):
+    def make(bf16, bf16_features=None, loss="mse"):
         """bf16: the pixel MLP on bf16 MFMA; bf16_features (default: as bf16)
         also stores the features in bf16.  Accumulation, depth, barycentrics
-        and every gradient stay fp32."""
+        and every gradient stay fp32.  loss: "mse" or "l1_ssim" (the
+        reference's 0.8 L1 + 0.2 (1 - SSIM) / 2)."""
         f16 = bf16 if bf16_features is None else bf16_features
         return fused.FusedStep(soup.resolutions, dims["T"], dims["V"], dims["alpha_level"], dims["levels"],
-                               dims["channels"], dims["hidden"], bf16=bf16, bf16_features=f16, device=dev)
+                               dims["channels"], dims["hidden"], bf16=bf16, bf16_features=f16, device=dev,
+                               loss=loss)
 
     def timed(step, steps, warmup, stage=None):
         for i in range(warmup):
@@ -124,14 +126,14 @@ def main():
         step.time_stage(None)
         return ms, wall / steps * 1e3, kms
 
-    def timed_graph(bf16, steps, warmup, bf16_features=None):
+    def timed_graph(bf16, steps, warmup, bf16_features=None, loss="mse"):
         """The captured training iteration: forward + backward + Adam/VectorAdam,
         one graph replay per step.  The optimiser reads and writes every
         parameter and moment each iteration but stores the updated parameters
         beside the inputs, so each iteration sees the configured scene (trained
         in place on a random target, the soup's triangles grow and the workload
         would drift with the step count)."""
-        step = make(bf16, bf16_features)
+        step = make(bf16, bf16_features, loss)
         params = fused.SoupParams.from_soup(soup, dev)
         updated = fused.SoupParams.from_soup(soup, dev)
         if step.bf16_features:
@@ -192,6 +194,10 @@ def main():
         msm, _, gm, _ = timed_graph(True, args.steps, args.warmup, bf16_features=False)
         del gm
         out["bf16_mlp_fp32_features_it_s"] = round(1000.0 / msm, 2)
+    # the same iteration with the reference's own image loss, 0.8 L1 + 0.2 (1 - SSIM) / 2
+    msl, _, gl, _ = timed_graph(False, args.steps, args.warmup, loss="l1_ssim")
+    del gl
+    out["l1_ssim_loss_it_s"] = round(1000.0 / msl, 2)
 
     # forward + backward only (no optimiser, fixed parameters, eager launches)
     st32 = make(False)
