@@ -358,7 +358,10 @@ def _kernel_key(name):
     if m:
         k = m.group(1)
         t = re.search(r"Cfg<([^>]*)>", name) or re.search(r"k_\w+<([^>(]*)>", name)
-        rows = ",bf16rows" if re.search(r"Cfg<[^>]*>,\s*true>", name) else ""
+        # k_shade<Cfg<...>, bf16 rows, two-pass form>
+        flags = re.search(r"Cfg<[^>]*>,\s*(true|false)(?:,\s*(true|false))?>", name)
+        rows = ",bf16rows" if flags and flags.group(1) == "true" else ""
+        rows += ",two-pass" if flags and flags.group(2) == "true" else ""
         return k + ("<" + t.group(1).replace(" ", "") + rows + ">" if t else "")
     if "neg" in name:
         return "calibration torch.neg 256 MiB"
@@ -394,11 +397,13 @@ def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
                "gpurun_out/pmc_WRITE_SIZE/run_counter_collection.csv " + os.path.relpath(PROFILE_MD, ROOT))
     out = ["# Fused training step: sphere+torus soup, 15K triangles, 8 views 800x800\n",
            "bench.py runs the fp32-MLP step (`k_shade<4,8,32,false>`), the bf16-MLP step with bf16 "
-           "feature rows (`k_shade<4,8,32,true,bf16rows>`) and with fp32 features (`k_shade<4,8,32,true>`); "
-           "every other kernel is shared.  `$REPO` is the repository root.\n",
+           "feature rows (`k_shade<4,8,32,true,bf16rows>`) and with fp32 features (`k_shade<4,8,32,true>`), "
+           "and the fp32 step with the L1 + SSIM loss (two `k_shade<4,8,32,false,two-pass>` passes around "
+           "`k_ssim_maps<3>`, `k_ssim_grad<3>` and `k_photo_loss`); every other kernel is shared.  `$REPO` is "
+           "the repository root.\n",
            "Kernel trace:\n", "```\n" + cmd_stats + "\n```\n",
            "| kernel | calls | avg us | min us | max us | % time |", "|---|---:|---:|---:|---:|---:|"]
-    for r in rows[:24]:
+    for r in rows[:30]:
         out.append(f"| `{_kernel_key(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                    f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
     out += ["\nHBM traffic per launch (two counter runs, one counter each):\n", "```\n" + cmd_pmc + "\n```\n",
@@ -407,7 +412,7 @@ def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
             f"x{fs:.3f}, write x{ws:.3f}.\n",
             "| kernel | fetch MiB | write MiB | traffic MiB |", "|---|---:|---:|---:|"]
     order = sorted(res, key=lambda k: -(res[k].get("FETCH_SIZE", 0) * fs + res[k].get("WRITE_SIZE", 0) * ws))
-    for k in order[:14]:
+    for k in order[:20]:
         if k.startswith("calibration") or not k.startswith("k_"):
             continue
         f = res[k].get("FETCH_SIZE", 0) * fs / 1024
@@ -419,9 +424,10 @@ def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
 
 
 def profile_driver(soup, dev):
-    """Counter-collection driver: four fp32 fused steps, then torch.neg on a
-    256 MiB tensor, a kernel that reads and writes exactly 256 MiB and so
-    calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    """Counter-collection driver: four fp32 fused steps, two with the L1 +
+    SSIM loss, then torch.neg on a 256 MiB tensor, a kernel that reads and
+    writes exactly 256 MiB and so calibrates the FETCH_SIZE / WRITE_SIZE
+    counters."""
     import torch
 
     from diffsoup_amd import fused
@@ -433,6 +439,11 @@ def profile_driver(soup, dev):
                          soup.feat.shape[2], soup.mlp_dims[1], device=dev)
     for i in range(4):
         st.run(prm, mvp, target, seed=100 + i)
+    del st
+    sl = fused.FusedStep(soup.resolutions, soup.T, soup.verts.shape[0], soup.alpha_level, soup.levels,
+                         soup.feat.shape[2], soup.mlp_dims[1], device=dev, loss="l1_ssim")
+    for i in range(2):
+        sl.run(prm, mvp, target, seed=200 + i)
     a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
     b = torch.empty_like(a)
     for _ in range(2):

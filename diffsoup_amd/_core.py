@@ -10,8 +10,9 @@ allocate internally. As with nanobind, tensors of the wrong dtype, rank or
 layout raise TypeError. A feature tensor whose size does not match the
 level range raises RuntimeError("Invalid feature size.").
 
-Remeshing (split_triangle_soup*) is not part of the GPU path and is not
-provided.
+The two soup splitters (split_triangle_soup, split_triangle_soup_clip,
+src/main.cpp:308-385) take and return NumPy arrays, as the reference's do,
+and run on the host (ds_split_soup / ds_split_soup_clip).
 
 The reference's Python layer needs one line changed to run on MI355X:
 ``from diffsoup_amd import _core``.
@@ -31,6 +32,7 @@ __all__ = [
     "multires_triangle_alpha", "backward_multires_triangle_alpha",
     "multires_triangle_color", "backward_multires_triangle_color",
     "accumulate_to_level_forward", "accumulate_to_level_backward",
+    "split_triangle_soup", "split_triangle_soup_clip",
 ]
 
 _INT32_MAX = 2**31 - 1
@@ -284,3 +286,45 @@ def accumulate_to_level_backward(min_level, max_level, target_level, grad_feat_a
     _lib.check(_lib.lib().ds_accumulate_bwd(T, int(min_level), int(max_level), int(target_level), C,
                                             _lib.ptr(grad_feat_all), _lib.ptr(grad_feat_max), _stream(grad_feat_all)),
                "accumulate_to_level_backward")
+
+
+def _np_arr(a, dtype, shape, name):
+    """nanobind's ndarray<dtype, numpy, shape<...>, c_contig> check."""
+    import numpy as np
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or a.ndim != len(shape) or not a.flags.c_contiguous or any(
+            s != -1 and s != d for s, d in zip(shape, a.shape)):
+        raise TypeError(f"{name}: expected a C-contiguous {np.dtype(dtype).name} NumPy array of shape "
+                        f"{tuple('*' if s == -1 else s for s in shape)}")
+    return a
+
+
+def split_triangle_soup(verts, faces, numSplits, tau):
+    """Longest-edge bisection in world space (src/main.cpp:308-345): NumPy in,
+    (verts (N',3) f32, faces (M',3) i32, face_mapping (M',), face_flags (M',))
+    NumPy out."""
+    import numpy as np
+    from .remesh import split_triangle_soup as _split
+    v = _np_arr(verts, np.float32, (-1, 3), "verts")
+    f = _np_arr(faces, np.int32, (-1, 3), "faces")
+    out = _split(torch.from_numpy(v), torch.from_numpy(f), int(numSplits), float(tau))
+    return tuple(t.numpy() for t in out)
+
+
+def split_triangle_soup_clip(mvp, verts, faces, valid_faces, numSplits, tau_ratio, aspectWH):
+    """Longest-edge bisection by NDC length through one camera
+    (src/main.cpp:347-385), NumPy in and out."""
+    import ctypes
+
+    import numpy as np
+    from . import remesh as _remesh
+    m = _np_arr(mvp, np.float32, (4, 4), "mvp")
+    v = _np_arr(verts, np.float32, (-1, 3), "verts")
+    f = _np_arr(faces, np.int32, (-1, 3), "faces")
+    ok = _np_arr(valid_faces, np.int32, (-1,), "valid_faces")
+    if ok.shape[0] != f.shape[0]:
+        raise TypeError("valid_faces: one entry per face expected")
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().ds_split_soup_clip(_remesh._ptr(m), _remesh._ptr(v), v.shape[0], _remesh._ptr(f),
+                                             f.shape[0], _remesh._ptr(ok), int(numSplits), float(tau_ratio),
+                                             float(aspectWH), ctypes.byref(h)), "split_triangle_soup_clip")
+    return tuple(t.numpy() for t in _remesh._collect(h, torch.device("cpu")))

@@ -252,7 +252,9 @@ __device__ __forceinline__ void dense(const float* blobs, int base, int lane, co
             out[m] = Mma<K::BF16>::tile(blob_at<K>(blobs, base, m, NIN, t), lane, in[t], out[m]);
 }
 
-template <class K, bool FB>
+// SPLIT selects the two-pass forms (a.fwd_only, a.ext_grad); the fused
+// single-pass instance carries neither test.
+template <class K, bool FB, bool SPLIT>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock, K::MIN_BLOCKS)
 k_shade(ShadeArgs a, int rowp) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
@@ -452,7 +454,7 @@ k_shade(ShadeArgs a, int rowp) {
                     const float rgb = x[0][c];
                     const float out = (1.0f - res) * rgb + res * sg;
                     float go;
-                    if (a.ext_grad) {
+                    if (SPLIT && a.ext_grad) {
                         go = a.ext_grad[(long long)p * 3 + c];
                     } else {
                         const float d = out - tg[c];
@@ -467,9 +469,9 @@ k_shade(ShadeArgs a, int rowp) {
                     d3[0][c] = go * res * sg * (1.0f - sg);
                 }
                 dxr[3] = dres;
-                if (!a.ext_grad) a.pix_loss[p] = l1;
+                if (!(SPLIT && a.ext_grad)) a.pix_loss[p] = l1;
             }
-            if (a.fwd_only) {   // wave-uniform: colour only, no adjoint
+            if (SPLIT && a.fwd_only) {   // wave-uniform: colour only, no adjoint
                 rec = next;
                 rtg = ntg;
                 continue;
@@ -594,7 +596,7 @@ k_shade(ShadeArgs a, int rowp) {
             rtg = ntg;
         }
 
-        if (a.fwd_only) {
+        if (SPLIT && a.fwd_only) {
             wave_lds_sync();
             t = t_next;
             te = te_next;
@@ -732,6 +734,20 @@ __global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ 
     }
 }
 
+template <class K, bool FB, bool SPLIT>
+hipError_t shade_launch(const ShadeArgs& a, int nblocks, int rowp, size_t bytes, hipStream_t s) {
+    static size_t opted_in = 65536;   // per instance: more LDS needs the kernel's opt-in
+    if (bytes > opted_in) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K, FB, SPLIT>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 static_cast<int>(bytes));
+        if (e != hipSuccess) return e;
+        opted_in = bytes;
+    }
+    k_shade<K, FB, SPLIT><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
+    return hipGetLastError();
+}
+
 template <class K, bool FB>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     using P = Plan<K, FB>;
@@ -740,16 +756,8 @@ hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
     const size_t bytes = lds > red ? lds : red;
     if (bytes > 160 * 1024) return hipErrorInvalidValue;
-    static size_t opted_in = 65536;
-    if (bytes > opted_in) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K, FB>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 static_cast<int>(bytes));
-        if (e != hipSuccess) return e;
-        opted_in = bytes;
-    }
-    k_shade<K, FB><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
-    return hipGetLastError();
+    if (a.fwd_only || a.ext_grad) return shade_launch<K, FB, true>(a, nblocks, rowp, bytes, s);
+    return shade_launch<K, FB, false>(a, nblocks, rowp, bytes, s);
 }
 
 template <class K>

@@ -15,16 +15,22 @@
 // per-window maps: dSSIM/dy(p) = sum_q G(q, p) [a(q) + 2 y(p) b(q) + x(p) c(q)]
 // with a = dS/dmy, b = dS/deyy, c = dS/dexy (each holding the others fixed).
 //
-//   k_ssim_maps  one workgroup per 64x16 tile of windows: x and y with a
-//                10-pixel halo staged in LDS one channel at a time, the
-//                horizontal pass of the five sums into LDS, the vertical pass
-//                in registers, then S (a per-tile partial sum) and the maps
-//                a, b, c written at the window's top-left pixel.
-//   k_ssim_grad  one workgroup per 64x16 tile of pixels: the three maps with
-//                their halo in LDS (zero outside the windows), the same two
-//                passes (the window is symmetric, so the transposed filter is
-//                the filter), then scale (a + 2 y b + x c) plus the L1 term
-//                l1 sign(y - x), stored or added to the gradient.
+//   k_ssim_maps  one workgroup per 64x16 tile of windows: x and y of all
+//                channels with a 10-pixel halo staged in LDS in one pass over
+//                contiguous row segments, then per channel the horizontal pass
+//                of the five sums into LDS, the vertical pass in registers, S
+//                (a per-tile partial sum) and the maps a, b, c written
+//                channel-planar at the window's top-left pixel.
+//   k_ssim_grad  one workgroup per 64x16 tile of pixels: per channel the
+//                three maps with their halo in LDS (zero outside the windows)
+//                and the same two passes (the window is symmetric, so the
+//                transposed filter is the filter); scale (a + 2 y b + x c)
+//                plus the L1 term l1 sign(y - x) collects in LDS for all
+//                channels and is stored (or added) in one coalesced pass.
+//
+// The images interleave their channels, so every global access walks whole
+// row segments of all channels at once: a per-channel stride would fetch
+// and write back each cache line once per channel.
 //
 // Images are packed like every per-pixel buffer of the library: [P, C] with
 // view b's pixels at [pix_base[b], pix_base[b+1]), views of any resolution.
@@ -41,7 +47,7 @@ int ssim_fail(int code, const char* msg);
 
 namespace {
 
-constexpr int kTW = 64, kTH = 16, kHalo = 10, kThreads = 256;
+constexpr int kTW = 64, kTH = 16, kHalo = 10, kThreads = 512;
 constexpr int kLW = kTW + kHalo, kLH = kTH + kHalo;
 constexpr int kMaxTaps = 11, kMaxC = 4;
 
@@ -70,166 +76,293 @@ __device__ __forceinline__ float wave_sum(float x) {
     return x;
 }
 
+// LDS floats of the two kernels for C channels: k_ssim_maps stages x and y
+// (all channels, with the halo) and the five horizontal sums of one channel;
+// k_ssim_grad stages the three maps of one channel with their halo, their
+// horizontal sums, the tile's x and y (all channels) and its gradient.
+template <int C>
+constexpr int maps_lds_floats() { return 2 * C * kLH * kLW + 5 * kLH * kTW; }
+template <int C>
+constexpr int grad_lds_floats() { return 3 * kLH * kLW + 3 * kLH * kTW + 3 * C * kTH * kTW; }
+
+// The separable passes, one per axis and tap count (the window's 11 taps,
+// or 1 on a side shorter than the window), with every tap loop unrolled so
+// that a thread's LDS reads issue back to back.
+
+// Horizontal sums of x, y, x^2, y^2, xy over `rows` staged rows.
+template <int KW>
+__device__ __forceinline__ void moments_h(const float* sx, const float* sy, float* hs, const float (&g)[kMaxTaps],
+                                          int rows, int tid) {
+    for (int i = tid; i < rows * kTW; i += kThreads) {
+        const int r = i / kTW, j = i - r * kTW;
+        float ax = 0.f, ay = 0.f, axx = 0.f, ayy = 0.f, axy = 0.f;
+#pragma unroll
+        for (int k = 0; k < KW; ++k) {
+            const float wk = KW == 1 ? 1.f : g[k];
+            const float xv = sx[r * kLW + j + k], yv = sy[r * kLW + j + k];
+            ax += wk * xv;
+            ay += wk * yv;
+            axx += wk * (xv * xv);
+            ayy += wk * (yv * yv);
+            axy += wk * (xv * yv);
+        }
+        hs[(0 * kLH + r) * kTW + j] = ax;
+        hs[(1 * kLH + r) * kTW + j] = ay;
+        hs[(2 * kLH + r) * kTW + j] = axx;
+        hs[(3 * kLH + r) * kTW + j] = ayy;
+        hs[(4 * kLH + r) * kTW + j] = axy;
+    }
+}
+
+// Vertical sums, S and the three derivative maps of channel c for the
+// tile's windows; returns this thread's sum of S.
+template <int C, int KH>
+__device__ __forceinline__ float ssim_v(const float* hs, const float (&g)[kMaxTaps], int c, float c1, float c2,
+                                        int Hv, int Wv, int y0, int x0, int W, long long base,
+                                        float* __restrict__ maps, long long P, int tid) {
+    float acc = 0.f;
+    for (int i = tid; i < kTH * kTW; i += kThreads) {
+        const int r = i / kTW, j = i - r * kTW;
+        const int oy = y0 + r, ox = x0 + j;
+        if (oy >= Hv || ox >= Wv) continue;
+        float mx = 0.f, my = 0.f, exx = 0.f, eyy = 0.f, exy = 0.f;
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const float wk = KH == 1 ? 1.f : g[k];
+            mx += wk * hs[(0 * kLH + r + k) * kTW + j];
+            my += wk * hs[(1 * kLH + r + k) * kTW + j];
+            exx += wk * hs[(2 * kLH + r + k) * kTW + j];
+            eyy += wk * hs[(3 * kLH + r + k) * kTW + j];
+            exy += wk * hs[(4 * kLH + r + k) * kTW + j];
+        }
+        const float sxx = exx - mx * mx, syy = eyy - my * my, sxy = exy - mx * my;
+        const float a1 = 2.f * mx * my + c1, b1 = mx * mx + my * my + c1;
+        const float a2 = 2.f * sxy + c2, b2 = sxx + syy + c2;
+        // both denominators are >= C1 or C2 > 0: one hardware reciprocal each
+        // (1 ulp) serves every quotient below
+        const float rb1 = __builtin_amdgcn_rcpf(b1), rb2 = __builtin_amdgcn_rcpf(b2);
+        const float lum = a1 * rb1, cs = a2 * rb2;
+        acc += lum * cs;
+        const long long o = base + (long long)oy * W + ox;
+        // dS/dmy = dL/dmy cs + L dcs/dmy; dS/deyy = -L cs / b2; dS/dexy = 2 L / b2
+        maps[(0 * C + c) * P + o] = 2.f * ((mx - lum * my) * rb1 * cs + lum * (my * cs - mx) * rb2);
+        maps[(1 * C + c) * P + o] = -lum * cs * rb2;
+        maps[(2 * C + c) * P + o] = 2.f * lum * rb2;
+    }
+    return acc;
+}
+
+// Horizontal sums of the three maps of one channel.
+template <int KW>
+__device__ __forceinline__ void maps_h(const float* sm, float* hs, const float (&g)[kMaxTaps], int rows, int tid) {
+    for (int i = tid; i < rows * kTW; i += kThreads) {
+        const int r = i / kTW, j = i - r * kTW;
+        float a = 0.f, bb = 0.f, cc = 0.f;
+#pragma unroll
+        for (int k = 0; k < KW; ++k) {
+            const float wk = KW == 1 ? 1.f : g[k];
+            a += wk * sm[(0 * kLH + r) * kLW + j + k];
+            bb += wk * sm[(1 * kLH + r) * kLW + j + k];
+            cc += wk * sm[(2 * kLH + r) * kLW + j + k];
+        }
+        hs[(0 * kLH + r) * kTW + j] = a;
+        hs[(1 * kLH + r) * kTW + j] = bb;
+        hs[(2 * kLH + r) * kTW + j] = cc;
+    }
+}
+
+// Vertical sums and the gradient of channel c for the tile's pixels, into tg.
+template <int C, int KH>
+__device__ __forceinline__ void grad_v(const float* hs, const float (&g)[kMaxTaps], int c, float sc, float l1s,
+                                       const float* tx_, const float* ty_, float* tg, int tid) {
+    constexpr int rowf = kTW * C;
+    for (int i = tid; i < kTH * kTW; i += kThreads) {
+        const int r = i / kTW, j = i - r * kTW;
+        float a = 0.f, bb = 0.f, cc = 0.f;
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const float wk = KH == 1 ? 1.f : g[k];
+            a += wk * hs[(0 * kLH + r + k) * kTW + j];
+            bb += wk * hs[(1 * kLH + r + k) * kTW + j];
+            cc += wk * hs[(2 * kLH + r + k) * kTW + j];
+        }
+        const int e = r * rowf + j * C + c;
+        const float xv = tx_[e], yv = ty_[e];
+        const float d = yv - xv;
+        const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
+        tg[e] = sc * (a + 2.f * yv * bb + xv * cc) + l1s * sg;
+    }
+}
+
+// Maps are channel-planar: map m of channel c at pixel p is maps[(m C + c) P + p].
+template <int C>
 __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float* __restrict__ X,
                                                         const float* __restrict__ Y, float c1, float c2,
-                                                        float* __restrict__ ma, float* __restrict__ mb,
-                                                        float* __restrict__ mc, float* __restrict__ partial) {
-    __shared__ float sx[kLH][kLW], sy[kLH][kLW];
-    __shared__ float hs[5][kLH][kTW];
+                                                        float* __restrict__ maps, long long P,
+                                                        float* __restrict__ partial) {
+    extern __shared__ float lds[];
+    float* xs = lds;                       // [C][kLH][kLW]
+    float* ys = xs + C * kLH * kLW;        // [C][kLH][kLW]
+    float* hs = ys + C * kLH * kLW;        // [5][kLH][kTW]
     __shared__ float red[kThreads / 64];
     const int tile = blockIdx.x;
     const int b = view_of_tile(v, tile);
     const int local = tile - v.tile_base[b];
     const int ty = local / v.tiles_x[b], tx = local - ty * v.tiles_x[b];
-    const int H = v.H[b], W = v.W[b], kh = v.kh[b], kw = v.kw[b], C = v.C;
+    const int H = v.H[b], W = v.W[b], kh = v.kh[b], kw = v.kw[b];
     const int Hv = H - kh + 1, Wv = W - kw + 1;
     const int y0 = ty * kTH, x0 = tx * kTW;
     const long long base = v.pix_base[b];
     const int tid = threadIdx.x;
 
+    // one pass over the tile's rows, all channels: each row segment of
+    // kLW pixels is kLW * C consecutive floats.  The loop is unrolled so a
+    // thread's loads are all in flight before the first LDS store waits.
+    constexpr int rowf = kLW * C;
+    constexpr int n_stage = kLH * rowf, it_stage = (n_stage + kThreads - 1) / kThreads;
+    float xv[it_stage], yv[it_stage];
+#pragma unroll
+    for (int it = 0; it < it_stage; ++it) {
+        const int i = tid + it * kThreads;
+        const int r = i / rowf, e = i - r * rowf;
+        const int gy = y0 + r, gx = x0 + e / C;
+        xv[it] = yv[it] = 0.f;
+        if (i < n_stage && gy < H && gx < W) {
+            const long long o = (base + (long long)gy * W + x0) * C + e;
+            xv[it] = X[o];
+            yv[it] = Y[o];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < it_stage; ++it) {
+        const int i = tid + it * kThreads;
+        if (i >= n_stage) break;
+        const int r = i / rowf, e = i - r * rowf;
+        const int q = e / C, c = e - q * C;
+        xs[(c * kLH + r) * kLW + q] = xv[it];
+        ys[(c * kLH + r) * kLW + q] = yv[it];
+    }
+    __syncthreads();
+    float g[kMaxTaps];   // the window in registers (every tap loop below is unrolled)
+    for (int k = 0; k < kMaxTaps; ++k) g[k] = v.win[k];
     for (int c = 0; c < C; ++c) {
-        for (int i = tid; i < kLH * kLW; i += kThreads) {
-            const int r = i / kLW, q = i - r * kLW;
-            const int gy = y0 + r, gx = x0 + q;
-            float xv = 0.f, yv = 0.f;
-            if (gy < H && gx < W) {
-                const long long o = (base + (long long)gy * W + gx) * C + c;
-                xv = X[o];
-                yv = Y[o];
-            }
-            sx[r][q] = xv;
-            sy[r][q] = yv;
-        }
+        const float* sx = xs + c * kLH * kLW;
+        const float* sy = ys + c * kLH * kLW;
+        if (kw == 1) moments_h<1>(sx, sy, hs, g, kTH + kh - 1, tid);
+        else moments_h<kMaxTaps>(sx, sy, hs, g, kTH + kh - 1, tid);
         __syncthreads();
-        for (int i = tid; i < (kTH + kh - 1) * kTW; i += kThreads) {
-            const int r = i / kTW, j = i - r * kTW;
-            float ax = 0.f, ay = 0.f, axx = 0.f, ayy = 0.f, axy = 0.f;
-            for (int k = 0; k < kw; ++k) {
-                const float wk = kw == 1 ? 1.f : v.win[k];
-                const float xv = sx[r][j + k], yv = sy[r][j + k];
-                ax += wk * xv;
-                ay += wk * yv;
-                axx += wk * (xv * xv);
-                ayy += wk * (yv * yv);
-                axy += wk * (xv * yv);
-            }
-            hs[0][r][j] = ax;
-            hs[1][r][j] = ay;
-            hs[2][r][j] = axx;
-            hs[3][r][j] = ayy;
-            hs[4][r][j] = axy;
-        }
-        __syncthreads();
-        float acc = 0.f;
-        for (int i = tid; i < kTH * kTW; i += kThreads) {
-            const int r = i / kTW, j = i - r * kTW;
-            const int oy = y0 + r, ox = x0 + j;
-            if (oy >= Hv || ox >= Wv) continue;
-            float mx = 0.f, my = 0.f, exx = 0.f, eyy = 0.f, exy = 0.f;
-            for (int k = 0; k < kh; ++k) {
-                const float wk = kh == 1 ? 1.f : v.win[k];
-                mx += wk * hs[0][r + k][j];
-                my += wk * hs[1][r + k][j];
-                exx += wk * hs[2][r + k][j];
-                eyy += wk * hs[3][r + k][j];
-                exy += wk * hs[4][r + k][j];
-            }
-            const float sxx = exx - mx * mx, syy = eyy - my * my, sxy = exy - mx * my;
-            const float a1 = 2.f * mx * my + c1, b1 = mx * mx + my * my + c1;
-            const float a2 = 2.f * sxy + c2, b2 = sxx + syy + c2;
-            const float lum = a1 / b1, cs = a2 / b2;
-            acc += lum * cs;
-            const long long o = (base + (long long)oy * W + ox) * C + c;
-            // dS/dmy = dL/dmy cs + L dcs/dmy; dS/deyy = -L cs / b2; dS/dexy = 2 L / b2
-            ma[o] = 2.f * (mx - lum * my) / b1 * cs + lum * 2.f * (my * cs - mx) / b2;
-            mb[o] = -lum * cs / b2;
-            mc[o] = 2.f * lum / b2;
-        }
+        float acc = kh == 1 ? ssim_v<C, 1>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid)
+                            : ssim_v<C, kMaxTaps>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid);
         acc = wave_sum(acc);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
-        __syncthreads();
+        __syncthreads();   // also orders this channel's reads of hs before the next one's writes
         if (tid == 0) {
-            float s = 0.f;
-            for (int w = 0; w < kThreads / 64; ++w) s += red[w];
-            partial[(long long)tile * C + c] = s;
+            float t = 0.f;
+            for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+            partial[(long long)tile * C + c] = t;
         }
     }
 }
 
+template <int C>
 __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float* __restrict__ X,
-                                                        const float* __restrict__ Y, const float* __restrict__ ma,
-                                                        const float* __restrict__ mb, const float* __restrict__ mc,
+                                                        const float* __restrict__ Y,
+                                                        const float* __restrict__ maps, long long P,
                                                         const float* __restrict__ dscale, float l1,
                                                         const float* __restrict__ dl1, float* __restrict__ grad,
                                                         int accumulate, float* __restrict__ l1_partial) {
-    __shared__ float sm[3][kLH][kLW];
-    __shared__ float hs[3][kLH][kTW];
+    extern __shared__ float lds[];
+    float* sm = lds;                       // [3][kLH][kLW]   one channel's maps with the halo
+    float* hs = sm + 3 * kLH * kLW;        // [3][kLH][kTW]   their horizontal sums
+    float* tx_ = hs + 3 * kLH * kTW;       // [kTH][kTW * C]  the tile's x, pixel-major
+    float* ty_ = tx_ + C * kTH * kTW;      // [kTH][kTW * C]  its y
+    float* tg = ty_ + C * kTH * kTW;       // [kTH][kTW * C]  its gradient
     __shared__ float red[kThreads / 64];
-    float l1_acc = 0.f;   // sum of |y - x| over this tile's pixels and channels
     const int tile = blockIdx.x;
     const int b = view_of_tile(v, tile);
     const int local = tile - v.tile_base[b];
-    const int ty = local / v.tiles_x[b], tx = local - ty * v.tiles_x[b];
-    const int H = v.H[b], W = v.W[b], kh = v.kh[b], kw = v.kw[b], C = v.C;
+    const int tyi = local / v.tiles_x[b], txi = local - tyi * v.tiles_x[b];
+    const int H = v.H[b], W = v.W[b], kh = v.kh[b], kw = v.kw[b];
     const int Hv = H - kh + 1, Wv = W - kw + 1;
     const int hy = kh - 1, hx = kw - 1;
-    const int y0 = ty * kTH, x0 = tx * kTW;
+    const int y0 = tyi * kTH, x0 = txi * kTW;
     const long long base = v.pix_base[b];
     const int tid = threadIdx.x;
     const float l1s = dl1 ? l1 * dl1[0] : l1;
+    constexpr int rowf = kTW * C;
+    float g[kMaxTaps];
+    for (int k = 0; k < kMaxTaps; ++k) g[k] = v.win[k];
 
+    // the tile's x and y, all channels, row segments of kTW * C floats (the
+    // staging loops are unrolled so a thread's loads are in flight together)
+    constexpr int n_tile = kTH * rowf, it_tile = (n_tile + kThreads - 1) / kThreads;
+    {
+        float xv[it_tile], yv[it_tile];
+#pragma unroll
+        for (int it = 0; it < it_tile; ++it) {
+            const int i = tid + it * kThreads;
+            const int r = i / rowf, e = i - r * rowf;
+            const int gy = y0 + r, gx = x0 + e / C;
+            xv[it] = yv[it] = 0.f;
+            if (i < n_tile && gy < H && gx < W) {
+                const long long o = (base + (long long)gy * W + x0) * C + e;
+                xv[it] = X[o];
+                yv[it] = Y[o];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < it_tile; ++it) {
+            const int i = tid + it * kThreads;
+            if (i < n_tile) {
+                tx_[i] = xv[it];
+                ty_[i] = yv[it];
+            }
+        }
+    }
+    constexpr int n_halo = kLH * kLW, it_halo = (n_halo + kThreads - 1) / kThreads;
     for (int c = 0; c < C; ++c) {
         // windows q with q <= p < q + taps: rows y0 - hy .. y0 + kTH - 1
-        for (int i = tid; i < kLH * kLW; i += kThreads) {
+        float a[it_halo], bb[it_halo], cc[it_halo];
+#pragma unroll
+        for (int it = 0; it < it_halo; ++it) {
+            const int i = tid + it * kThreads;
             const int r = i / kLW, q = i - r * kLW;
             const int wy = y0 - hy + r, wx = x0 - hx + q;
-            float a = 0.f, bb = 0.f, cc = 0.f;
-            if (r < kTH + hy && q < kTW + hx && wy >= 0 && wy < Hv && wx >= 0 && wx < Wv) {
-                const long long o = (base + (long long)wy * W + wx) * C + c;
-                a = ma[o];
-                bb = mb[o];
-                cc = mc[o];
+            a[it] = bb[it] = cc[it] = 0.f;
+            if (i < n_halo && r < kTH + hy && q < kTW + hx && wy >= 0 && wy < Hv && wx >= 0 && wx < Wv) {
+                const long long o = base + (long long)wy * W + wx;
+                a[it] = maps[(0 * C + c) * P + o];
+                bb[it] = maps[(1 * C + c) * P + o];
+                cc[it] = maps[(2 * C + c) * P + o];
             }
-            sm[0][r][q] = a;
-            sm[1][r][q] = bb;
-            sm[2][r][q] = cc;
+        }
+#pragma unroll
+        for (int it = 0; it < it_halo; ++it) {
+            const int i = tid + it * kThreads;
+            if (i < n_halo) {
+                sm[0 * n_halo + i] = a[it];
+                sm[1 * n_halo + i] = bb[it];
+                sm[2 * n_halo + i] = cc[it];
+            }
         }
         __syncthreads();
-        for (int i = tid; i < (kTH + hy) * kTW; i += kThreads) {
-            const int r = i / kTW, j = i - r * kTW;
-            float a = 0.f, bb = 0.f, cc = 0.f;
-            for (int k = 0; k < kw; ++k) {
-                const float wk = kw == 1 ? 1.f : v.win[k];
-                a += wk * sm[0][r][j + k];
-                bb += wk * sm[1][r][j + k];
-                cc += wk * sm[2][r][j + k];
-            }
-            hs[0][r][j] = a;
-            hs[1][r][j] = bb;
-            hs[2][r][j] = cc;
-        }
+        if (kw == 1) maps_h<1>(sm, hs, g, kTH + hy, tid);
+        else maps_h<kMaxTaps>(sm, hs, g, kTH + hy, tid);
         __syncthreads();
         const float sc = dscale ? v.scale[b * C + c] * dscale[b * C + c] : v.scale[b * C + c];
-        for (int i = tid; i < kTH * kTW; i += kThreads) {
-            const int r = i / kTW, j = i - r * kTW;
-            const int py = y0 + r, px = x0 + j;
-            if (py >= H || px >= W) continue;
-            float a = 0.f, bb = 0.f, cc = 0.f;
-            for (int k = 0; k < kh; ++k) {
-                const float wk = kh == 1 ? 1.f : v.win[k];
-                a += wk * hs[0][r + k][j];
-                bb += wk * hs[1][r + k][j];
-                cc += wk * hs[2][r + k][j];
-            }
-            const long long o = (base + (long long)py * W + px) * C + c;
-            const float xv = X[o], yv = Y[o];
-            const float d = yv - xv;
-            const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-            const float gval = sc * (a + 2.f * yv * bb + xv * cc) + l1s * sg;
-            grad[o] = accumulate ? grad[o] + gval : gval;
-            l1_acc += fabsf(d);
-        }
-        __syncthreads();
+        if (kh == 1) grad_v<C, 1>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
+        else grad_v<C, kMaxTaps>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
+        __syncthreads();   // the next channel overwrites sm and hs
+    }
+    // the gradient of all channels, stored in row segments of kTW * C floats
+    float l1_acc = 0.f;   // sum of |y - x| over the tile's pixels and channels
+    for (int i = tid; i < kTH * rowf; i += kThreads) {
+        const int r = i / rowf, e = i - r * rowf;
+        const int gy = y0 + r, gx = x0 + e / C;
+        if (gy >= H || gx >= W) continue;
+        const long long o = (base + (long long)gy * W + x0) * C + e;
+        grad[o] = accumulate ? grad[o] + tg[i] : tg[i];
+        l1_acc += fabsf(ty_[i] - tx_[i]);
     }
     if (l1_partial) {
         l1_acc = wave_sum(l1_acc);
@@ -243,6 +376,63 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
     }
 }
 
+// Launch either kernel for the runtime channel count (1 to 4).
+template <int C>
+hipError_t run_maps(const SsimTiles& t, long long tiles, const float* X, const float* Y, float c1, float c2,
+                    float* maps, float* partial, hipStream_t s) {
+    const size_t bytes = sizeof(float) * maps_lds_floats<C>();
+    static bool opted_in = false;   // more than 64 KiB of LDS needs the kernel's opt-in, once
+    if (!opted_in && bytes > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ssim_maps<C>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+        if (e != hipSuccess) return e;
+        opted_in = true;
+    }
+    k_ssim_maps<C><<<static_cast<unsigned>(tiles), kThreads, bytes, s>>>(t, X, Y, c1, c2, maps, t.pix_base[t.n],
+                                                                          partial);
+    return hipGetLastError();
+}
+
+template <int C>
+hipError_t run_grad(const SsimTiles& t, long long tiles, const float* X, const float* Y, const float* maps,
+                    const float* dscale, float l1, const float* dl1, float* grad, int accumulate, float* l1_partial,
+                    hipStream_t s) {
+    const size_t bytes = sizeof(float) * grad_lds_floats<C>();
+    static bool opted_in = false;
+    if (!opted_in && bytes > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ssim_grad<C>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+        if (e != hipSuccess) return e;
+        opted_in = true;
+    }
+    k_ssim_grad<C><<<static_cast<unsigned>(tiles), kThreads, bytes, s>>>(t, X, Y, maps, t.pix_base[t.n], dscale,
+                                                                          l1, dl1, grad, accumulate, l1_partial);
+    return hipGetLastError();
+}
+
+hipError_t dispatch_maps(const SsimTiles& t, long long tiles, const float* X, const float* Y, float c1, float c2,
+                         float* maps, float* partial, hipStream_t s) {
+    switch (t.C) {
+        case 1: return run_maps<1>(t, tiles, X, Y, c1, c2, maps, partial, s);
+        case 2: return run_maps<2>(t, tiles, X, Y, c1, c2, maps, partial, s);
+        case 3: return run_maps<3>(t, tiles, X, Y, c1, c2, maps, partial, s);
+        case 4: return run_maps<4>(t, tiles, X, Y, c1, c2, maps, partial, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t dispatch_grad(const SsimTiles& t, long long tiles, const float* X, const float* Y, const float* maps,
+                         const float* dscale, float l1, const float* dl1, float* grad, int accumulate,
+                         float* l1_partial, hipStream_t s) {
+    switch (t.C) {
+        case 1: return run_grad<1>(t, tiles, X, Y, maps, dscale, l1, dl1, grad, accumulate, l1_partial, s);
+        case 2: return run_grad<2>(t, tiles, X, Y, maps, dscale, l1, dl1, grad, accumulate, l1_partial, s);
+        case 3: return run_grad<3>(t, tiles, X, Y, maps, dscale, l1, dl1, grad, accumulate, l1_partial, s);
+        case 4: return run_grad<4>(t, tiles, X, Y, maps, dscale, l1, dl1, grad, accumulate, l1_partial, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 // The training step's photometric loss from the partial sums, one workgroup:
 // w_l1 * sum|y - x| / (C P) + w_ssim * (1 - SSIM) / 2, SSIM being the mean
 // over views of each view's mean over channels and windows.
@@ -252,11 +442,12 @@ __global__ __launch_bounds__(1024) void k_photo_loss(SsimTiles v, const float* _
     __shared__ double rs[1024 / 64], rl[1024 / 64];
     const int C = v.C;
     double s = 0.0, l = 0.0;
-    for (int i = threadIdx.x; i < v.tile_base[v.n] * C; i += blockDim.x) {
-        const int b = view_of_tile(v, i / C);
-        const int kh = v.kh[b], kw = v.kw[b];
-        const double windows = (double)(v.H[b] - kh + 1) * (v.W[b] - kw + 1);
-        s += ssim_partial[i] / windows;
+    for (int b = 0; b < v.n; ++b) {   // view by view: uniform weights, no per-lane lookups
+        const double inv = 1.0 / ((double)(v.H[b] - v.kh[b] + 1) * (v.W[b] - v.kw[b] + 1));
+        double sb = 0.0;
+        for (int i = v.tile_base[b] * C + threadIdx.x; i < v.tile_base[b + 1] * C; i += blockDim.x)
+            sb += ssim_partial[i];
+        s += sb * inv;
     }
     for (int i = threadIdx.x; i < n_pix_tiles; i += blockDim.x) l += l1_partial[i];
     for (int o = 32; o > 0; o >>= 1) {
@@ -282,8 +473,8 @@ __global__ __launch_bounds__(1024) void k_photo_loss(SsimTiles v, const float* _
 // Fill the tile table for windows (valid = true) or pixels; returns tiles or -1.
 long long make_tiles(int B, const int* H, const int* W, int C, int taps, const float* win, bool valid,
                      SsimTiles& t) {
-    if (B < 1 || B > kMaxViews || !H || !W || C < 1 || C > kMaxC || taps < 1 || taps > kMaxTaps || !(taps & 1))
-        return -1;
+    // the kernels unroll the window's 11 taps (pytorch_msssim's default size)
+    if (B < 1 || B > kMaxViews || !H || !W || C < 1 || C > kMaxC || taps != kMaxTaps) return -1;
     std::fill(reinterpret_cast<char*>(&t), reinterpret_cast<char*>(&t) + sizeof(t), 0);
     t.n = B;
     t.C = C;
@@ -317,10 +508,7 @@ hipError_t launch_ssim_maps(int B, const int* H, const int* W, int C, const floa
     const long long tiles = make_tiles(B, H, W, C, taps, win, true, t);
     if (tiles < 0) return hipErrorInvalidValue;
     if (tiles == 0) return hipSuccess;
-    const long long n = t.pix_base[B] * C;
-    k_ssim_maps<<<static_cast<unsigned>(tiles), kThreads, 0, s>>>(t, X, Y, c1, c2, maps, maps + n, maps + 2 * n,
-                                                                    partial);
-    return hipGetLastError();
+    return dispatch_maps(t, tiles, X, Y, c1, c2, maps, partial, s);
 }
 
 hipError_t launch_ssim_grad(int B, const int* H, const int* W, int C, const float* X, const float* Y, int taps,
@@ -331,10 +519,7 @@ hipError_t launch_ssim_grad(int B, const int* H, const int* W, int C, const floa
     if (tiles < 0) return hipErrorInvalidValue;
     for (int i = 0; i < B * C; ++i) t.scale[i] = scale[i];
     if (tiles == 0) return hipSuccess;
-    const long long n = t.pix_base[B] * C;
-    k_ssim_grad<<<static_cast<unsigned>(tiles), kThreads, 0, s>>>(t, X, Y, maps, maps + n, maps + 2 * n, dscale, l1,
-                                                                    dl1, grad, accumulate, nullptr);
-    return hipGetLastError();
+    return dispatch_grad(t, tiles, X, Y, maps, dscale, l1, dl1, grad, accumulate, nullptr, s);
 }
 
 // ---- the training step's L1 + SSIM loss (views of any resolution)
@@ -369,9 +554,7 @@ hipError_t launch_photo_loss(const Views& views, int C, const float* target, con
     float* spart = maps + 3 * n;
     float* lpart = spart + tw * C;
     const float c1 = 0.01f * 0.01f, c2 = 0.03f * 0.03f;
-    k_ssim_maps<<<static_cast<unsigned>(tw), kThreads, 0, s>>>(w, target, color, c1, c2, maps, maps + n,
-                                                                 maps + 2 * n, spart);
-    hipError_t e = hipGetLastError();
+    hipError_t e = dispatch_maps(w, tw, target, color, c1, c2, maps, spart, s);
     if (e != hipSuccess) return e;
     for (int b = 0; b < views.n; ++b) {
         const double windows = (double)(views.H[b] - p.kh[b] + 1) * (views.W[b] - p.kw[b] + 1);
@@ -379,9 +562,8 @@ hipError_t launch_photo_loss(const Views& views, int C, const float* target, con
             p.scale[b * C + c] = static_cast<float>(-0.5 * w_ssim / ((double)views.n * C * windows));
     }
     const float l1s = static_cast<float>(w_l1 / ((double)C * w.pix_base[views.n]));
-    k_ssim_grad<<<static_cast<unsigned>(tp), kThreads, 0, s>>>(p, target, color, maps, maps + n, maps + 2 * n,
-                                                                 nullptr, l1s, nullptr, grad, 0, lpart);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = dispatch_grad(p, tp, target, color, maps, nullptr, l1s, nullptr, grad, 0, lpart, s)) != hipSuccess)
+        return e;
     k_photo_loss<<<1, 1024, 0, s>>>(w, spart, lpart, static_cast<int>(tp), w_l1, w_ssim, loss);
     return hipGetLastError();
 }
@@ -411,7 +593,7 @@ int ds_ssim_fwd(int B, const int* H, const int* W, int C, const float* X, const 
     const hipError_t e = ds::launch_ssim_maps(B, H, W, C, X, Y, win_size, win, c1, c2, maps, partial,
                                               static_cast<hipStream_t>(stream));
     if (e == hipErrorInvalidValue)
-        return ds::ssim_fail(DS_ERR_INVALID, "ssim_fwd: bad views, channel count (1-4) or window size (odd, <= 11)");
+        return ds::ssim_fail(DS_ERR_INVALID, "ssim_fwd: bad views, channel count (1-4) or window size (11)");
     if (e != hipSuccess) return ds::ssim_fail(DS_ERR_HIP, hipGetErrorString(e));
     return DS_OK;
 }
@@ -424,7 +606,7 @@ int ds_ssim_bwd(int B, const int* H, const int* W, int C, const float* X, const 
     const hipError_t e = ds::launch_ssim_grad(B, H, W, C, X, Y, win_size, win, maps, scale, dscale, l1_scale, dl1,
                                               grad, accumulate, static_cast<hipStream_t>(stream));
     if (e == hipErrorInvalidValue)
-        return ds::ssim_fail(DS_ERR_INVALID, "ssim_bwd: bad views, channel count (1-4) or window size (odd, <= 11)");
+        return ds::ssim_fail(DS_ERR_INVALID, "ssim_bwd: bad views, channel count (1-4) or window size (11)");
     if (e != hipSuccess) return ds::ssim_fail(DS_ERR_HIP, hipGetErrorString(e));
     return DS_OK;
 }

@@ -63,7 +63,7 @@ class _Views:
 def _fwd(views, C, X, Y, win, c1, c2):
     """Per-(view, channel) mean SSIM [B, C] (float64) and the derivative maps."""
     P = views.B * views.H * views.W
-    maps = torch.empty((3, P, C), dtype=torch.float32, device=X.device)
+    maps = torch.empty((3, C, P), dtype=torch.float32, device=X.device)
     partial = torch.empty((max(views.tiles, 1), C), dtype=torch.float32, device=X.device)
     wa = (ctypes.c_float * len(win))(*win.tolist())
     _lib.check(_lib.lib().ds_ssim_fwd(views.B, views.Ha, views.Wa, C, _lib.ptr(X), _lib.ptr(Y), len(win), wa,
@@ -138,6 +138,8 @@ def ssim(X: torch.Tensor, Y: torch.Tensor, data_range: float = 255, size_average
         wnp = gaussian_window(win_size, win_sigma)
     if not np.array_equal(wnp, wnp[::-1]):
         raise ValueError("the window must be symmetric")
+    if wnp.size != 11:
+        raise ValueError("the SSIM kernels take an 11-tap window (pytorch_msssim's default win_size)")
     c1 = (K[0] * data_range) ** 2
     c2 = (K[1] * data_range) ** 2
     per_channel = _SsimPerChannel.apply(_channels_last(X, "X"), _channels_last(Y, "Y"), wnp, c1, c2)

@@ -348,9 +348,10 @@ int ds_count_triangle_ids(int64_t P, const float* rast, int T, int64_t* counts, 
 int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 
 /* ------------------------------------------------------------------------
- * Structural similarity of pytorch_msssim.ssim (Gaussian window of win_size
- * odd taps <= 11 given in win, "valid" filtering, a side shorter than the
- * window left unfiltered, c1 = (K1 R)^2, c2 = (K2 R)^2), the term of the
+ * Structural similarity of pytorch_msssim.ssim (a symmetric window of
+ * win_size = 11 taps given in win, as pytorch_msssim's default Gaussian,
+ * "valid" filtering, a side shorter than the window left unfiltered,
+ * c1 = (K1 R)^2, c2 = (K2 R)^2), the term of the
  * reference's loss 0.8 L1 + 0.2 (1 - SSIM) / 2 (examples/02_synthetic.py:
  * 376-380).  X (reference image) and Y are [P, C] packed over B views like
  * rast, 1 <= C <= 4.
@@ -359,8 +360,8 @@ int ds_fill_f32(float* p, int64_t n, float value, void* stream);
  * array, times C); tile_base (host, B + 1 entries, may be NULL) receives
  * where each view's tiles start.
  * ds_ssim_fwd: per-tile, per-channel sums of the SSIM map into partial
- * [tiles, C], and the three derivative maps into maps [3, P, C] (entries
- * outside the windows are not written).
+ * [tiles, C], and the three derivative maps into maps [3, C, P]
+ * (channel-planar; entries outside the windows are not written).
  * ds_ssim_bwd: grad [P, C] = scale[b, c] * dscale[b, c] * dSSIM_sum/dY +
  * l1_scale * dl1 * sign(Y - X), stored or added (accumulate).  scale is a
  * host array [B, C]; dscale (device [B, C]) and dl1 (device scalar) may be

@@ -53,20 +53,23 @@ def test_bad_arguments_fail_without_a_gpu(L):
     assert rc == -1 and b"level range" in L.ds_last_error()
     Hbig = _lib.int_array([70000])
     assert L.ds_raster_tile_count(1, Hbig, W, 16) == -1
+    from diffsoup_amd import losses  # noqa: F401  (declares the SSIM signatures)
+    assert L.ds_ssim_tiles(1, H, W, 11, None) == 4          # 54 x 54 windows in 64 x 16 tiles
+    assert L.ds_ssim_tiles(1, H, W, 7, None) == -1 and b"ssim_tiles" in L.ds_last_error()
 
 
 def test_core_dropin_surface_and_type_checks(L):
-    """The drop-in _core module offers every compute function the reference's
-    Python layer binds (src/main.cpp:30-304, remeshing aside) and, like
-    nanobind, raises TypeError on a wrong dtype, rank or layout before any
-    GPU work."""
+    """The drop-in _core module offers every function the reference's Python
+    layer binds (src/main.cpp:30-385) and, like nanobind, raises TypeError on
+    a wrong dtype, rank or layout before any GPU work."""
     import torch
 
     from diffsoup_amd import _core
     ref_names = ["compute_triangle_rects", "compute_fragments", "depth_test", "filter_valid_fragments",
                  "backward_edge_grad", "backward_opacity_aux_loss", "encode_view_dir_sh2",
                  "multires_triangle_alpha", "backward_multires_triangle_alpha", "multires_triangle_color",
-                 "backward_multires_triangle_color", "accumulate_to_level_forward", "accumulate_to_level_backward"]
+                 "backward_multires_triangle_color", "accumulate_to_level_forward", "accumulate_to_level_backward",
+                 "split_triangle_soup", "split_triangle_soup_clip"]
     assert all(callable(getattr(_core, n)) for n in ref_names)
     assert _core.__version__ == "0.1.0"
     pos = torch.zeros(1, 3, 4, dtype=torch.float64)

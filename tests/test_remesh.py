@@ -231,3 +231,19 @@ def test_expand_and_remove_unreferenced():
     np.testing.assert_array_equal(v2.numpy(), v.numpy()[[1, 4, 5]])
     np.testing.assert_array_equal(f2.numpy(), [[1, 0, 2], [2, 0, 1]])
     assert f2.dtype == torch.int32
+
+
+def test_core_dropin_splitters_match_reference_vectors(golden):
+    """The drop-in _core entry points, NumPy in and out, as the reference's
+    remesh.py calls them (python/diffsoup/remesh.py:37-39, 123-131)."""
+    from diffsoup_amd import _core
+    g = golden
+    ns, tau = g["w_cap_args"].tolist()
+    got = _core.split_triangle_soup(g["w_cap_in_verts"], g["w_cap_in_faces"], int(ns), float(tau))
+    _same(got, [g[f"w_cap_{k}"] for k in ("verts", "faces", "face_map", "flags")])
+    ns, tau_ratio, aspect = g["c_wide_args"].tolist()
+    got = _core.split_triangle_soup_clip(g["c_wide_mvp"], g["c_wide_in_verts"], g["c_wide_in_faces"],
+                                         g["c_wide_valid"], int(ns), float(tau_ratio), float(aspect))
+    _same(got, [g[f"c_wide_{k}"] for k in ("verts", "faces", "face_map", "flags")])
+    with pytest.raises(TypeError):
+        _core.split_triangle_soup(g["w_cap_in_verts"].astype(np.float64), g["w_cap_in_faces"], 1, 0.0)
