@@ -15,13 +15,13 @@
 // per-window maps: dSSIM/dy(p) = sum_q G(q, p) [a(q) + 2 y(p) b(q) + x(p) c(q)]
 // with a = dS/dmy, b = dS/deyy, c = dS/dexy (each holding the others fixed).
 //
-//   k_ssim_maps  one workgroup per 64x16 tile of windows: x and y of all
+//   k_ssim_maps  one workgroup per 60x16 tile of windows: x and y of all
 //                channels with a 10-pixel halo staged in LDS in one pass over
 //                contiguous row segments, then per channel the horizontal pass
 //                of the five sums into LDS, the vertical pass in registers, S
 //                (a per-tile partial sum) and the maps a, b, c written
 //                channel-planar at the window's top-left pixel.
-//   k_ssim_grad  one workgroup per 64x16 tile of pixels: per channel the
+//   k_ssim_grad  one workgroup per 60x16 tile of pixels: per channel the
 //                three maps with their halo in LDS (zero outside the windows)
 //                and the same two passes (the window is symmetric, so the
 //                transposed filter is the filter); scale (a + 2 y b + x c)
@@ -30,7 +30,12 @@
 //
 // The images interleave their channels, so every global access walks whole
 // row segments of all channels at once: a per-channel stride would fetch
-// and write back each cache line once per channel.
+// and write back each cache line once per channel.  In LDS the filter passes
+// are register-blocked: a horizontal pass forms four adjacent outputs from
+// fourteen staged values read as four float4s, a vertical pass two rows from
+// twelve, so each staged value is read about a third as often as with one
+// output per thread.  The 60-pixel tile width keeps two 8-wave workgroups per
+// CU within the LDS once the staged rows are padded to a 16-byte stride.
 //
 // Images are packed like every per-pixel buffer of the library: [P, C] with
 // view b's pixels at [pix_base[b], pix_base[b+1]), views of any resolution.
@@ -47,8 +52,12 @@ int ssim_fail(int code, const char* msg);
 
 namespace {
 
-constexpr int kTW = 64, kTH = 16, kHalo = 10, kThreads = 512;
+constexpr int kTW = 60, kTH = 16, kHalo = 10, kThreads = 512;
 constexpr int kLW = kTW + kHalo, kLH = kTH + kHalo;
+constexpr int kLWp = (kLW + 3) & ~3;   // staged row stride: rows start 16-byte aligned
+constexpr int kQ = 4;                  // horizontal outputs per thread (one float4 of results)
+constexpr int kR = 2;                  // vertical outputs per thread
+static_assert(kTW % kQ == 0 && kTH % kR == 0, "tile edges must be multiples of the blocking");
 constexpr int kMaxTaps = 11, kMaxC = 4;
 
 struct SsimTiles {
@@ -81,116 +90,162 @@ __device__ __forceinline__ float wave_sum(float x) {
 // k_ssim_grad stages the three maps of one channel with their halo, their
 // horizontal sums, the tile's x and y (all channels) and its gradient.
 template <int C>
-constexpr int maps_lds_floats() { return 2 * C * kLH * kLW + 5 * kLH * kTW; }
+constexpr int maps_lds_floats() { return 2 * C * kLH * kLWp + 5 * kLH * kTW; }
 template <int C>
-constexpr int grad_lds_floats() { return 3 * kLH * kLW + 3 * kLH * kTW + 3 * C * kTH * kTW; }
+constexpr int grad_lds_floats() { return 3 * kLH * kLWp + 3 * kLH * kTW + 3 * C * kTH * kTW; }
 
 // The separable passes, one per axis and tap count (the window's 11 taps,
 // or 1 on a side shorter than the window), with every tap loop unrolled so
 // that a thread's LDS reads issue back to back.
 
-// Horizontal sums of x, y, x^2, y^2, xy over `rows` staged rows.
+// Horizontal sums of x, y, x^2, y^2, xy over `rows` staged rows; a thread
+// forms kQ adjacent outputs from kQ + KW - 1 staged values read as float4s.
 template <int KW>
 __device__ __forceinline__ void moments_h(const float* sx, const float* sy, float* hs, const float (&g)[kMaxTaps],
                                           int rows, int tid) {
-    for (int i = tid; i < rows * kTW; i += kThreads) {
-        const int r = i / kTW, j = i - r * kTW;
-        float ax = 0.f, ay = 0.f, axx = 0.f, ayy = 0.f, axy = 0.f;
+    constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
+    for (int i = tid; i < rows * (kTW / kQ); i += kThreads) {
+        const int r = i / (kTW / kQ), j0 = (i - r * (kTW / kQ)) * kQ;
+        float xv[4 * N4], yv[4 * N4];
 #pragma unroll
-        for (int k = 0; k < KW; ++k) {
-            const float wk = KW == 1 ? 1.f : g[k];
-            const float xv = sx[r * kLW + j + k], yv = sy[r * kLW + j + k];
-            ax += wk * xv;
-            ay += wk * yv;
-            axx += wk * (xv * xv);
-            ayy += wk * (yv * yv);
-            axy += wk * (xv * yv);
+        for (int q = 0; q < N4; ++q) {
+            const float4 a = *reinterpret_cast<const float4*>(sx + r * kLWp + j0 + 4 * q);
+            const float4 b = *reinterpret_cast<const float4*>(sy + r * kLWp + j0 + 4 * q);
+            xv[4 * q] = a.x; xv[4 * q + 1] = a.y; xv[4 * q + 2] = a.z; xv[4 * q + 3] = a.w;
+            yv[4 * q] = b.x; yv[4 * q + 1] = b.y; yv[4 * q + 2] = b.z; yv[4 * q + 3] = b.w;
         }
-        hs[(0 * kLH + r) * kTW + j] = ax;
-        hs[(1 * kLH + r) * kTW + j] = ay;
-        hs[(2 * kLH + r) * kTW + j] = axx;
-        hs[(3 * kLH + r) * kTW + j] = ayy;
-        hs[(4 * kLH + r) * kTW + j] = axy;
+        // the products once per staged value, shared by the kQ outputs
+        float xx[NV], yy[NV], xy[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            xx[k] = xv[k] * xv[k];
+            yy[k] = yv[k] * yv[k];
+            xy[k] = xv[k] * yv[k];
+        }
+        float ax[kQ], ay[kQ], axx[kQ], ayy[kQ], axy[kQ];
+#pragma unroll
+        for (int o = 0; o < kQ; ++o) {
+            ax[o] = ay[o] = axx[o] = ayy[o] = axy[o] = 0.f;
+#pragma unroll
+            for (int k = 0; k < KW; ++k) {
+                const float wk = KW == 1 ? 1.f : g[k];
+                ax[o] += wk * xv[o + k];
+                ay[o] += wk * yv[o + k];
+                axx[o] += wk * xx[o + k];
+                ayy[o] += wk * yy[o + k];
+                axy[o] += wk * xy[o + k];
+            }
+        }
+        *reinterpret_cast<float4*>(hs + (0 * kLH + r) * kTW + j0) = make_float4(ax[0], ax[1], ax[2], ax[3]);
+        *reinterpret_cast<float4*>(hs + (1 * kLH + r) * kTW + j0) = make_float4(ay[0], ay[1], ay[2], ay[3]);
+        *reinterpret_cast<float4*>(hs + (2 * kLH + r) * kTW + j0) = make_float4(axx[0], axx[1], axx[2], axx[3]);
+        *reinterpret_cast<float4*>(hs + (3 * kLH + r) * kTW + j0) = make_float4(ayy[0], ayy[1], ayy[2], ayy[3]);
+        *reinterpret_cast<float4*>(hs + (4 * kLH + r) * kTW + j0) = make_float4(axy[0], axy[1], axy[2], axy[3]);
     }
 }
 
 // Vertical sums, S and the three derivative maps of channel c for the
-// tile's windows; returns this thread's sum of S.
+// tile's windows, kR rows per thread; returns this thread's sum of S.
 template <int C, int KH>
 __device__ __forceinline__ float ssim_v(const float* hs, const float (&g)[kMaxTaps], int c, float c1, float c2,
                                         int Hv, int Wv, int y0, int x0, int W, long long base,
                                         float* __restrict__ maps, long long P, int tid) {
+    constexpr int NV = kR + KH - 1;
     float acc = 0.f;
-    for (int i = tid; i < kTH * kTW; i += kThreads) {
-        const int r = i / kTW, j = i - r * kTW;
-        const int oy = y0 + r, ox = x0 + j;
-        if (oy >= Hv || ox >= Wv) continue;
-        float mx = 0.f, my = 0.f, exx = 0.f, eyy = 0.f, exy = 0.f;
+    for (int i = tid; i < (kTH / kR) * kTW; i += kThreads) {
+        const int rg = i / kTW, j = i - rg * kTW;
+        const int r0 = rg * kR;
+        if (y0 + r0 >= Hv || x0 + j >= Wv) continue;
+        float h[5][NV];
 #pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const float wk = KH == 1 ? 1.f : g[k];
-            mx += wk * hs[(0 * kLH + r + k) * kTW + j];
-            my += wk * hs[(1 * kLH + r + k) * kTW + j];
-            exx += wk * hs[(2 * kLH + r + k) * kTW + j];
-            eyy += wk * hs[(3 * kLH + r + k) * kTW + j];
-            exy += wk * hs[(4 * kLH + r + k) * kTW + j];
+        for (int q = 0; q < 5; ++q)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) h[q][k] = hs[(q * kLH + r0 + k) * kTW + j];
+#pragma unroll
+        for (int rr = 0; rr < kR; ++rr) {
+            const int oy = y0 + r0 + rr, ox = x0 + j;
+            if (oy >= Hv) break;
+            float m[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                m[q] = 0.f;
+#pragma unroll
+                for (int k = 0; k < KH; ++k) m[q] += (KH == 1 ? 1.f : g[k]) * h[q][rr + k];
+            }
+            const float mx = m[0], my = m[1], exx = m[2], eyy = m[3], exy = m[4];
+            const float sxx = exx - mx * mx, syy = eyy - my * my, sxy = exy - mx * my;
+            const float a1 = 2.f * mx * my + c1, b1 = mx * mx + my * my + c1;
+            const float a2 = 2.f * sxy + c2, b2 = sxx + syy + c2;
+            // both denominators are >= C1 or C2 > 0: one hardware reciprocal each
+            // (1 ulp) serves every quotient below
+            const float rb1 = __builtin_amdgcn_rcpf(b1), rb2 = __builtin_amdgcn_rcpf(b2);
+            const float lum = a1 * rb1, cs = a2 * rb2;
+            acc += lum * cs;
+            const long long o = base + (long long)oy * W + ox;
+            // dS/dmy = dL/dmy cs + L dcs/dmy; dS/deyy = -L cs / b2; dS/dexy = 2 L / b2
+            maps[(0 * C + c) * P + o] = 2.f * ((mx - lum * my) * rb1 * cs + lum * (my * cs - mx) * rb2);
+            maps[(1 * C + c) * P + o] = -lum * cs * rb2;
+            maps[(2 * C + c) * P + o] = 2.f * lum * rb2;
         }
-        const float sxx = exx - mx * mx, syy = eyy - my * my, sxy = exy - mx * my;
-        const float a1 = 2.f * mx * my + c1, b1 = mx * mx + my * my + c1;
-        const float a2 = 2.f * sxy + c2, b2 = sxx + syy + c2;
-        // both denominators are >= C1 or C2 > 0: one hardware reciprocal each
-        // (1 ulp) serves every quotient below
-        const float rb1 = __builtin_amdgcn_rcpf(b1), rb2 = __builtin_amdgcn_rcpf(b2);
-        const float lum = a1 * rb1, cs = a2 * rb2;
-        acc += lum * cs;
-        const long long o = base + (long long)oy * W + ox;
-        // dS/dmy = dL/dmy cs + L dcs/dmy; dS/deyy = -L cs / b2; dS/dexy = 2 L / b2
-        maps[(0 * C + c) * P + o] = 2.f * ((mx - lum * my) * rb1 * cs + lum * (my * cs - mx) * rb2);
-        maps[(1 * C + c) * P + o] = -lum * cs * rb2;
-        maps[(2 * C + c) * P + o] = 2.f * lum * rb2;
     }
     return acc;
 }
 
-// Horizontal sums of the three maps of one channel.
+// Horizontal sums of the three maps of one channel, kQ outputs per thread.
 template <int KW>
 __device__ __forceinline__ void maps_h(const float* sm, float* hs, const float (&g)[kMaxTaps], int rows, int tid) {
-    for (int i = tid; i < rows * kTW; i += kThreads) {
-        const int r = i / kTW, j = i - r * kTW;
-        float a = 0.f, bb = 0.f, cc = 0.f;
+    constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
+    for (int i = tid; i < rows * (kTW / kQ); i += kThreads) {
+        const int r = i / (kTW / kQ), j0 = (i - r * (kTW / kQ)) * kQ;
 #pragma unroll
-        for (int k = 0; k < KW; ++k) {
-            const float wk = KW == 1 ? 1.f : g[k];
-            a += wk * sm[(0 * kLH + r) * kLW + j + k];
-            bb += wk * sm[(1 * kLH + r) * kLW + j + k];
-            cc += wk * sm[(2 * kLH + r) * kLW + j + k];
+        for (int mm = 0; mm < 3; ++mm) {
+            float v[4 * N4];
+#pragma unroll
+            for (int q = 0; q < N4; ++q) {
+                const float4 a = *reinterpret_cast<const float4*>(sm + (mm * kLH + r) * kLWp + j0 + 4 * q);
+                v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+            }
+            float out[kQ];
+#pragma unroll
+            for (int o = 0; o < kQ; ++o) {
+                out[o] = 0.f;
+#pragma unroll
+                for (int k = 0; k < KW; ++k) out[o] += (KW == 1 ? 1.f : g[k]) * v[o + k];
+            }
+            *reinterpret_cast<float4*>(hs + (mm * kLH + r) * kTW + j0) = make_float4(out[0], out[1], out[2], out[3]);
         }
-        hs[(0 * kLH + r) * kTW + j] = a;
-        hs[(1 * kLH + r) * kTW + j] = bb;
-        hs[(2 * kLH + r) * kTW + j] = cc;
     }
 }
 
-// Vertical sums and the gradient of channel c for the tile's pixels, into tg.
+// Vertical sums and the gradient of channel c for the tile's pixels, kR
+// rows per thread, into tg.
 template <int C, int KH>
 __device__ __forceinline__ void grad_v(const float* hs, const float (&g)[kMaxTaps], int c, float sc, float l1s,
                                        const float* tx_, const float* ty_, float* tg, int tid) {
-    constexpr int rowf = kTW * C;
-    for (int i = tid; i < kTH * kTW; i += kThreads) {
-        const int r = i / kTW, j = i - r * kTW;
-        float a = 0.f, bb = 0.f, cc = 0.f;
+    constexpr int rowf = kTW * C, NV = kR + KH - 1;
+    for (int i = tid; i < (kTH / kR) * kTW; i += kThreads) {
+        const int rg = i / kTW, j = i - rg * kTW;
+        const int r0 = rg * kR;
+        float h[3][NV];
 #pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const float wk = KH == 1 ? 1.f : g[k];
-            a += wk * hs[(0 * kLH + r + k) * kTW + j];
-            bb += wk * hs[(1 * kLH + r + k) * kTW + j];
-            cc += wk * hs[(2 * kLH + r + k) * kTW + j];
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) h[q][k] = hs[(q * kLH + r0 + k) * kTW + j];
+#pragma unroll
+        for (int rr = 0; rr < kR; ++rr) {
+            float m[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                m[q] = 0.f;
+#pragma unroll
+                for (int k = 0; k < KH; ++k) m[q] += (KH == 1 ? 1.f : g[k]) * h[q][rr + k];
+            }
+            const int e = (r0 + rr) * rowf + j * C + c;
+            const float xv = tx_[e], yv = ty_[e];
+            const float d = yv - xv;
+            const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
+            tg[e] = sc * (m[0] + 2.f * yv * m[1] + xv * m[2]) + l1s * sg;
         }
-        const int e = r * rowf + j * C + c;
-        const float xv = tx_[e], yv = ty_[e];
-        const float d = yv - xv;
-        const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-        tg[e] = sc * (a + 2.f * yv * bb + xv * cc) + l1s * sg;
     }
 }
 
@@ -201,9 +256,9 @@ __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float
                                                         float* __restrict__ maps, long long P,
                                                         float* __restrict__ partial) {
     extern __shared__ float lds[];
-    float* xs = lds;                       // [C][kLH][kLW]
-    float* ys = xs + C * kLH * kLW;        // [C][kLH][kLW]
-    float* hs = ys + C * kLH * kLW;        // [5][kLH][kTW]
+    float* xs = lds;                       // [C][kLH][kLWp]
+    float* ys = xs + C * kLH * kLWp;       // [C][kLH][kLWp]
+    float* hs = ys + C * kLH * kLWp;       // [5][kLH][kTW]
     __shared__ float red[kThreads / 64];
     const int tile = blockIdx.x;
     const int b = view_of_tile(v, tile);
@@ -239,15 +294,15 @@ __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float
         if (i >= n_stage) break;
         const int r = i / rowf, e = i - r * rowf;
         const int q = e / C, c = e - q * C;
-        xs[(c * kLH + r) * kLW + q] = xv[it];
-        ys[(c * kLH + r) * kLW + q] = yv[it];
+        xs[(c * kLH + r) * kLWp + q] = xv[it];
+        ys[(c * kLH + r) * kLWp + q] = yv[it];
     }
     __syncthreads();
     float g[kMaxTaps];   // the window in registers (every tap loop below is unrolled)
     for (int k = 0; k < kMaxTaps; ++k) g[k] = v.win[k];
     for (int c = 0; c < C; ++c) {
-        const float* sx = xs + c * kLH * kLW;
-        const float* sy = ys + c * kLH * kLW;
+        const float* sx = xs + c * kLH * kLWp;
+        const float* sy = ys + c * kLH * kLWp;
         if (kw == 1) moments_h<1>(sx, sy, hs, g, kTH + kh - 1, tid);
         else moments_h<kMaxTaps>(sx, sy, hs, g, kTH + kh - 1, tid);
         __syncthreads();
@@ -272,8 +327,8 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
                                                         const float* __restrict__ dl1, float* __restrict__ grad,
                                                         int accumulate, float* __restrict__ l1_partial) {
     extern __shared__ float lds[];
-    float* sm = lds;                       // [3][kLH][kLW]   one channel's maps with the halo
-    float* hs = sm + 3 * kLH * kLW;        // [3][kLH][kTW]   their horizontal sums
+    float* sm = lds;                       // [3][kLH][kLWp]  one channel's maps with the halo
+    float* hs = sm + 3 * kLH * kLWp;       // [3][kLH][kTW]   their horizontal sums
     float* tx_ = hs + 3 * kLH * kTW;       // [kTH][kTW * C]  the tile's x, pixel-major
     float* ty_ = tx_ + C * kTH * kTW;      // [kTH][kTW * C]  its y
     float* tg = ty_ + C * kTH * kTW;       // [kTH][kTW * C]  its gradient
@@ -320,9 +375,11 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
         }
     }
     constexpr int n_halo = kLH * kLW, it_halo = (n_halo + kThreads - 1) / kThreads;
-    for (int c = 0; c < C; ++c) {
-        // windows q with q <= p < q + taps: rows y0 - hy .. y0 + kTH - 1
-        float a[it_halo], bb[it_halo], cc[it_halo];
+    // One channel's maps with their halo: windows q with q <= p < q + taps,
+    // rows y0 - hy .. y0 + kTH - 1.  The next channel's loads are issued
+    // before this channel is filtered, so their latency hides behind it.
+    float a[it_halo], bb[it_halo], cc[it_halo];
+    auto load_maps = [&](int c) {
 #pragma unroll
         for (int it = 0; it < it_halo; ++it) {
             const int i = tid + it * kThreads;
@@ -336,16 +393,21 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
                 cc[it] = maps[(2 * C + c) * P + o];
             }
         }
+    };
+    load_maps(0);
+    for (int c = 0; c < C; ++c) {
 #pragma unroll
         for (int it = 0; it < it_halo; ++it) {
             const int i = tid + it * kThreads;
             if (i < n_halo) {
-                sm[0 * n_halo + i] = a[it];
-                sm[1 * n_halo + i] = bb[it];
-                sm[2 * n_halo + i] = cc[it];
+                const int r = i / kLW, q = i - r * kLW;
+                sm[(0 * kLH + r) * kLWp + q] = a[it];
+                sm[(1 * kLH + r) * kLWp + q] = bb[it];
+                sm[(2 * kLH + r) * kLWp + q] = cc[it];
             }
         }
         __syncthreads();
+        if (c + 1 < C) load_maps(c + 1);
         if (kw == 1) maps_h<1>(sm, hs, g, kTH + hy, tid);
         else maps_h<kMaxTaps>(sm, hs, g, kTH + hy, tid);
         __syncthreads();

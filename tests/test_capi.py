@@ -54,7 +54,7 @@ def test_bad_arguments_fail_without_a_gpu(L):
     Hbig = _lib.int_array([70000])
     assert L.ds_raster_tile_count(1, Hbig, W, 16) == -1
     from diffsoup_amd import losses  # noqa: F401  (declares the SSIM signatures)
-    assert L.ds_ssim_tiles(1, H, W, 11, None) == 4          # 54 x 54 windows in 64 x 16 tiles
+    assert L.ds_ssim_tiles(1, H, W, 11, None) == 4          # 54 x 54 windows in 60 x 16 tiles
     assert L.ds_ssim_tiles(1, H, W, 7, None) == -1 and b"ssim_tiles" in L.ds_last_error()
 
 
