@@ -7,7 +7,8 @@
 namespace ds {
 
 // One-workgroup (1024 threads) exclusive or inclusive scan of n int32 counts
-// into out, in passes of 8192: a coalesced load into LDS, a run of 8
+// into out, in passes of 8192 (each pass's loads issued during the previous
+// pass): a coalesced load into LDS, a run of 8
 // consecutive items per thread (LDS index padded by one word per 32 so the
 // runs are read without bank conflicts), one wave scan of the run totals,
 // one barrier for the wave totals, and - for int output - the results staged
@@ -22,13 +23,25 @@ __device__ inline long long block_scan_1024(const int* in, long long n, OUT* out
     const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
     auto pad = [](int i) { return i + (i >> 5); };
     long long carry = 0;
+    // the next pass's loads are issued before this pass is scanned, so their
+    // latency overlaps the LDS work instead of following it
+    int nxt[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = k * 1024 + tid;
+        nxt[k] = j < n ? in[j] : 0;
+    }
     for (long long base = 0; base < n; base += kPass) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int j = k * 1024 + tid;
-            buf[pad(j)] = base + j < n ? in[base + j] : 0;
-        }
+        for (int k = 0; k < 8; ++k) buf[pad(k * 1024 + tid)] = nxt[k];
         __syncthreads();
+        if (base + kPass < n) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const long long j = base + kPass + k * 1024 + tid;
+                nxt[k] = j < n ? in[j] : 0;
+            }
+        }
         int v[8];
         long long s = 0;
 #pragma unroll

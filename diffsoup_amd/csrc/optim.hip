@@ -19,7 +19,8 @@ namespace {
 
 constexpr int kMaxGroups = 16;
 constexpr int kThreads = 256;
-constexpr int kChunk = 4 * kThreads;   // units per chunk: one float4 per thread
+constexpr int kPerThread = 1;          // float4s per thread per chunk, issued together
+constexpr int kChunk = 4 * kPerThread * kThreads;   // units per chunk
 constexpr int kResidentBlocks = 256 * 8;  // 256 CUs x 8 blocks of 4 waves
 
 struct Group {
@@ -66,53 +67,69 @@ __device__ __forceinline__ float adam_update(const Group& G, float step_size, fl
 // resident wave of blocks and no tail of stragglers is left.
 __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks,
                                                    const long long* __restrict__ step_before) {
-    __shared__ double bc[kMaxGroups][2];
+    // per group, once per block: the bias corrections (in double, as torch
+    // forms beta ** step on the host) and the fp32 factors derived from them
+    __shared__ float coef[kMaxGroups][4];   // lr / bc1, sqrt(bc2), bc1, bc2
     if (threadIdx.x < (unsigned)tab.n) {
         const Group& G = tab.grp[threadIdx.x];
         const double t = static_cast<double>(*step_before + 1);
-        bc[threadIdx.x][0] = 1.0 - pow(G.b1, t);
-        bc[threadIdx.x][1] = 1.0 - pow(G.b2, t);
+        const double bc1 = 1.0 - pow(G.b1, t), bc2 = 1.0 - pow(G.b2, t);
+        coef[threadIdx.x][0] = static_cast<float>(G.lr / bc1);
+        coef[threadIdx.x][1] = static_cast<float>(sqrt(bc2));
+        coef[threadIdx.x][2] = static_cast<float>(bc1);
+        coef[threadIdx.x][3] = static_cast<float>(bc2);
     }
     __syncthreads();
     for (long long c = blockIdx.x; c < n_chunks; c += gridDim.x) {
         int gi = 0;
         while (gi + 1 < tab.n && c >= tab.grp[gi + 1].block_base) ++gi;
         const Group& G = tab.grp[gi];
-        const double bc1 = bc[gi][0], bc2 = bc[gi][1];
         const long long u0 = (c - G.block_base) * kChunk;
         const long long u1 = min(u0 + kChunk, G.units);
         if (G.dim == 0) {
-            const float step_size = static_cast<float>(G.lr / bc1);
-            const float bc2_sqrt = static_cast<float>(sqrt(bc2));
+            const float step_size = coef[gi][0];
+            const float bc2_sqrt = coef[gi][1];
             const bool vec = ((reinterpret_cast<uintptr_t>(G.p) | reinterpret_cast<uintptr_t>(G.out) |
                                reinterpret_cast<uintptr_t>(G.g) | reinterpret_cast<uintptr_t>(G.m) |
                                reinterpret_cast<uintptr_t>(G.v)) & 15u) == 0 &&
                              (reinterpret_cast<uintptr_t>(G.out16) & 7u) == 0;
             long long i = u0;
             if (vec) {
-                // one float4 per thread: u0 is a multiple of kChunk, so the
-                // accesses stay 16-byte aligned
+                // kPerThread float4s per thread, all loaded before any is
+                // used: u0 is a multiple of kChunk, so the accesses stay
+                // 16-byte aligned
                 const long long n4 = (u1 - u0) / 4;
-                const long long q = threadIdx.x;
-                if (q < n4) {
-                    const long long e = u0 + 4 * q;
-                    const float4 p = *reinterpret_cast<const float4*>(G.p + e);
-                    const float4 g = *reinterpret_cast<const float4*>(G.g + e);
-                    float4 m = *reinterpret_cast<const float4*>(G.m + e);
-                    float4 v = *reinterpret_cast<const float4*>(G.v + e);
-                    float4 o;
-                    o.x = adam_update(G, step_size, bc2_sqrt, p.x, g.x, m.x, v.x);
-                    o.y = adam_update(G, step_size, bc2_sqrt, p.y, g.y, m.y, v.y);
-                    o.z = adam_update(G, step_size, bc2_sqrt, p.z, g.z, m.z, v.z);
-                    o.w = adam_update(G, step_size, bc2_sqrt, p.w, g.w, m.w, v.w);
-                    *reinterpret_cast<float4*>(G.m + e) = m;
-                    *reinterpret_cast<float4*>(G.v + e) = v;
-                    *reinterpret_cast<float4*>(G.out + e) = o;
-                    if (G.out16) {
-                        uint2 h;
-                        h.x = bf16_bits(o.x) | (bf16_bits(o.y) << 16);
-                        h.y = bf16_bits(o.z) | (bf16_bits(o.w) << 16);
-                        *reinterpret_cast<uint2*>(G.out16 + e) = h;
+                float4 p[kPerThread], g[kPerThread], m[kPerThread], v[kPerThread];
+#pragma unroll
+                for (int k = 0; k < kPerThread; ++k) {
+                    const long long q = threadIdx.x + k * kThreads;
+                    if (q < n4) {
+                        const long long e = u0 + 4 * q;
+                        p[k] = *reinterpret_cast<const float4*>(G.p + e);
+                        g[k] = *reinterpret_cast<const float4*>(G.g + e);
+                        m[k] = *reinterpret_cast<const float4*>(G.m + e);
+                        v[k] = *reinterpret_cast<const float4*>(G.v + e);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kPerThread; ++k) {
+                    const long long q = threadIdx.x + k * kThreads;
+                    if (q < n4) {
+                        const long long e = u0 + 4 * q;
+                        float4 o;
+                        o.x = adam_update(G, step_size, bc2_sqrt, p[k].x, g[k].x, m[k].x, v[k].x);
+                        o.y = adam_update(G, step_size, bc2_sqrt, p[k].y, g[k].y, m[k].y, v[k].y);
+                        o.z = adam_update(G, step_size, bc2_sqrt, p[k].z, g[k].z, m[k].z, v[k].z);
+                        o.w = adam_update(G, step_size, bc2_sqrt, p[k].w, g[k].w, m[k].w, v[k].w);
+                        *reinterpret_cast<float4*>(G.m + e) = m[k];
+                        *reinterpret_cast<float4*>(G.v + e) = v[k];
+                        *reinterpret_cast<float4*>(G.out + e) = o;
+                        if (G.out16) {
+                            uint2 h;
+                            h.x = bf16_bits(o.x) | (bf16_bits(o.y) << 16);
+                            h.y = bf16_bits(o.z) | (bf16_bits(o.w) << 16);
+                            *reinterpret_cast<uint2*>(G.out16 + e) = h;
+                        }
                     }
                 }
                 i = u0 + 4 * n4;
@@ -128,7 +145,7 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
         } else {
             // VectorAdam: g1 = b1 g1 + (1 - b1) g; g2 = b2 g2 + (1 - b2) |g|^2;
             // p -= lr * (g1 / bc1) / (sqrt(g2 / bc2) + 1e-8)
-            const float ibc1 = static_cast<float>(bc1), ibc2 = static_cast<float>(bc2);
+            const float ibc1 = coef[gi][2], ibc2 = coef[gi][3];
             const int k = G.dim;
             for (long long u = u0 + threadIdx.x; u < u1; u += kThreads) {
                 const long long base = u * k;

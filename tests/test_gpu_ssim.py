@@ -70,7 +70,7 @@ def test_ssim_reads_permuted_channels_last_images(dev):
     b = y.float().to(dev).permute(0, 3, 1, 2).contiguous().requires_grad_(True)
     s2 = losses.ssim(x.float().to(dev).permute(0, 3, 1, 2).contiguous(), b, data_range=1.0)
     s2.backward()
-    assert float(s1) == float(s2)
+    assert float(s1.detach()) == float(s2.detach())
     np.testing.assert_array_equal(a.grad.cpu().numpy(), b.grad.permute(0, 2, 3, 1).cpu().numpy())
 
 
@@ -80,7 +80,7 @@ def test_identical_images(dev):
     y = x.float().to(dev).requires_grad_(True)
     s = losses.ssim(x.float().to(dev), y, data_range=1.0)
     s.backward()
-    assert abs(float(s) - 1.0) < 1e-6
+    assert abs(float(s.detach()) - 1.0) < 1e-6
     assert float(y.grad.abs().max()) < 1e-4   # SSIM is stationary at Y = X
 
 
