@@ -247,3 +247,31 @@ def test_core_dropin_splitters_match_reference_vectors(golden):
     _same(got, [g[f"c_wide_{k}"] for k in ("verts", "faces", "face_map", "flags")])
     with pytest.raises(TypeError):
         _core.split_triangle_soup(g["w_cap_in_verts"].astype(np.float64), g["w_cap_in_faces"], 1, 0.0)
+
+
+@pytest.mark.skipif(not O.ref_remesh_available(), reason="oracle/_ref not built (the reference's sources are absent)")
+def test_edge_cases_match_reference_build():
+    """Empty soups, vertices without faces, degenerate triangles (repeated
+    corners: zero-length edges are still queued in world space and never in
+    screen space), nothing eligible, a zero and a one-split budget."""
+    import diffsoup_amd as ds
+    empty_v, empty_f = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32)
+    _same(ds.split_triangle_soup(_t(empty_v), _t(empty_f), 5), O.ref_split_soup(empty_v, empty_f, 5))
+    lone = np.random.default_rng(4).uniform(size=(7, 3)).astype(np.float32)
+    _same(ds.split_triangle_soup(_t(lone), _t(empty_f), 5, 0.1), O.ref_split_soup(lone, empty_f, 5, 0.1))
+    v, f = random_soup(20, 11)
+    f[3] = [f[3][0], f[3][0], f[3][0]]          # a point
+    f[4] = [f[4][0], f[4][1], f[4][1]]          # a segment
+    for ns, tau in [(0, 0.0), (1, 0.0), (60, 0.0), (-1, 0.08)]:
+        _same(ds.split_triangle_soup(_t(v), _t(f), ns, tau), O.ref_split_soup(v, f, ns, tau))
+    from diffsoup_amd import scenes
+    s = scenes.small_random_soup(3)
+    m = s.mvp[0].astype(np.float32)
+    none = np.zeros(s.T, np.int32)
+    _same(ds.split_triangle_soup_clip((256, 256), _t(m), _t(s.verts), _t(s.tri), _t(none), -1, 0.01),
+          O.ref_split_soup_clip(m, s.verts, s.tri, none, -1, 0.01, 1.0))
+    tri = s.tri.copy()
+    tri[:50, 1] = tri[:50, 0]                    # degenerate on screen too
+    ok = np.ones(s.T, np.int32)
+    _same(ds.split_triangle_soup_clip((256, 256), _t(m), _t(s.verts), _t(tri), _t(ok), 400, 0.0),
+          O.ref_split_soup_clip(m, s.verts, tri, ok, 400, 0.0, 1.0))
