@@ -89,9 +89,16 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     if ((int)blockIdx.x < R) {
         const int k = blockIdx.x * 64 + tx;
-        float sum = 0.f;
-        if (k < nparam)
-            for (int b = ty; b < nblocks; b += 16) sum += partials[(long long)b * nparam + k];
+        // four independent sums, so a thread has four loads in flight
+        float s4[4] = {0.f, 0.f, 0.f, 0.f};
+        if (k < nparam) {
+            int b = ty;
+            for (; b + 48 < nblocks; b += 64)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s4[u] += partials[(long long)(b + 16 * u) * nparam + k];
+            for (; b < nblocks; b += 16) s4[0] += partials[(long long)b * nparam + k];
+        }
+        const float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         acc[ty][tx] = sum;
         __syncthreads();
         if (ty == 0 && k < nparam) {
@@ -102,8 +109,15 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
         return;
     }
     if ((int)blockIdx.x == R) {
-        float sum = 0.f;
-        for (int i = threadIdx.x; i < n_bg; i += blockDim.x) sum += bg[i];
+        float s4[4] = {0.f, 0.f, 0.f, 0.f};
+        {
+            int i = threadIdx.x;
+            for (; i + 3 * (int)blockDim.x < n_bg; i += 4 * blockDim.x)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s4[u] += bg[i + u * blockDim.x];
+            for (; i < n_bg; i += blockDim.x) s4[0] += bg[i];
+        }
+        float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
         for (int b = threadIdx.x; b < nblocks; b += blockDim.x) sum += partials[(long long)b * nparam + nparam - 1];
         for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
         if (tx == 0) acc[0][ty] = sum;
@@ -116,8 +130,15 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
         return;
     }
     if ((int)blockIdx.x == R + 1) {
-        long long c = 0;
-        for (int i = threadIdx.x; i < n_tiles; i += blockDim.x) c += tile_cov[i];
+        long long c4[4] = {0, 0, 0, 0};
+        {
+            int i = threadIdx.x;
+            for (; i + 3 * (int)blockDim.x < n_tiles; i += 4 * blockDim.x)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) c4[u] += tile_cov[i + u * blockDim.x];
+            for (; i < n_tiles; i += blockDim.x) c4[0] += tile_cov[i];
+        }
+        long long c = (c4[0] + c4[1]) + (c4[2] + c4[3]);
         for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
         if (tx == 0) cov[ty] = c;
         __syncthreads();
