@@ -60,6 +60,9 @@ class _Views:
         self.windows = (H - kh + 1) * (W - kw + 1)
 
 
+MAX_VIEWS = 64   # images per launch of the SSIM kernels; larger batches run in chunks
+
+
 def _fwd(views, C, X, Y, win, c1, c2):
     """Per-(view, channel) mean SSIM [B, C] (float64) and the derivative maps."""
     P = views.B * views.H * views.W
@@ -74,13 +77,37 @@ def _fwd(views, C, X, Y, win, c1, c2):
     return sums / views.windows, maps
 
 
-def _bwd(views, C, X, Y, win, maps, scale, dscale, l1_scale, dl1):
-    grad = torch.empty_like(Y)
+def _bwd(views, C, X, Y, win, maps, scale, dscale, l1_scale, dl1, grad):
     wa = (ctypes.c_float * len(win))(*win.tolist())
     sa = (ctypes.c_float * (views.B * C))(*[float(s) for s in scale])
     _lib.check(_lib.lib().ds_ssim_bwd(views.B, views.Ha, views.Wa, C, _lib.ptr(X), _lib.ptr(Y), len(win), wa,
                                       _lib.ptr(maps), sa, _lib.ptr(dscale), float(l1_scale), _lib.ptr(dl1),
                                       _lib.ptr(grad), 0, _lib.stream_ptr(Y.device)), "ds_ssim_bwd")
+
+
+def _fwd_batch(X, Y, win, c1, c2):
+    """_fwd over [B, H, W, C] images in chunks of MAX_VIEWS: the per-(image,
+    channel) means [B, C] and one (first image, views, maps) per chunk."""
+    B, H, W, C = Y.shape
+    means, chunks = [], []
+    for b0 in range(0, B, MAX_VIEWS):
+        b1 = min(B, b0 + MAX_VIEWS)
+        views = _Views(b1 - b0, H, W, len(win), Y.device)
+        s, maps = _fwd(views, C, X[b0:b1], Y[b0:b1], win, c1, c2)
+        means.append(s)
+        chunks.append((b0, views, maps))
+    return torch.cat(means), chunks
+
+
+def _bwd_batch(chunks, X, Y, win, scale, dscale, l1_scale, dl1):
+    """The gradient w.r.t. Y of sum_{b,c} scale dscale[b, c] S[b, c] (+ the L1
+    term), chunk by chunk into one [B, H, W, C] tensor."""
+    C = Y.shape[3]
+    grad = torch.empty_like(Y)
+    for b0, views, maps in chunks:
+        b1 = b0 + views.B
+        _bwd(views, C, X[b0:b1], Y[b0:b1], win, maps, [scale] * (views.B * C),
+             dscale[b0 * C:b1 * C].contiguous(), l1_scale, dl1, grad[b0:b1])
     return grad
 
 
@@ -89,27 +116,24 @@ class _SsimPerChannel(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, X, Y, win, c1, c2):
-        B, H, W, C = Y.shape
-        views = _Views(B, H, W, len(win), Y.device)
-        s, maps = _fwd(views, C, X, Y, win, c1, c2)
-        maps_x = None
+        s, chunks = _fwd_batch(X, Y, win, c1, c2)
+        chunks_x = None
         if ctx.needs_input_grad[0]:
-            _, maps_x = _fwd(views, C, Y, X, win, c1, c2)   # S is symmetric in its two images
-        ctx.save_for_backward(X, Y, maps, maps_x)
-        ctx.views, ctx.win, ctx.C = views, win, C
+            _, chunks_x = _fwd_batch(Y, X, win, c1, c2)   # S is symmetric in its two images
+        ctx.save_for_backward(X, Y)
+        ctx.chunks, ctx.chunks_x, ctx.win = chunks, chunks_x, win
         return s.float()
 
     @staticmethod
     def backward(ctx, g):
-        X, Y, maps, maps_x = ctx.saved_tensors
-        views, C = ctx.views, ctx.C
-        scale = [1.0 / views.windows] * (views.B * C)
-        dscale = g.float().contiguous()
+        X, Y = ctx.saved_tensors
+        windows = ctx.chunks[0][1].windows
+        dscale = g.float().contiguous().reshape(-1)
         gx = gy = None
         if ctx.needs_input_grad[1]:
-            gy = _bwd(views, C, X, Y, ctx.win, maps, scale, dscale, 0.0, None)
+            gy = _bwd_batch(ctx.chunks, X, Y, ctx.win, 1.0 / windows, dscale, 0.0, None)
         if ctx.needs_input_grad[0]:
-            gx = _bwd(views, C, Y, X, ctx.win, maps_x, scale, dscale, 0.0, None)
+            gx = _bwd_batch(ctx.chunks_x, Y, X, ctx.win, 1.0 / windows, dscale, 0.0, None)
         return gx, gy, None, None, None
 
 
@@ -154,23 +178,22 @@ class _L1Ssim(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, color, target, w_l1, w_ssim, win, c1, c2):
-        B, H, W, C = color.shape
-        views = _Views(B, H, W, len(win), color.device)
-        s, maps = _fwd(views, C, target, color, win, c1, c2)
+        s, chunks = _fwd_batch(target, color, win, c1, c2)
         l1 = (color - target).abs().mean(dtype=torch.float64)
-        ctx.save_for_backward(color, target, maps)
-        ctx.views, ctx.win, ctx.C, ctx.w = views, win, C, (w_l1, w_ssim)
+        ctx.save_for_backward(color, target)
+        ctx.chunks, ctx.win, ctx.w = chunks, win, (w_l1, w_ssim)
         return (w_l1 * l1 + w_ssim * 0.5 * (1.0 - s.mean())).float()
 
     @staticmethod
     def backward(ctx, g):
-        color, target, maps = ctx.saved_tensors
-        views, C = ctx.views, ctx.C
+        color, target = ctx.saved_tensors
+        B, _, _, C = color.shape
         w_l1, w_ssim = ctx.w
-        scale = [-0.5 * w_ssim / (views.B * C * views.windows)] * (views.B * C)
+        windows = ctx.chunks[0][1].windows
         gs = g.float().reshape(1).contiguous()
-        dscale = gs.expand(views.B * C).contiguous()
-        grad = _bwd(views, C, target, color, ctx.win, maps, scale, dscale, w_l1 / color.numel(), gs)
+        dscale = gs.expand(B * C).contiguous()
+        grad = _bwd_batch(ctx.chunks, target, color, ctx.win, -0.5 * w_ssim / (B * C * windows), dscale,
+                          w_l1 / color.numel(), gs)
         return grad, None, None, None, None, None, None
 
 

@@ -94,3 +94,26 @@ def test_l1_ssim_loss(dev):
     ref_loss, ref_grad = O.l1_ssim_loss_fp64(y.float().double().numpy(), x.float().double().numpy())
     assert abs(float(loss) - ref_loss) <= 1e-6
     np.testing.assert_allclose(c.grad.cpu().numpy(), ref_grad, rtol=0, atol=GRAD_TOL * np.abs(ref_grad).max())
+
+
+def test_batches_beyond_one_launch(dev):
+    """More images than one launch takes (64): the batch runs in chunks and
+    matches the oracle, per image and through the loss."""
+    from diffsoup_amd import losses
+    x, y = _images((70, 3, 12, 14), 6)
+    yg = y.float().to(dev).requires_grad_(True)
+    s = losses.ssim(x.float().to(dev), yg, data_range=1.0, size_average=False)
+    (s * torch.linspace(0.5, 1.5, 70, device=dev)).sum().backward()
+    y64 = y.clone().requires_grad_(True)
+    ref = O.ssim_fp64(x, y64, data_range=1.0, size_average=False)
+    (ref * torch.linspace(0.5, 1.5, 70, dtype=torch.float64)).sum().backward()
+    np.testing.assert_allclose(s.detach().cpu().numpy(), ref.detach().numpy(), rtol=0, atol=VAL_TOL)
+    r = y64.grad.numpy()
+    np.testing.assert_allclose(yg.grad.cpu().numpy(), r, rtol=0, atol=GRAD_TOL * np.abs(r).max())
+    xc, yc = _images((66, 9, 13, 3), 7)
+    c = yc.float().to(dev).requires_grad_(True)
+    loss = losses.l1_ssim_loss(c, xc.float().to(dev))
+    loss.backward()
+    ref_loss, ref_grad = O.l1_ssim_loss_fp64(yc.float().double().numpy(), xc.float().double().numpy())
+    assert abs(float(loss.detach()) - ref_loss) <= 1e-6
+    np.testing.assert_allclose(c.grad.cpu().numpy(), ref_grad, rtol=0, atol=GRAD_TOL * np.abs(ref_grad).max())
