@@ -3,13 +3,16 @@
 Drop-in for the operator API of python/diffsoup (kenji-tojo/diffsoup): the same
 names, arguments and autograd behaviour, with the arithmetic in hand-written
 HIP kernels for gfx950 behind the C-ABI of libdiffsoup_hip.so
-(include/diffsoup_hip.h).  There is no CPU execution path.
+(include/diffsoup_hip.h).  There is no CPU execution path for the GPU
+operators; the soup splitters are host code in the same library, as they are
+in the reference.  ``losses`` holds the reference's L1 + SSIM image loss and
+``export`` the on-disk formats of its checkpoints and viewers.
 """
 from __future__ import annotations
 
 __version__ = "0.1.0"
 
-from . import optimize
+from . import export, losses, optimize
 from .multires import (
     ColorMLP,
     accumulate_to_level,

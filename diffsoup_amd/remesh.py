@@ -160,7 +160,7 @@ def split_edges_from_training_views(resolution, mvps: torch.Tensor, verts: torch
     H, W = resolution
     dev = verts.device
     n_src = faces.shape[0]
-    perm = torch.randperm(mvps.shape[0], generator=generator, device="cpu" if generator is not None else dev)
+    perm = torch.randperm(mvps.shape[0], generator=generator, device=generator.device if generator is not None else dev)
     chosen = mvps[perm[: min(mvps.shape[0], num_views_cap)].to(mvps.device)]
     pos = project_clip(verts, chosen)
     rast = rasterize_views([(H, W)] * chosen.shape[0], pos, faces.int().contiguous(), level,
