@@ -4,11 +4,13 @@
  *
  * Conventions shared by every entry point:
  *  - pointers are device pointers unless the parameter is documented as a
- *    host array (the per-view H / W resolution arrays);
+ *    host array (the per-view H / W resolution arrays, the SSIM window and
+ *    scales, and everything the host-side soup splitters take);
  *  - `stream` is a hipStream_t (pass torch.cuda.current_stream().cuda_stream);
  *    all work is enqueued on it, nothing synchronises the host;
  *  - nothing allocates device memory; scratch space is a caller-owned,
- *    256-byte aligned workspace sized by ds_raster_workspace_size;
+ *    256-byte aligned workspace sized by ds_raster_workspace_size (the SSIM
+ *    maps and partial sums are caller-owned outputs too);
  *  - the return value is DS_OK (0) or a negative DS_ERR_* code, with a
  *    message in ds_last_error() (thread-local); nothing calls exit();
  *  - views of one batch may differ in resolution: view b's pixels occupy
@@ -349,7 +351,8 @@ int ds_fill_f32(float* p, int64_t n, float value, void* stream);
 
 /* ------------------------------------------------------------------------
  * Structural similarity of pytorch_msssim.ssim (a symmetric window of
- * win_size = 11 taps given in win, as pytorch_msssim's default Gaussian,
+ * win_size = 11 taps given in the host array win, as pytorch_msssim's
+ * default Gaussian,
  * "valid" filtering, a side shorter than the window left unfiltered,
  * c1 = (K1 R)^2, c2 = (K2 R)^2), the term of the
  * reference's loss 0.8 L1 + 0.2 (1 - SSIM) / 2 (examples/02_synthetic.py:
