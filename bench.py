@@ -61,6 +61,8 @@ def parse():
                     help="run four fused steps and a 256 MiB calibration kernel (for rocprofv3 --pmc), print nothing")
     ap.add_argument("--workload", default="soup15k", choices=["soup15k"] + sorted(WORKLOADS),
                     help="soup15k (default) is the headline measurement; the others print their own line")
+    ap.add_argument("--summarize-sq", nargs=3, metavar=("PASS1_CSV", "PASS2_CSV", "OUT_MD"),
+                    help="write the SQ counter summary of two --pmc passes of --profile-driver")
     ap.add_argument("--summarize-rocprof", nargs=4, metavar=("KERNEL_STATS_CSV", "FETCH_CSV", "WRITE_CSV", "OUT_MD"),
                     help="write the markdown summary of a kernel trace and two counter runs, then exit")
     return ap.parse_args()
@@ -70,6 +72,9 @@ def main():
     args = parse()
     if args.summarize_rocprof:
         summarize_rocprof(*args.summarize_rocprof)
+        return
+    if args.summarize_sq:
+        summarize_sq(*args.summarize_sq)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -419,6 +424,53 @@ def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
         w = res[k].get("WRITE_SIZE", 0) * ws / 1024
         out.append(f"| `{k}` | {f:.1f} | {w:.1f} | {f + w:.1f} |")
     out += ["", "Summary written by:", "```\n" + cmd_sum + "\n```"]
+    with open(out_md, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+SQ_PASS1 = ("SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU "
+            "SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_VMEM")
+SQ_PASS2 = ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_LDS_BANK_CONFLICT "
+            "SQ_VALU_MFMA_BUSY_CYCLES SQ_INSTS_MFMA")
+
+
+def summarize_sq(pass1_csv, pass2_csv, out_md):
+    """Markdown table of where each kernel's waves spend their cycles: the
+    fractions of wave-cycles spent waiting (SQ_WAIT_ANY, SQ_WAIT_INST_ANY)
+    and issuing VALU, LDS and memory instructions, with instruction counts
+    and LDS bank-conflict cycles per dispatch."""
+    import collections
+    import csv
+
+    res = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in (pass1_csv, pass2_csv):
+        for r in csv.DictReader(open(path)):
+            res[_kernel_key(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    cmd = "\n".join(
+        f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc {ctrs} --output-format csv -d $REPO/gpurun_out/sq{i} "
+        f"-o run -- python3 $REPO/bench.py --profile-driver" for i, ctrs in ((1, SQ_PASS1), (2, SQ_PASS2)))
+    out = ["# Where the step's waves spend their cycles: 15K-triangle soup, 8 views 800x800\n",
+           "Two counter passes over `bench.py --profile-driver` (four fp32 MSE steps, two L1 + SSIM steps); "
+           "`$REPO` is the repository root. Fractions are of SQ_WAVE_CYCLES (cycles summed over resident "
+           "waves); counts are per dispatch.\n", "```\n" + cmd + "\n```\n",
+           "| kernel | waves | waiting (any) | waiting (instruction) | issuing VALU | issuing LDS | VALU insts "
+           "| LDS insts | LDS bank-conflict cycles | MFMA insts |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k, ctr in res.items():
+        d = {c: sum(v) / len(v) for c, v in ctr.items()}
+        wc = d.get("SQ_WAVE_CYCLES", 0.0)
+        if not k.startswith("k_") or wc <= 0:
+            continue
+        rows.append((wc, k, d))
+    for wc, k, d in sorted(rows, reverse=True):
+        f = lambda c: d.get(c, 0.0) / wc
+        out.append(f"| `{k}` | {d.get('SQ_WAVES', 0):.0f} | {f('SQ_WAIT_ANY'):.2f} | {f('SQ_WAIT_INST_ANY'):.2f} | "
+                   f"{f('SQ_ACTIVE_INST_VALU'):.2f} | {f('SQ_ACTIVE_INST_LDS'):.2f} | {d.get('SQ_INSTS_VALU', 0):.3g} | "
+                   f"{d.get('SQ_INSTS_LDS', 0):.3g} | {d.get('SQ_LDS_BANK_CONFLICT', 0):.3g} | "
+                   f"{d.get('SQ_INSTS_MFMA', 0):.3g} |")
+    out += ["", "Summary written by:", "```\npython3 bench.py --summarize-sq " + pass1_csv + " " + pass2_csv + " "
+            + out_md + "\n```"]
     with open(out_md, "w") as fh:
         fh.write("\n".join(out) + "\n")
 

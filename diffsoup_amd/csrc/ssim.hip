@@ -58,6 +58,8 @@ constexpr int kLWp = (kLW + 3) & ~3;   // staged row stride: rows start 16-byte 
 constexpr int kQ = 4;                  // horizontal outputs per thread (one float4 of results)
 constexpr int kR = 2;                  // vertical outputs per thread
 static_assert(kTW % kQ == 0 && kTH % kR == 0, "tile edges must be multiples of the blocking");
+constexpr int kLanesPerRow = 16;       // lanes given to one staged row in the horizontal passes
+static_assert(kTW / kQ <= kLanesPerRow, "a row's float4 groups must fit its lanes");
 constexpr int kMaxTaps = 11, kMaxC = 4;
 
 struct SsimTiles {
@@ -104,8 +106,12 @@ template <int KW>
 __device__ __forceinline__ void moments_h(const float* sx, const float* sy, float* hs, const float (&g)[kMaxTaps],
                                           int rows, int tid) {
     constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
-    for (int i = tid; i < rows * (kTW / kQ); i += kThreads) {
-        const int r = i / (kTW / kQ), j0 = (i - r * (kTW / kQ)) * kQ;
+    // one row per 16 lanes (the last idle): a float4 read serves 16 lanes per
+    // LDS pass, and a pass within one row touches each bank group once
+    for (int i = tid; i < rows * kLanesPerRow; i += kThreads) {
+        const int r = i / kLanesPerRow, quad = i - r * kLanesPerRow;
+        if (quad >= kTW / kQ) continue;
+        const int j0 = quad * kQ;
         float xv[4 * N4], yv[4 * N4];
 #pragma unroll
         for (int q = 0; q < N4; ++q) {
@@ -195,8 +201,10 @@ __device__ __forceinline__ float ssim_v(const float* hs, const float (&g)[kMaxTa
 template <int KW>
 __device__ __forceinline__ void maps_h(const float* sm, float* hs, const float (&g)[kMaxTaps], int rows, int tid) {
     constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
-    for (int i = tid; i < rows * (kTW / kQ); i += kThreads) {
-        const int r = i / (kTW / kQ), j0 = (i - r * (kTW / kQ)) * kQ;
+    for (int i = tid; i < rows * kLanesPerRow; i += kThreads) {   // one row per 16 lanes, as above
+        const int r = i / kLanesPerRow, quad = i - r * kLanesPerRow;
+        if (quad >= kTW / kQ) continue;
+        const int j0 = quad * kQ;
 #pragma unroll
         for (int mm = 0; mm < 3; ++mm) {
             float v[4 * N4];
