@@ -95,6 +95,34 @@ def test_edge_grad(dev, scene):
     np.testing.assert_array_equal(c_t.grad.cpu().numpy(), gcol)
 
 
+
+@pytest.mark.parametrize("name", ["silhouette", "occlusion", "intersection", "interior"])
+def test_edge_grad_against_finite_differences(dev, name):
+    """The GPU rasterizer and edge-gradient kernel against float64 central
+    differences of L = sum g . colour from an independent point-sampling
+    renderer: to the pixel discretisation across silhouettes, occlusion edges
+    and intersection lines, twice dL/dpos inside a triangle (the estimator visits each
+    same-triangle pixel pair from both pixels), at the tolerances of the
+    oracle's own check."""
+    import diffsoup_amd as ds
+    import fd_render as F
+    sc = F.scene(name)
+    H, W = sc["H"], sc["W"]
+    T = sc["tri"].shape[0]
+    pos_t = torch.from_numpy(sc["pos"][None].copy()).to(dev)
+    tri_t = torch.from_numpy(sc["tri"]).to(dev)
+    rast = ds.rasterize_multires_triangle_alpha((H, W), pos_t, tri_t, 0, torch.ones(T, 3, 1, device=dev),
+                                                stochastic=False)
+    ids, _ = F.render(sc["pos"], sc["tri"], H, W)
+    np.testing.assert_array_equal(rast[0, ..., 3].cpu().numpy().astype(int) - 1, ids)
+    color, g = F.image(sc)
+    pos_t.requires_grad_(True)
+    out = ds.edge_grad(torch.from_numpy(color).to(dev), rast, pos_t, tri_t)
+    out.backward(torch.from_numpy(g).to(dev))
+    got = pos_t.grad[0].cpu().numpy().astype(np.float64)
+    want = F.expected(sc)
+    np.testing.assert_allclose(got, want, rtol=0, atol=F.TOLERANCE[name] * np.abs(want).max())
+
 def test_opacity_aux(dev, scene):
     import diffsoup_amd as ds
     s, pos, alpha, rast = scene

@@ -349,3 +349,34 @@ def test_opacity_aux_estimator_is_unbiased():
         d1 = (expected_loss(a0, a1 + h) - expected_loss(a0, a1 - h)) / (2 * h)
         np.testing.assert_allclose(mean_est, [d0, d1], rtol=1e-4)
         assert abs(d0) > 1.0 and abs(d1) > 1.0
+
+
+# --------------------------------------------------------------------------
+# Edge gradients: the estimator against finite differences of the image
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["silhouette", "occlusion", "intersection", "interior"])
+def test_edge_grad_against_finite_differences(name):
+    """The oracle's edge-gradient estimator against float64 central
+    differences of L = sum g . colour, rendered by an independent
+    point-sampling renderer (tests/fd_render.py): across silhouettes,
+    occlusion edges and intersection lines it matches dL/dpos to the pixel
+    discretisation, and inside a triangle it is exactly twice dL/dpos (each
+    same-triangle pixel pair is visited from both pixels).  A wrong sign,
+    axis, image scale (W/2 vs H/2), dropped perspective (w) or depth term or
+    a mis-attributed occlusion edge fails."""
+    import fd_render as F
+    sc = F.scene(name)
+    rast = O.rasterize((sc["H"], sc["W"]), sc["pos"][None], sc["tri"], 0,
+                       np.ones((sc["tri"].shape[0], 3), np.float32), stochastic=False)
+    ids, _ = F.render(sc["pos"], sc["tri"], sc["H"], sc["W"])
+    np.testing.assert_array_equal(rast[0, ..., 3].astype(int) - 1, ids)
+    color, g = F.image(sc)
+    est = O.edge_grad(color, g, rast, sc["pos"][None], sc["tri"])[0].astype(np.float64)
+    want = F.expected(sc)
+    assert np.abs(want[:, [0, 1, 3]]).max(0).min() > 0.05 * np.abs(want).max()   # x, y and w terms all matter
+    if name == "intersection":
+        assert np.abs(want[:, 2]).min() > 0.1 * np.abs(want).max()   # depth moves the intersection line
+    else:
+        assert np.all(want[:, 2] == 0) and np.all(est[:, 2] == 0)    # depth moves no pixel here
+    np.testing.assert_allclose(est, want, rtol=0, atol=F.TOLERANCE[name] * np.abs(want).max())
