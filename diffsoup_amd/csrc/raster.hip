@@ -174,12 +174,21 @@ __device__ __forceinline__ TileWalk tile_walk(const Views& v, int T, const int* 
     return w;
 }
 
+// One chunk of a tile's bin, staged in LDS.  Rectangles are packed as two
+// 16-bit pairs, (y0 | y1 << 16, x0 | x1 << 16): views are at most 65535
+// pixels a side, so every bound fits, and the 8 bytes saved per entry keep
+// the 16x16 resolve and opacity kernels at eight workgroups per CU.
 template <int TS>
 struct TileStage {
     float4 p0[TS * TS], p1[TS * TS], p2[TS * TS];
-    int4 rect[TS * TS];
+    uint2 rect[TS * TS];
     int tri[TS * TS];
 };
+
+__device__ __forceinline__ uint2 pack_rect(int4 r) {
+    return make_uint2(static_cast<uint32_t>(r.x) | static_cast<uint32_t>(r.y) << 16,
+                      static_cast<uint32_t>(r.z) | static_cast<uint32_t>(r.w) << 16);
+}
 
 template <int TS>
 __device__ __forceinline__ void stage_chunk(TileStage<TS>& st, const TileWalk& w, int c0, int m, int T,
@@ -190,20 +199,9 @@ __device__ __forceinline__ void stage_chunk(TileStage<TS>& st, const TileWalk& w
         st.p0[threadIdx.x] = r.p0;
         st.p1[threadIdx.x] = r.p1;
         st.p2[threadIdx.x] = r.p2;
-        st.rect[threadIdx.x] = r.rect;
+        st.rect[threadIdx.x] = pack_rect(r.rect);
         st.tri[threadIdx.x] = t;
     }
-}
-
-// Wave-uniform test: does rectangle r touch this wave's 8x8 pixel quad?
-__device__ __forceinline__ bool quad_hit(int4 r, int qx0, int qy0) {
-    const int y0 = __builtin_amdgcn_readfirstlane(r.x), y1 = __builtin_amdgcn_readfirstlane(r.y);
-    const int x0 = __builtin_amdgcn_readfirstlane(r.z), x1 = __builtin_amdgcn_readfirstlane(r.w);
-    return !(x1 <= qx0 || x0 >= qx0 + 8 || y1 <= qy0 || y0 >= qy0 + 8);
-}
-
-__device__ __forceinline__ bool in_rect(int4 r, int px, int py) {
-    return px >= r.z && px < r.w && py >= r.x && py < r.y;
 }
 
 // Dense (triangle, pixel) candidates of one wave's 8x8 quad.  The staged
@@ -216,7 +214,7 @@ __device__ __forceinline__ bool in_rect(int4 r, int px, int py) {
 // stay busy: a triangle covers ~10 of a quad's 64 pixels.
 struct CandGroup {
     int start[32];    // exclusive prefix of the candidate counts
-    int4 box[32];     // (x0, y0, width, staged index) of each intersection
+    uint2 box[32];    // (x0 | y0 << 16, width | staged index << 8) of each intersection
     float ndcx[8];    // pixel-centre NDC of the quad's columns / rows
     float ndcy[8];
 };
@@ -233,10 +231,11 @@ __device__ __forceinline__ void for_each_candidate(const TileStage<TS>& st, int 
         int area = 0, ix0 = 0, iy0 = 0, wd = 1;
         const int j = g0 + lane;
         if (lane < 32 && j < m) {
-            const int4 r = st.rect[j];
-            ix0 = max(r.z, w.qx0);
-            iy0 = max(r.x, w.qy0);
-            const int ix1 = min(r.w, w.qx0 + 8), iy1 = min(r.y, w.qy0 + 8);
+            const uint2 r = st.rect[j];
+            ix0 = max(static_cast<int>(r.y & 0xffffu), w.qx0);
+            iy0 = max(static_cast<int>(r.x & 0xffffu), w.qy0);
+            const int ix1 = min(static_cast<int>(r.y >> 16), w.qx0 + 8);
+            const int iy1 = min(static_cast<int>(r.x >> 16), w.qy0 + 8);
             if (ix1 > ix0 && iy1 > iy0) {
                 wd = ix1 - ix0;
                 area = wd * (iy1 - iy0);
@@ -250,7 +249,8 @@ __device__ __forceinline__ void for_each_candidate(const TileStage<TS>& st, int 
         const int total = __shfl(inc, 31, kWave);
         if (lane < 32) {
             c.start[lane] = inc - area;
-            c.box[lane] = make_int4(ix0, iy0, wd, j);
+            c.box[lane] = make_uint2(static_cast<uint32_t>(ix0) | static_cast<uint32_t>(iy0) << 16,
+                                     static_cast<uint32_t>(wd) | static_cast<uint32_t>(j) << 8);
         }
         wave_lds_sync();
         for (int k0 = 0; k0 < total; k0 += kWave) {
@@ -261,11 +261,13 @@ __device__ __forceinline__ void for_each_candidate(const TileStage<TS>& st, int 
                     const int mid = (lo + hi + 1) >> 1;
                     if (c.start[mid] <= k) lo = mid; else hi = mid - 1;
                 }
-                const int4 bx = c.box[lo];
+                const uint2 bx = c.box[lo];
+                const int bw = static_cast<int>(bx.y & 0xffu);
                 const int local = k - c.start[lo];
-                const int dy = local / bx.z, dx = local - dy * bx.z;
-                const int px = bx.x + dx, py = bx.y + dy;
-                f(bx.w, px, py, (py - w.qy0) * 8 + (px - w.qx0), c.ndcx[px - w.qx0], c.ndcy[py - w.qy0]);
+                const int dy = local / bw, dx = local - dy * bw;
+                const int px = static_cast<int>(bx.x & 0xffffu) + dx, py = static_cast<int>(bx.x >> 16) + dy;
+                f(static_cast<int>(bx.y >> 8), px, py, (py - w.qy0) * 8 + (px - w.qx0), c.ndcx[px - w.qx0],
+                  c.ndcy[py - w.qy0]);
             }
         }
         wave_lds_sync();
