@@ -228,9 +228,15 @@ __device__ __forceinline__ bool edge_eval(const Views& v, int T, int C, const fl
 
         Hit h;
         if (it1 == it0) {
-            // interior: the boundary does not change triangles
+            // interior: the boundary does not change triangles.  The
+            // neighbour, covered by the same triangle, evaluates this pair
+            // too and gets the same contribution bit for bit (the adjoint
+            // and the step direction both change sign; the midpoint is
+            // shared), so the -x / -y pixel of the pair skips it and the
+            // +x / +y pixel adds it twice (exactly 2x, by linearity).
+            if (DX[e] + DY[e] < 0) continue;
             if (cover(my, mx, P0[0], P0[1], P0[2], h)) {
-                spread(g, h.b0, h.b1, h.z, h.w, mx, my, gx, gy, 0.f, false);
+                spread(g, h.b0, h.b1, h.z, h.w, mx, my, 2.f * gx, 2.f * gy, 0.f, false);
                 touched = true;
             }
             continue;

@@ -600,7 +600,7 @@ size_t photo_loss_floats(const Views& views, int C) {
     const long long tw = make_tiles(views.n, views.H, views.W, C, kMaxTaps, ones, true, w);
     const long long tp = make_tiles(views.n, views.H, views.W, C, kMaxTaps, ones, false, p);
     if (tw < 0 || tp < 0) return 0;
-    // maps [3, P, C], window partials [tw, C], pixel-tile L1 partials [tp]
+    // maps [3, C, P], window partials [tw, C], pixel-tile L1 partials [tp]
     return 3 * (size_t)w.pix_base[views.n] * C + (size_t)tw * C + (size_t)tp;
 }
 
