@@ -42,6 +42,23 @@ struct Table {
     int n;
 };
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+// The updated parameters and moments are not read again before the next
+// step's optimiser launch, so their stores carry the non-temporal hint and
+// leave the caches to the next step's raster and shading kernels (measured:
+// the step gains ~0.7%; non-temporal loads as well made the update itself
+// slower).
+__device__ __forceinline__ void st_nt(float* p, float4 v) {
+    f4v x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<f4v*>(p));
+}
+__device__ __forceinline__ void st_nt2(unsigned short* p, uint2 v) {
+    u2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<u2v*>(p));
+}
+
 // bf16 bits of x, rounded to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint32_t bf16_bits(float x) {
     const uint32_t u = __float_as_uint(x);
@@ -121,14 +138,14 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                         o.y = adam_update(G, step_size, bc2_sqrt, p[k].y, g[k].y, m[k].y, v[k].y);
                         o.z = adam_update(G, step_size, bc2_sqrt, p[k].z, g[k].z, m[k].z, v[k].z);
                         o.w = adam_update(G, step_size, bc2_sqrt, p[k].w, g[k].w, m[k].w, v[k].w);
-                        *reinterpret_cast<float4*>(G.m + e) = m[k];
-                        *reinterpret_cast<float4*>(G.v + e) = v[k];
-                        *reinterpret_cast<float4*>(G.out + e) = o;
+                        st_nt(G.m + e, m[k]);
+                        st_nt(G.v + e, v[k]);
+                        st_nt(G.out + e, o);
                         if (G.out16) {
                             uint2 h;
                             h.x = bf16_bits(o.x) | (bf16_bits(o.y) << 16);
                             h.y = bf16_bits(o.z) | (bf16_bits(o.w) << 16);
-                            *reinterpret_cast<uint2*>(G.out16 + e) = h;
+                            st_nt2(G.out16 + e, h);
                         }
                     }
                 }
