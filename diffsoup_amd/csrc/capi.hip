@@ -68,6 +68,9 @@ int make_views(int B, const int* H, const int* W, int tile, ds::Views& v) {
     }
     v.tile_base[B] = static_cast<int>(tiles);
     v.pix_base[B] = static_cast<int>(pix);
+    v.interleave = 1;
+    for (int b = 1; b < B; ++b)
+        if (v.tile_base[b + 1] - v.tile_base[b] != v.tile_base[1] || v.tiles_x[b] != v.tiles_x[0]) v.interleave = 0;
     return DS_OK;
 }
 

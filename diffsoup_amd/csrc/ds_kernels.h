@@ -30,6 +30,7 @@ struct Views {
     int tiles_x[kMaxViews];
     int tile_base[kMaxViews + 1];
     int pix_base[kMaxViews + 1];
+    int interleave;                // 1 when every view has the same tile grid
 };
 
 // Per-(view, triangle) setup record: clip-space corners and the conservative

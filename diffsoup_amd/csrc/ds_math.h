@@ -201,12 +201,22 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// The two rounds: the first depends on (seed, view) only, so a kernel that
+// draws many thresholds of one view forms it once.
+__host__ __device__ __forceinline__ uint64_t mask_view_hash(uint64_t seed, uint32_t view) {
+    return mix64(seed + 0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(view) + 1ull));
+}
+
+__host__ __device__ __forceinline__ float mask_threshold_in_view(uint64_t view_hash, uint32_t tri, uint32_t y,
+                                                                 uint32_t x) {
+    const uint64_t h = mix64(view_hash ^ ((static_cast<uint64_t>(tri) << 32) | (static_cast<uint64_t>(y) << 16) |
+                                          static_cast<uint64_t>(x)));
+    return static_cast<float>(static_cast<uint32_t>(h >> 40)) * (1.0f / 16777216.0f);
+}
+
 __host__ __device__ __forceinline__ float mask_threshold(uint64_t seed, uint32_t view, uint32_t tri,
                                                          uint32_t y, uint32_t x) {
-    uint64_t h = mix64(seed + 0x9e3779b97f4a7c15ull * (static_cast<uint64_t>(view) + 1ull));
-    h = mix64(h ^ ((static_cast<uint64_t>(tri) << 32) | (static_cast<uint64_t>(y) << 16) |
-                   static_cast<uint64_t>(x)));
-    return static_cast<float>(static_cast<uint32_t>(h >> 40)) * (1.0f / 16777216.0f);
+    return mask_threshold_in_view(mask_view_hash(seed, view), tri, y, x);
 }
 
 }  // namespace ds
