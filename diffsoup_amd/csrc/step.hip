@@ -82,7 +82,8 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
                                                    const float* __restrict__ alpha,
                                                    const float* __restrict__ galpha, float* __restrict__ glogits,
                                                    const int* __restrict__ tile_cov, int n_tiles,
-                                                   long long* __restrict__ stats) {
+                                                   long long* __restrict__ stats,
+                                                   unsigned long long* __restrict__ seed_advance) {
     __shared__ float acc[16][65];
     __shared__ long long cov[16];
     const int R = (nparam + 63) / 64;
@@ -146,6 +147,8 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
             long long t = 0;
             for (int q = 0; q < 16; ++q) t += cov[q];
             stats[kStatCovered] = t;
+            // the mask seed's last reader (the resolve) has run
+            if (seed_advance) *seed_advance += 1ull;
         }
         return;
     }
@@ -372,7 +375,10 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
             partials, L.nblocks, L.nparam, d->grad_mlp, bgloss, L.n_tiles, inv_n, photo ? nullptr : d->loss, d->B,
             d->V, d->mvp,
             reinterpret_cast<const float4*>(gpos), d->grad_verts, n_alpha, d->alpha, galpha, d->grad_alpha_logits,
-            rw.tile_cov, L.n_tiles, rw.stats);
+            rw.tile_cov, L.n_tiles, rw.stats,
+            d->seed_device && d->advance_seed
+                ? const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(d->seed_device))
+                : nullptr);
         DS_TRY(hipGetLastError(), "epilogue");
     }
     DS_TRY(mark(DS_STAGE_FINAL, false), "event");

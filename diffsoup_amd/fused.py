@@ -46,7 +46,7 @@ class StepDesc(ctypes.Structure):
         ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("capacity", ctypes.c_int64),
         ("timed_stage", ctypes.c_int), ("ev_begin", ctypes.c_void_p), ("ev_end", ctypes.c_void_p),
         ("seed_device", ctypes.c_void_p), ("feat_bf16", ctypes.c_void_p),
-        ("loss_kind", ctypes.c_int),
+        ("loss_kind", ctypes.c_int), ("advance_seed", ctypes.c_int),
     ]
 
 
@@ -393,8 +393,8 @@ def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5
 
 class CapturedStep:
     """One training iteration captured in a HIP graph: ds_train_step with the
-    mask seed read from device memory, the optimiser step, and a seed
-    increment.  Replay i after ``set_seed(s)`` uses seed s + i.
+    mask seed read from device memory and advanced by the step itself, and
+    the optimiser step.  Replay i after ``set_seed(s)`` uses seed s + i.
 
     The graph owns the kernel arguments of the capture: the tensors in
     ``params``, ``mvp``, ``target`` and the step's buffers must stay alive
@@ -411,15 +411,16 @@ class CapturedStep:
         step.run(params, mvp, target, seed)
         torch.cuda.synchronize(step.dev)
         step.desc.seed_device = self.seed_dev.data_ptr()
+        step.desc.advance_seed = 1
         self.graph = torch.cuda.CUDAGraph()
         try:
             with torch.cuda.graph(self.graph):
                 step.run(params, mvp, target, 0)
                 if optimizer is not None:
                     optimizer.step()
-                self.seed_dev.add_(1)
         finally:
             step.desc.seed_device = None
+            step.desc.advance_seed = 0
         self.set_seed(seed)
 
     def set_seed(self, seed: int) -> None:

@@ -213,6 +213,10 @@ typedef struct ds_step_desc {
                                     views of pytorch_msssim.ssim per view; loss
                                     then holds that value and grad_color its
                                     gradient (pix_loss stays the per-pixel L1) */
+    int advance_seed;            /* with seed_device: the step adds 1 to
+                                    *seed_device after its last read, so that
+                                    replays of a captured step draw a new mask
+                                    each without a launch of their own */
 } ds_step_desc;
 
 #define DS_LOSS_MSE 0
