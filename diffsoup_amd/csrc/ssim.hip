@@ -46,6 +46,11 @@
 #include "diffsoup_hip.h"
 #include "ds_kernels.h"
 
+// The library builds with -ffp-contract=off so that the raster arithmetic
+// rounds like the CPU restatement.  Nothing here decides a branch or has to
+// match bit for bit, so the filter taps and the SSIM terms contract into FMAs.
+#pragma clang fp contract(fast)
+
 namespace ds {
 
 int ssim_fail(int code, const char* msg);
