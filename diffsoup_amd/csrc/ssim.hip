@@ -510,21 +510,52 @@ hipError_t dispatch_grad(const SsimTiles& t, long long tiles, const float* X, co
 
 // The training step's photometric loss from the partial sums, one workgroup:
 // w_l1 * sum|y - x| / (C P) + w_ssim * (1 - SSIM) / 2, SSIM being the mean
-// over views of each view's mean over channels and windows.
+// over views of each view's mean over channels and windows.  Each thread
+// strides over all views' partials with four loads in flight, tracking the
+// view of its (increasing) index, so the loads of different views overlap.
 __global__ __launch_bounds__(1024) void k_photo_loss(SsimTiles v, const float* __restrict__ ssim_partial,
                                                      const float* __restrict__ l1_partial, int n_pix_tiles,
                                                      float w_l1, float w_ssim, float* __restrict__ loss) {
+    constexpr int U = 4;
     __shared__ double rs[1024 / 64], rl[1024 / 64];
+    __shared__ double inv[kMaxViews];   // 1 / windows of each view
+    __shared__ int end[kMaxViews];      // end of each view's partials
     const int C = v.C;
-    double s = 0.0, l = 0.0;
-    for (int b = 0; b < v.n; ++b) {   // view by view: uniform weights, no per-lane lookups
-        const double inv = 1.0 / ((double)(v.H[b] - v.kh[b] + 1) * (v.W[b] - v.kw[b] + 1));
-        double sb = 0.0;
-        for (int i = v.tile_base[b] * C + threadIdx.x; i < v.tile_base[b + 1] * C; i += blockDim.x)
-            sb += ssim_partial[i];
-        s += sb * inv;
+    if ((int)threadIdx.x < v.n) {
+        const int b = threadIdx.x;
+        inv[b] = 1.0 / ((double)(v.H[b] - v.kh[b] + 1) * (v.W[b] - v.kw[b] + 1));
+        end[b] = v.tile_base[b + 1] * C;
     }
-    for (int i = threadIdx.x; i < n_pix_tiles; i += blockDim.x) l += l1_partial[i];
+    __syncthreads();
+    const int total = v.tile_base[v.n] * C;
+    double s = 0.0, l = 0.0;
+    int b = 0;
+    for (int i0 = threadIdx.x; i0 < total; i0 += U * blockDim.x) {
+        float x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * blockDim.x;
+            x[u] = i < total ? ssim_partial[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < total) {
+                while (i >= end[b]) ++b;
+                s += x[u] * inv[b];
+            }
+        }
+    }
+    for (int i0 = threadIdx.x; i0 < n_pix_tiles; i0 += U * blockDim.x) {
+        float x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * blockDim.x;
+            x[u] = i < n_pix_tiles ? l1_partial[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) l += x[u];
+    }
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o, 64);
         l += __shfl_xor(l, o, 64);
