@@ -226,6 +226,13 @@ def main():
     out["roofline"] = stages[args.stage]
     out["stages"] = {k: {"avg_launch_us": round(v["avg_launch_ms"] * 1e3, 1), "bound": v["bound"],
                          "frac": v["frac"]} for k, v in stages.items()}
+    # the whole iteration against HBM: its compulsory bytes (forward, backward
+    # and the optimiser's 28 B per parameter) at the measured rate
+    it_bytes = st32.iteration_bytes(adam=True)
+    it_gbs = it_bytes * out["value"] / 1e9
+    out["iteration_roofline"] = {"bound": "hbm", "algorithmic_bytes": int(it_bytes), "achieved": round(it_gbs, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(it_gbs / HBM_PEAK_GBS, 4),
+                                 "ceiling_it_s": round(HBM_PEAK_GBS * 1e9 / it_bytes, 1)}
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(soup, args.cpu_seconds)
@@ -304,6 +311,11 @@ def other_workload(args, dev):
            "stages": {k: {"avg_launch_us": round(v["avg_launch_ms"] * 1e3, 1), "bound": v["bound"],
                           "achieved": v["achieved"], "unit": v["unit"], "frac": v["frac"]} for k, v in stages.items()},
            "scene_generation_s": round(gen_s, 1)}
+    it_bytes = step.iteration_bytes(adam=True)
+    it_gbs = it_bytes * out["value"] / 1e9
+    out["iteration_roofline"] = {"bound": "hbm", "algorithmic_bytes": int(it_bytes), "achieved": round(it_gbs, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(it_gbs / HBM_PEAK_GBS, 4),
+                                 "ceiling_it_s": round(HBM_PEAK_GBS * 1e9 / it_bytes, 1)}
     print(json.dumps(out), flush=True)
 
 

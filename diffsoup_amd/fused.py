@@ -317,6 +317,21 @@ class FusedStep:
             return 16.0 * P + 4.0 * P + 64.0 * self.B * self.T + 4.0 * pairs + 8.0 * self.T * self.Sa
         raise ValueError(stage)
 
+    def iteration_bytes(self, adam: bool = True) -> float:
+        """Compulsory HBM bytes of one training iteration: forward (clip
+        positions, triangles, rast, feature table, opacities, colour) and
+        backward (target, rast, feature, opacity and position gradients),
+        plus 28 B per parameter for Adam / VectorAdam (p, g, m, v read;
+        p, m, v written) over features, opacity logits, MLP and vertices."""
+        P, T, B, V = float(self.P), float(self.T), float(self.B), float(self.V)
+        n_feat = T * self.S_tot * self.C
+        n_alpha = T * self.Sa
+        s_f = 2.0 if self.bf16_features else 4.0
+        fwd = B * V * 16 + T * 12 + P * 16 + n_feat * s_f + n_alpha * 4 + P * 12
+        bwd = P * 12 + P * 16 + n_feat * 4 + n_alpha * 4 + B * V * 16
+        opt = 28.0 * (n_feat + n_alpha + self.nparam + 3 * V) if adam else 0.0
+        return fwd + bwd + opt
+
     def algorithmic_flops(self, stage: str) -> float:
         """Compulsory floating-point work of one launch of the shading stage.
 
