@@ -65,7 +65,33 @@ def parse():
                     help="write the SQ counter summary of two --pmc passes of --profile-driver")
     ap.add_argument("--summarize-rocprof", nargs=4, metavar=("KERNEL_STATS_CSV", "FETCH_CSV", "WRITE_CSV", "OUT_MD"),
                     help="write the markdown summary of a kernel trace and two counter runs, then exit")
+    ap.add_argument("--summarize-workload-trace", nargs=3, metavar=("WORKLOAD", "KERNEL_STATS_CSV", "OUT_MD"),
+                    help="write the markdown summary of a kernel trace of `bench.py --workload WORKLOAD`")
     return ap.parse_args()
+
+
+def summarize_workload_trace(workload, stats_csv, out_md):
+    """Markdown table of a kernel trace of one larger workload's bench run
+    (autotuning, the captured iterations and the per-stage timing runs)."""
+    import csv
+
+    rows = list(csv.DictReader(open(stats_csv)))
+    d = os.path.basename(os.path.dirname(stats_csv))
+    cmd = (f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+           f"-d $REPO/gpurun_out/{d} -o trace -- python3 $REPO/bench.py --workload {workload}")
+    out = [f"# Kernel trace: `bench.py --workload {workload}`\n",
+           f"Workload: {WORKLOADS[workload][2]}.  The run autotunes the tile, times the captured iteration and "
+           "then each stage; every launch of every kernel is in the table.  `$REPO` is the repository root.\n",
+           "```\n" + cmd + "\n```\n",
+           "| kernel | calls | avg us | min us | max us | % time |", "|---|---:|---:|---:|---:|---:|"]
+    for r in rows[:24]:
+        out.append(f"| `{_kernel_key(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    out += ["", "Summary written by:",
+            f"```\npython3 bench.py --summarize-workload-trace {workload} {os.path.relpath(stats_csv, ROOT)} "
+            f"{os.path.relpath(out_md, ROOT)}\n```"]
+    with open(out_md, "w") as fh:
+        fh.write("\n".join(out) + "\n")
 
 
 def main():
@@ -75,6 +101,9 @@ def main():
         return
     if args.summarize_sq:
         summarize_sq(*args.summarize_sq)
+        return
+    if args.summarize_workload_trace:
+        summarize_workload_trace(*args.summarize_workload_trace)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
