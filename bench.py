@@ -9,10 +9,14 @@ a radius-4 orbit, stochastic opacity masking with a new seed per iteration,
 MSE to a uniform random target, edge and opacity gradients.  Inputs are
 generated on the host once and are resident in HBM before timing starts.
 
-Prints one JSON line.  ``value`` is fp32 it/s of the fused step; the bf16
-variant (MLP on bf16 MFMA, features stored in bf16, fp32 accumulation and
-gradients), covered fragments per second, the roofline of the dominant
-kernel and the CPU baseline are reported beside it.
+Prints one JSON line.  ``value`` is fp32 it/s of the fused step; beside it:
+the bf16 variant (MLP on bf16 MFMA with fp32 accumulation, feature rows
+stored in bf16, every gradient fp32) with its image error against a float64
+evaluation, covered fragments per second, the roofline of the dominant
+kernel, the captured iteration of the three larger workloads (50K
+triangles over 32 uneven views, 100K triangles at 16 x 1920x1080 in bf16,
+300K triangles over 64 views) and the CPU baseline on the host's cores at
+the 1K-triangle 256x256 and the 15K-triangle 8 x 800x800 configurations.
 """
 from __future__ import annotations
 
@@ -51,8 +55,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the three larger workloads in the headline line")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: the process's CPU share, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-bf16", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--stage", default="shade", choices=["resolve", "shade", "edge", "aux"],
                     help="kernel stage reported as the roofline (the dominant one by default)")
     ap.add_argument("--no-plain", action="store_true")
@@ -160,7 +167,7 @@ def main():
         step.time_stage(None)
         return ms, wall / steps * 1e3, kms
 
-    def timed_graph(bf16, steps, warmup, bf16_features=None, loss="mse"):
+    def timed_graph(bf16, steps, warmup, bf16_features=None, loss="mse", keep_params=False):
         """The captured training iteration: forward + backward + Adam/VectorAdam,
         one graph replay per step.  The optimiser reads and writes every
         parameter and moment each iteration but stores the updated parameters
@@ -188,6 +195,8 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        if keep_params:
+            return e0.elapsed_time(e1) / steps, wall / steps * 1e3, step, tune, params
         return e0.elapsed_time(e1) / steps, wall / steps * 1e3, step, tune
 
     ms32, wall32, gstep, tune32 = timed_graph(False, args.steps, args.warmup)
@@ -221,13 +230,18 @@ def main():
         "gfrag_per_s": round(covered * 1000.0 / ms32 / 1e9, 4),
     }
     if not args.no_bf16:
-        ms16, _, g16, _ = timed_graph(True, args.steps, args.warmup)
-        del g16
+        # bf16: MLP on bf16 MFMA (fp32 accumulation), bf16 feature rows (the
+        # four residual-blend channels read from the fp32 master), fp32
+        # depth, barycentrics and gradients
+        ms16, _, g16, _, p16 = timed_graph(True, args.steps, args.warmup, keep_params=True)
         out["bf16_it_s"] = round(1000.0 / ms16, 2)
         out["bf16_vs_fp32"] = round(ms32 / ms16, 3)
-        msm, _, gm, _ = timed_graph(True, args.steps, args.warmup, bf16_features=False)
-        del gm
+        out["bf16_image_err_vs_fp64"] = float(f"{image_err_vs_fp64(g16, p16):.3e}")
+        del g16, p16
+        msm, _, gm, _, pm = timed_graph(True, args.steps, args.warmup, bf16_features=False, keep_params=True)
         out["bf16_mlp_fp32_features_it_s"] = round(1000.0 / msm, 2)
+        out["bf16_mlp_fp32_features_image_err_vs_fp64"] = float(f"{image_err_vs_fp64(gm, pm):.3e}")
+        del gm, pm
     # the same iteration with the reference's own image loss, 0.8 L1 + 0.2 (1 - SSIM) / 2
     msl, _, gl, _ = timed_graph(False, args.steps, args.warmup, loss="l1_ssim")
     del gl
@@ -262,10 +276,45 @@ def main():
     out["iteration_roofline"] = {"bound": "hbm", "algorithmic_bytes": int(it_bytes), "achieved": round(it_gbs, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(it_gbs / HBM_PEAK_GBS, 4),
                                  "ceiling_it_s": round(HBM_PEAK_GBS * 1e9 / it_bytes, 1)}
+    del st32
+
+    if not args.no_workloads:
+        # the larger workloads' captured iterations, measured in this run
+        out["workloads"] = {}
+        for name in WORKLOADS:
+            torch.cuda.empty_cache()
+            w = run_workload(name, args, dev)
+            out["workloads"][name] = {k: w[k] for k in ("value", "unit", "ms_per_step", "dtype", "gfrag_per_s",
+                                                        "iteration_roofline", "stages")}
+            out["workloads"][name]["workload"] = w["config"]["workload"]
+            out["workloads"][name]["tile"] = w["config"]["tile"]
+            print(f"workload {name}: {w['value']} it/s", file=sys.stderr, flush=True)
 
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(soup, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(soup, args.cpu_threads)
     print(json.dumps(out), flush=True)
+
+
+def image_err_vs_fp64(step, params):
+    """Largest |colour - float64 colour| over the covered pixels of the step's
+    last output.  The float64 colour takes the features gathered from the
+    fp32 master table on the step's own raster (ds_multires_color_fwd,
+    levels concatenated; its fp32 rounding is ~1e-8 here), then the MLP,
+    sigmoid and residual blend in float64 (torch on the GPU) with the fp32
+    master weights."""
+    import torch
+
+    from diffsoup_amd.multires import multires_feature_gather
+
+    cov = step.rast[:, 3] > 0
+    x = multires_feature_gather(step.rast[cov].contiguous(), step.lo, step.hi, params.feat, mode="concat").double()
+    h = x
+    for i, (w, b) in enumerate(((params.w1, params.b1), (params.w2, params.b2), (params.w3, params.b3))):
+        h = h @ w.double().T + b.double()
+        h = torch.relu(h) if i < 2 else torch.sigmoid(h)
+    r = x[:, 3:4]
+    ref = (1.0 - r) * x[:, :3] + r * h
+    return float((step.color[cov].double() - ref).abs().max().item())
 
 
 WORKLOADS = {
@@ -282,15 +331,20 @@ WORKLOADS = {
 
 
 def other_workload(args, dev):
+    """One of the larger workloads on its own: prints its JSON line."""
+    print(json.dumps(run_workload(args.workload, args, dev)), flush=True)
+
+
+def run_workload(name, args, dev):
     """One of the larger workloads: the captured training iteration
     (forward + backward + Adam/VectorAdam written out of place) with the tile
-    chosen by the autotuner, per-stage kernel times and the shading roofline.
-    Prints one JSON line."""
+    chosen by the autotuner, per-stage kernel times and their rooflines, and
+    the whole iteration against HBM."""
     import torch
 
     from diffsoup_amd import fused, scenes
 
-    gen, bf16, desc = WORKLOADS[args.workload]
+    gen, bf16, desc = WORKLOADS[name]
     t0 = time.perf_counter()
     soup = getattr(scenes, gen)(0)
     gen_s = time.perf_counter() - t0
@@ -330,7 +384,7 @@ def other_workload(args, dev):
         kms = sum(a.elapsed_time(b) for a, b in evs) / 3
         step.time_stage(None)
         stages[stage] = roofline(step, stage, kms, {})
-    out = {"metric": f"fwd+bwd iters/s ({args.workload})", "value": round(1000.0 / ms, 2), "unit": "it/s",
+    out = {"metric": f"fwd+bwd iters/s ({name})", "value": round(1000.0 / ms, 2), "unit": "it/s",
            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
            "dtype": "bf16" if bf16 else "f32", "data": "synthetic",
            "config": {"workload": desc, "triangles": soup.T, "views": soup.B, "pixels_per_step": soup.pixels,
@@ -345,21 +399,34 @@ def other_workload(args, dev):
     out["iteration_roofline"] = {"bound": "hbm", "algorithmic_bytes": int(it_bytes), "achieved": round(it_gbs, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(it_gbs / HBM_PEAK_GBS, 4),
                                  "ceiling_it_s": round(HBM_PEAK_GBS * 1e9 / it_bytes, 1)}
-    print(json.dumps(out), flush=True)
+    del cap, opt, step, params, updated
+    return out
 
 
 def roofline(step, stage, kms, traffic):
-    """Roofline entry of one stage's kernel: shading is compared with the fp32
-    MFMA peak (its arithmetic intensity is above the fp32 ridge of 19.7
-    flop/B), the raster kernels with HBM bandwidth."""
+    """Roofline entry of one stage's kernel.  The raster kernels are compared
+    with HBM bandwidth.  Shading is compared with the lower of its two roofs:
+    the MFMA peak of its precision, or its arithmetic intensity (algorithmic
+    flops over algorithmic bytes) times HBM bandwidth.  The fp32 MLP is above
+    the fp32 ridge (157.3 / 8 = 19.7 flop/B), so MFMA bounds it; the bf16
+    MLP is far below the bf16 ridge (312 flop/B), so HBM bounds it."""
     kernel = stage_kernel(stage, step)
     tr = traffic.get(kernel)
     if stage == "shade":
         flops = step.algorithmic_flops(stage)
-        achieved = flops / (kms * 1e-3) / 1e12
-        peak = MFMA_BF16_PEAK_TFLOPS if step.bf16 else MFMA_FP32_PEAK_TFLOPS
-        r = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-             "algorithmic_flops": int(flops)}
+        nbytes = step.algorithmic_bytes(stage)
+        ai = flops / nbytes
+        mfma = MFMA_BF16_PEAK_TFLOPS if step.bf16 else MFMA_FP32_PEAK_TFLOPS
+        if mfma <= ai * HBM_PEAK_GBS / 1e3:
+            achieved = flops / (kms * 1e-3) / 1e12
+            peak = mfma
+            r = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s"}
+        else:
+            achieved = nbytes / (kms * 1e-3) / 1e9
+            peak = HBM_PEAK_GBS
+            r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": peak, "unit": "GB/s",
+                 "achieved_tflops": round(flops / (kms * 1e-3) / 1e12, 2), "mfma_peak_tflops": mfma}
+        r.update(algorithmic_flops=int(flops), algorithmic_bytes=int(nbytes), flop_per_byte=round(ai, 1))
     else:
         nbytes = step.algorithmic_bytes(stage)
         achieved = nbytes / (kms * 1e-3) / 1e9
@@ -585,44 +652,73 @@ def plain_its(soup, prm, mvp, target, steps):
     return steps / (time.perf_counter() - t0)
 
 
-def cpu_baseline(soup, budget_s):
-    """The CPU oracle's full iteration on a bounded sample, one thread: C
-    raster / gather / edge / opacity, torch CPU fp32 MLP, then torch Adam and
-    VectorAdam updates of every parameter from the gradients it produced."""
+def cpu_threads(requested=0):
+    """Threads for the CPU baseline: the requested count, else the CPUs this
+    process may run on, capped by OMP_NUM_THREADS when that is set."""
+    if requested > 0:
+        return requested
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_baseline(soup, threads=0):
+    """The CPU restatement's full training iteration on the host's cores:
+    the C raster / gather / edge / opacity kernels on OpenMP threads, the
+    MLP, blend and MSE in torch CPU fp32, then torch Adam and VectorAdam
+    updates of every parameter from the gradients it produced.  Median of 5
+    iterations after one warm-up, at the 15K-triangle 8 x 800x800 workload
+    (``value``) and at the 1K-triangle 256x256 single-view configuration."""
     import numpy as np
     import torch
 
+    from diffsoup_amd import scenes
     from diffsoup_amd.optimize import VectorAdam
     from oracle import oracle
 
-    torch.set_num_threads(1)
+    n = cpu_threads(threads)
+    torch.set_num_threads(n)
+    oracle.set_threads(n)
     t = lambda a: torch.tensor(np.asarray(a, np.float32))
-    feat, alog, verts = t(soup.feat), t(soup.alpha_logits), t(soup.verts)
-    mlp = [t(x) for wb in soup.mlp for x in wb]
-    opts = [torch.optim.Adam([{"params": [feat], "lr": 5e-2}, {"params": [alog], "lr": 5e-2}], foreach=False),
-            VectorAdam([verts], lr=1e-3), torch.optim.Adam(mlp, lr=1e-2, foreach=False)]
-    n, t0 = 0, time.perf_counter()
-    while n < 1 or (time.perf_counter() - t0 < budget_s and n < 20):
-        r = oracle.reference_step(soup, 2000 + n, dtype=torch.float32)
-        feat.grad, alog.grad, verts.grad = t(r["grad_feat"]), t(r["grad_alpha_logits"]), t(r["grad_verts"])
-        g, o = r["grad_mlp"], 0
-        flat = {}
-        for i, (w, b) in enumerate(soup.mlp):
-            flat[2 * i] = (o, w.shape)
-            o += w.size
-        for i, (w, b) in enumerate(soup.mlp):
-            flat[2 * i + 1] = (o, b.shape)
-            o += b.size
-        for k, p in enumerate(mlp):
-            off, shape = flat[k]
-            p.grad = t(g[off:off + int(np.prod(shape))].reshape(shape))
-        for opt in opts:
-            opt.step()
-        n += 1
-    dt = (time.perf_counter() - t0) / n
-    return {"value": round(1.0 / dt, 4), "unit": "it/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full iterations of the same workload (single-threaded C oracle forward + backward, "
-                      f"torch CPU fp32 MLP, torch Adam + VectorAdam updates)"}
+
+    def its(s, reps=5):
+        feat, alog, verts = t(s.feat), t(s.alpha_logits), t(s.verts)
+        mlp = [t(x) for wb in s.mlp for x in wb]
+        opts = [torch.optim.Adam([{"params": [feat], "lr": 5e-2}, {"params": [alog], "lr": 5e-2}]),
+                VectorAdam([verts], lr=1e-3), torch.optim.Adam(mlp, lr=1e-2)]
+        order = [0, 2, 4, 1, 3, 5]   # grad_mlp holds dW1, dW2, dW3, db1, db2, db3
+
+        def one(k):
+            r = oracle.reference_step(s, 2000 + k, dtype=torch.float32)
+            feat.grad, alog.grad, verts.grad = t(r["grad_feat"]), t(r["grad_alpha_logits"]), t(r["grad_verts"])
+            g, o = r["grad_mlp"], 0
+            for slot in order:
+                size = mlp[slot].numel()
+                mlp[slot].grad = t(g[o:o + size].reshape(mlp[slot].shape))
+                o += size
+            for opt in opts:
+                opt.step()
+
+        one(0)
+        ts = []
+        for k in range(reps):
+            t0 = time.perf_counter()
+            one(1 + k)
+            ts.append(time.perf_counter() - t0)
+        return 1.0 / float(np.median(ts))
+
+    try:
+        v2 = its(soup)
+        v1 = its(scenes.small_random_soup(0))
+    finally:
+        oracle.set_threads(1)
+    return {"value": round(v2, 4), "unit": "it/s", "cores": n, "kind": "port",
+            "sample": "median of 5 full iterations after 1 warm-up of the same 15K-triangle 8 x 800x800 workload: "
+                      "the C restatement's raster, gather, scatter, edge and opacity kernels on OpenMP threads, "
+                      "torch CPU fp32 MLP and MSE, torch Adam + VectorAdam updates",
+            "config1_1k_tris_256x256_it_s": round(v1, 3)}
 
 
 if __name__ == "__main__":

@@ -414,10 +414,23 @@ k_shade(ShadeArgs a, int rowp) {
                     const int lv = f0 / C, c = f0 % C;
                     const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
                     crs[tt] = cr;
-                    const FT* fl = frow + loff[lv] * C + c;
-                    const float4 v0 = ld4(fl + cr.idx[0] * C);
-                    const float4 v1 = ld4(fl + cr.idx[1] * C);
-                    const float4 v2 = ld4(fl + cr.idx[2] * C);
+                    float4 v0, v1, v2;
+                    if (FB && f0 == 0) {
+                        // channels 0-3 of the first level feed the residual
+                        // blend, i.e. the output colour directly: with bf16
+                        // rows they are read from the fp32 master (three
+                        // 16-byte loads per pixel, cache-resident within a
+                        // triangle), so bf16 storage does not reach the colour
+                        const float* fg = a.feat + (long long)t * row;
+                        v0 = *reinterpret_cast<const float4*>(fg + cr.idx[0] * C);
+                        v1 = *reinterpret_cast<const float4*>(fg + cr.idx[1] * C);
+                        v2 = *reinterpret_cast<const float4*>(fg + cr.idx[2] * C);
+                    } else {
+                        const FT* fl = frow + loff[lv] * C + c;
+                        v0 = ld4(fl + cr.idx[0] * C);
+                        v1 = ld4(fl + cr.idx[1] * C);
+                        v2 = ld4(fl + cr.idx[2] * C);
+                    }
                     float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
                     e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
                     e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;

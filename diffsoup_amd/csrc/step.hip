@@ -270,6 +270,10 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
                          d->pix_loss, d->grad_verts, d->grad_alpha_logits, d->grad_feat, d->grad_mlp, d->loss};
     for (const void* p : req)
         if (!p && d->T > 0) return step_fail(DS_ERR_INVALID, "null pointer in step descriptor");
+    if (d->feat_bf16 && d->bf16 && d->T > 0 &&
+        (!d->feat || (reinterpret_cast<uintptr_t>(d->feat) & 15u)))
+        return step_fail(DS_ERR_INVALID, "bf16 feature rows also need the 16-byte aligned fp32 master feat "
+                                         "(the residual-blend channels are read from it)");
     if ((reinterpret_cast<uintptr_t>(d->pos) | reinterpret_cast<uintptr_t>(d->rast) |
          reinterpret_cast<uintptr_t>(feat_in) | reinterpret_cast<uintptr_t>(d->grad_feat)) & 15u)
         return step_fail(DS_ERR_INVALID, "pos, rast, feat (or feat_bf16) and grad_feat must be 16-byte aligned");

@@ -143,6 +143,7 @@ class FusedStep:
         self._ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.dev)
         self.desc.workspace = (self._ws.data_ptr() + 255) & ~255
         self.desc.workspace_bytes = nbytes
+        self._captured = False   # set by CapturedStep: workspace and tile are then fixed
 
     def _fill_static(self):
         d = self.desc
@@ -208,11 +209,17 @@ class FusedStep:
 
     def _ensure_workspace(self, nbytes: int) -> None:
         if nbytes > self.desc.workspace_bytes:
+            if self._captured:
+                raise _lib.DiffSoupError(
+                    "a CapturedStep replays this step's workspace and tile: the workspace cannot grow after "
+                    "capture (choose the tile before capturing, or build a new FusedStep)")
             self._ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.dev)
             self.desc.workspace = (self._ws.data_ptr() + 255) & ~255
             self.desc.workspace_bytes = nbytes
 
     def set_tile(self, tile: int) -> None:
+        if self._captured and int(tile) != self.tile:
+            raise _lib.DiffSoupError("the tile of a captured step is fixed: the graph replays the tile it recorded")
         n = self.workspace_bytes(tile)
         if n == 0:
             raise _lib.DiffSoupError(f"tile {tile}: " + _lib.lib().ds_last_error().decode())
@@ -252,6 +259,8 @@ class FusedStep:
                     table = json.load(fh)
             except (OSError, ValueError):
                 table = {}
+        if self._captured:
+            raise _lib.DiffSoupError("autotune on a captured step: tune before capturing")
         if key in table and int(table[key]) in tiles:
             self.set_tile(int(table[key]))
             return {int(table[key]): None}
@@ -298,9 +307,10 @@ class FusedStep:
         shade: per covered pixel 60 B (rast, index, target, colour, colour
         gradient, L1) plus one read of the feature table (fp32 or bf16) and
         one write of its fp32 gradient.
-        edge: per covered pixel its list record (16 B) and the rast, colour
-        and colour gradient (40 B) of itself and its 4 neighbours, plus the
-        clip-position gradient (16 B per view-vertex).
+        edge: per covered pixel its list record (16 B) and its rast, colour
+        and colour gradient (40 B), each read once (a neighbour's values are
+        another pixel's own reads), plus the clip-position gradient (16 B
+        per view-vertex).
         """
         P = self.P
         cov = self.covered_pixels()
@@ -312,7 +322,7 @@ class FusedStep:
         if stage == "shade":
             return 60.0 * cov + feat_read + 4.0 * n_feat
         if stage == "edge":
-            return (16.0 + 5 * 40.0) * cov + 16.0 * self.B * self.V
+            return (16.0 + 40.0) * cov + 16.0 * self.B * self.V
         if stage == "aux":
             return 16.0 * P + 4.0 * P + 64.0 * self.B * self.T + 4.0 * pairs + 8.0 * self.T * self.Sa
         raise ValueError(stage)
@@ -353,35 +363,6 @@ class FusedStep:
         return self._ws[base:base + 64].view(torch.int64)
 
 
-def smoke_step(dev) -> None:
-    """One small fused forward + backward on ``dev``, checked against the
-    oracle (raster, bit-exact) and a float64 torch evaluation of the MLP."""
-    from oracle import oracle
-
-    from . import scenes
-
-    s = scenes.small_random_soup(0)
-    prm = SoupParams.from_soup(s, dev)
-    st = FusedStep(s.resolutions, s.T, s.verts.shape[0], s.alpha_level, s.levels, 8, 16, device=dev)
-    mvp = torch.from_numpy(s.mvp).to(dev)
-    target = torch.from_numpy(s.target).to(dev)
-    print("smoke: fused fwd+bwd on the 1k-triangle soup", flush=True)
-    loss = st.run(prm, mvp, target, seed=3)
-    torch.cuda.synchronize()
-    pos = st.pos.cpu().numpy()
-    alpha = st.alpha.cpu().numpy()
-    ref = oracle.rasterize(s.resolutions[0], pos, s.tri, s.alpha_level, alpha, True, 3)
-    if not np.array_equal(st.rast.cpu().numpy().view(np.uint32), ref.reshape(-1, 4).view(np.uint32)):
-        raise AssertionError("smoke: fused-step raster differs from the oracle")
-    x = oracle.multires_color(ref, s.levels[0], s.levels[1], s.feat, True).reshape(-1, 8)
-    covered = ref.reshape(-1, 4)[:, 3] > 0
-    color64, loss64 = oracle.shade_reference(x, covered, s.mlp, s.target)
-    err = np.abs(st.color.cpu().numpy() - color64).max()
-    if err > 1e-5 or abs(float(loss.item()) - loss64) > 1e-5 * max(1.0, loss64):
-        raise AssertionError(f"smoke: fused colour/loss mismatch (max err {err})")
-    print(f"smoke: fused step ok (loss {loss.item():.6f}, colour max err vs fp64 {err:.2e})", flush=True)
-
-
 def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5e-2, lr_verts=1e-3,
                    lr_mlp=1e-2, out: SoupParams | None = None):
     """The reference's optimisers for a soup (examples/02_synthetic.py:310-315):
@@ -413,7 +394,9 @@ class CapturedStep:
 
     The graph owns the kernel arguments of the capture: the tensors in
     ``params``, ``mvp``, ``target`` and the step's buffers must stay alive
-    and in place (update them with copy_, not by rebinding).
+    and in place (update them with copy_, not by rebinding).  The capture
+    holds the step's workspace and fixes its tile: set_tile and autotune on
+    the step raise afterwards.
     """
 
     def __init__(self, step: FusedStep, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor,
@@ -428,6 +411,8 @@ class CapturedStep:
         step.desc.seed_device = self.seed_dev.data_ptr()
         step.desc.advance_seed = 1
         self.graph = torch.cuda.CUDAGraph()
+        self._workspace = step._ws   # the replays write into this allocation
+        step._captured = True
         try:
             with torch.cuda.graph(self.graph):
                 step.run(params, mvp, target, 0)
@@ -439,7 +424,10 @@ class CapturedStep:
         self.set_seed(seed)
 
     def set_seed(self, seed: int) -> None:
-        self.seed_dev.fill_(int(seed) & (2**63 - 1))
+        """Seed of the next replay: all 64 bits, as FusedStep.run uses them
+        (stored as the two's-complement int64 of the unsigned seed)."""
+        v = int(seed) & (2**64 - 1)
+        self.seed_dev.fill_(v - 2**64 if v >= 2**63 else v)
 
     def replay(self) -> torch.Tensor:
         self.graph.replay()

@@ -205,7 +205,11 @@ typedef struct ds_step_desc {
     const uint16_t* feat_bf16;   /* if non-null, the features are read from this
                                     bf16 copy [T, S_tot, channels] (round to
                                     nearest even of the fp32 master) instead of
-                                    feat; grad_feat stays fp32 */
+                                    feat; grad_feat stays fp32.  With the bf16
+                                    MLP, channels 0-3 of the first level (the
+                                    residual-blend inputs, which reach the
+                                    colour directly) are still read from feat,
+                                    which must then be given too */
     int loss_kind;               /* DS_LOSS_MSE (0): mean squared error; or
                                     DS_LOSS_L1_SSIM: the reference's
                                     0.8 L1 + 0.2 (1 - SSIM) / 2 (examples/

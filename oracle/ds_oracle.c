@@ -23,8 +23,41 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #define DSO_MIN(a, b) ((a) < (b) ? (a) : (b))
 #define DSO_MAX(a, b) ((a) > (b) ? (a) : (b))
+
+/* ------------------------------------------------------------------------ */
+/* Threads.  One by default: every loop then runs in the order written and  */
+/* every result is deterministic (the parity tests use this).  With more,   */
+/* the per-pixel and per-fragment loops are spread over OpenMP threads for  */
+/* the timed CPU baseline: the fragment list keeps its canonical order, the */
+/* z-buffer takes an atomic minimum (order-independent), and scatters into  */
+/* shared gradients become atomic adds, whose float sums then depend on the */
+/* interleaving.                                                            */
+/* ------------------------------------------------------------------------ */
+static int dso_nthreads = 1;
+
+void dso_set_threads(int n) {
+    dso_nthreads = n < 1 ? 1 : n;
+#ifdef _OPENMP
+    omp_set_num_threads(dso_nthreads);
+#endif
+}
+
+int dso_get_threads(void) { return dso_nthreads; }
+
+static inline void dso_addf(float* p, float v) {
+    if (dso_nthreads > 1) {
+#pragma omp atomic
+        *p += v;
+    } else {
+        *p += v;
+    }
+}
 
 /* ------------------------------------------------------------------------ */
 /* Stochastic-mask hash: SplitMix64 output function (Steele, Lea & Flood,   */
@@ -151,8 +184,72 @@ static float dso_pixel_ndc(int i, int n) { return -1.f + 2.f * ((float)i + 0.5f)
 /* frag_pix [N,3] = (view, y, x); frag_attrs [N,4] = (b0, b1, z/w, t+1).   */
 /* Output order: view, triangle, row-major position in the rectangle.       */
 /* ------------------------------------------------------------------------ */
+/* One candidate: 1 and its fragment when the pixel centre is inside the
+ * triangle with depth in [-1, 1] (src/cuda/rasterize.cu:245-277). */
+static int dso_candidate(int H, int W, int b, const float* p0, const float* p1, const float* p2, int t, int y,
+                         int x, int32_t* fp, float* fa) {
+    if (x < 0 || x >= W || y < 0 || y >= H) return 0;
+    const float ny = dso_pixel_ndc(y, H), nx = dso_pixel_ndc(x, W);
+    float b0, b1, cz, cw;
+    if (!dso_intersect(ny, nx, p0, p1, p2, &b0, &b1, &cz, &cw)) return 0;
+    const float depth = cz / cw;
+    if (!(depth >= -1.f && depth <= 1.f)) return 0;
+    fp[0] = b;
+    fp[1] = y;
+    fp[2] = x;
+    fa[0] = b0;
+    fa[1] = b1;
+    fa[2] = depth;
+    fa[3] = (float)(t + 1);
+    return 1;
+}
+
+/* Threaded form: every (view, triangle) fills its own slice of a candidate
+ * buffer (src/cuda/rasterize.cu:208-277, one slot per rectangle pixel), then
+ * one ordered pass compacts the valid ones (:402-456 with a fixed order). */
+static int64_t dso_fragments_threaded(int H, int W, int B, int V, const float* pos, int T, const int32_t* tri,
+                                      const int32_t* rects, int32_t* frag_pix, float* frag_attrs) {
+    const int64_t BT = (int64_t)B * T;
+    int64_t* start = (int64_t*)malloc(sizeof(int64_t) * (size_t)(BT + 1));
+    start[0] = 0;
+    for (int64_t i = 0; i < BT; ++i) {
+        const int32_t* r = rects + 4 * i;
+        start[i + 1] = start[i] + ((r[1] > 0 && r[3] > 0) ? (int64_t)r[1] * r[3] : 0);
+    }
+    const int64_t total = start[BT];
+    int32_t* cfp = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(total > 0 ? total : 1));
+    float* cfa = (float*)malloc(sizeof(float) * 4 * (size_t)(total > 0 ? total : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < BT; ++i) {
+        const int b = (int)(i / T), t = (int)(i % T);
+        const int32_t* r = rects + 4 * i;
+        const float* bp = pos + (int64_t)b * V * 4;
+        const float* p0 = bp + 4 * (int64_t)tri[3 * t + 0];
+        const float* p1 = bp + 4 * (int64_t)tri[3 * t + 1];
+        const float* p2 = bp + 4 * (int64_t)tri[3 * t + 2];
+        for (int64_t k = 0; k < start[i + 1] - start[i]; ++k) {
+            const int64_t c = start[i] + k;
+            cfp[3 * c] = -1;
+            dso_candidate(H, W, b, p0, p1, p2, t, r[0] + (int)(k / r[3]), r[2] + (int)(k % r[3]), cfp + 3 * c,
+                          cfa + 4 * c);
+        }
+    }
+    int64_t n = 0;
+    for (int64_t c = 0; c < total; ++c) {
+        if (cfp[3 * c] < 0) continue;
+        memcpy(frag_pix + 3 * n, cfp + 3 * c, 3 * sizeof(int32_t));
+        memcpy(frag_attrs + 4 * n, cfa + 4 * c, 4 * sizeof(float));
+        ++n;
+    }
+    free(cfa);
+    free(cfp);
+    free(start);
+    return n;
+}
+
 int64_t dso_fragments(int H, int W, int B, int V, const float* pos, int T, const int32_t* tri,
                       const int32_t* rects, int32_t* frag_pix, float* frag_attrs) {
+    if (dso_nthreads > 1) return dso_fragments_threaded(H, W, B, V, pos, T, tri, rects, frag_pix, frag_attrs);
     int64_t n = 0;
     for (int b = 0; b < B; ++b) {
         const float* bp = pos + (int64_t)b * V * 4;
@@ -235,7 +332,7 @@ static void dso_interp_bwd(float b0, float b1, int lo, int hi, int C, float* gra
         dso_cell(b0, b1, (uint32_t)l, idx, wt);
         float* d = grad + (int64_t)off * C;
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < C; ++j) d[(int64_t)idx[i] * C + j] += g[j] * wt[i];
+            for (int j = 0; j < C; ++j) dso_addf(&d[(int64_t)idx[i] * C + j], g[j] * wt[i]);
         off += dso_feats_at_level((uint32_t)l);
     }
 }
@@ -243,6 +340,7 @@ static void dso_interp_bwd(float b0, float b1, int lo, int hi, int C, float* gra
 /* Per-fragment opacity: src/cuda/multires.cu:130-156. */
 void dso_multires_alpha(int64_t N, const float* frag_attrs, int level, int S, const float* alpha_src,
                         float* frag_alpha) {
+#pragma omp parallel for schedule(static) if (dso_nthreads > 1)
     for (int64_t i = 0; i < N; ++i) {
         const int t = (int)frag_attrs[4 * i + 3] - 1;
         frag_alpha[i] = 0.f;
@@ -256,6 +354,7 @@ void dso_multires_alpha(int64_t N, const float* frag_attrs, int level, int S, co
  * (python/diffsoup/multires.py:97-100 with the hash in place of torch.rand). */
 void dso_mask_thresholds(int64_t N, const int32_t* frag_pix, const float* frag_attrs, int stochastic,
                          uint64_t seed, float* thresh) {
+#pragma omp parallel for schedule(static) if (dso_nthreads > 1)
     for (int64_t i = 0; i < N; ++i) {
         thresh[i] = stochastic ? dso_mask_threshold(seed, (uint32_t)frag_pix[3 * i],
                                                     (uint32_t)((int)frag_attrs[4 * i + 3] - 1),
@@ -273,6 +372,7 @@ void dso_depth_test(int B, int H, int W, int64_t N, const int32_t* frag_pix, con
     const int64_t P = (int64_t)B * H * W;
     int64_t* zbuf = (int64_t*)malloc(sizeof(int64_t) * (size_t)(P > 0 ? P : 1));
     for (int64_t p = 0; p < P; ++p) zbuf[p] = INT64_MAX;
+#pragma omp parallel for schedule(static) if (dso_nthreads > 1)
     for (int64_t i = 0; i < N; ++i) {
         if (frag_pix[3 * i] < 0) continue;
         const int64_t pix = ((int64_t)frag_pix[3 * i] * H + frag_pix[3 * i + 1]) * W + frag_pix[3 * i + 2];
@@ -283,7 +383,14 @@ void dso_depth_test(int B, int H, int W, int64_t N, const int32_t* frag_pix, con
         uint32_t bits;
         memcpy(&bits, &z2, 4);
         const int64_t key = ((int64_t)(int32_t)bits << 32) | (int64_t)(uint32_t)i;
-        if (key < zbuf[pix]) zbuf[pix] = key;
+        if (dso_nthreads > 1) {
+            int64_t cur = __atomic_load_n(&zbuf[pix], __ATOMIC_RELAXED);
+            while (key < cur &&
+                   !__atomic_compare_exchange_n(&zbuf[pix], &cur, key, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+            }
+        } else if (key < zbuf[pix]) {
+            zbuf[pix] = key;
+        }
     }
     for (int64_t p = 0; p < P; ++p) {
         if (zbuf[p] == INT64_MAX) continue;
@@ -303,6 +410,7 @@ void dso_multires_color(int64_t P, const float* rast, int lo, int hi, int C, con
     const int width = concat ? nl * C : C;
     uint32_t S = 0;
     for (int l = lo; l <= hi; ++l) S += dso_feats_at_level((uint32_t)l);
+#pragma omp parallel for schedule(static) if (dso_nthreads > 1)
     for (int64_t p = 0; p < P; ++p) {
         float* o = out + p * width;
         for (int j = 0; j < width; ++j) o[j] = 0.f;
@@ -327,6 +435,7 @@ void dso_multires_color_bwd(int64_t P, const float* rast, int lo, int hi, int C,
     const int width = concat ? nl * C : C;
     uint32_t S = 0;
     for (int l = lo; l <= hi; ++l) S += dso_feats_at_level((uint32_t)l);
+#pragma omp parallel for schedule(static) if (dso_nthreads > 1)
     for (int64_t p = 0; p < P; ++p) {
         const int t = (int)rast[4 * p + 3] - 1;
         if (t < 0) continue;
@@ -524,9 +633,10 @@ static void dso_mid(float g[12], const float* P[3], float mx, float my, float gx
 void dso_edge_grad(int B, int H, int W, int C, const float* color, const float* grad_color,
                    const float* rast, int V, const float* pos, float* grad_pos, int T, const int32_t* tri) {
     static const int DX[4] = {1, -1, 0, 0}, DY[4] = {0, 0, -1, 1};
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) if (dso_nthreads > 1)
     for (int b = 0; b < B; ++b) {
-        const float* bp = pos + (int64_t)b * V * 4;
-        for (int y0 = 0; y0 < H; ++y0)
+        for (int y0 = 0; y0 < H; ++y0) {
+            const float* bp = pos + (int64_t)b * V * 4;
             for (int x0 = 0; x0 < W; ++x0) {
                 const int64_t p0 = ((int64_t)b * H + y0) * W + x0;
                 const int it0 = (int)rast[4 * p0 + 3] - 1;
@@ -607,8 +717,9 @@ void dso_edge_grad(int B, int H, int W, int C, const float* color, const float* 
                 }
                 float* gp = grad_pos + (int64_t)b * V * 4;
                 for (int k = 0; k < 3; ++k)
-                    for (int j = 0; j < 4; ++j) gp[4 * (int64_t)v0[k] + j] += g[4 * k + j];
+                    for (int j = 0; j < 4; ++j) dso_addf(&gp[4 * (int64_t)v0[k] + j], g[4 * k + j]);
             }
+        }
     }
 }
 
@@ -621,6 +732,7 @@ void dso_opacity_aux(int B, int H, int W, int C, const float* color, const float
                      const float* frag_alpha, int level, int S, float* grad_alpha) {
     (void)B;
     const float eps = 1e-5f;
+#pragma omp parallel for schedule(static) if (dso_nthreads > 1)
     for (int64_t i = 0; i < N; ++i) {
         if (frag_pix[3 * i] < 0) continue;
         const int64_t p = ((int64_t)frag_pix[3 * i] * H + frag_pix[3 * i + 1]) * W + frag_pix[3 * i + 2];
@@ -639,6 +751,6 @@ void dso_opacity_aux(int B, int H, int W, int C, const float* color, const float
         float wt[3];
         dso_cell(frag_attrs[4 * i], frag_attrs[4 * i + 1], (uint32_t)level, idx, wt);
         float* d = grad_alpha + (int64_t)tf * S;
-        for (int k = 0; k < 3; ++k) d[idx[k]] += g * wt[k];
+        for (int k = 0; k < 3; ++k) dso_addf(&d[idx[k]], g * wt[k]);
     }
 }

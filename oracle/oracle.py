@@ -137,8 +137,21 @@ def lib():
         L.dso_edge_grad.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p, _f32p, _c.c_int, _f32p, _f32p, _c.c_int, _i32p]
         L.dso_opacity_aux.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _f32p, _f32p, _f32p, _c.c_int64,
                                       _i32p, _f32p, _f32p, _c.c_int, _c.c_int, _f32p]
+        L.dso_set_threads.argtypes = [_c.c_int]
+        L.dso_get_threads.restype = _c.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the C restatement (1, the default, keeps every loop
+    in its written order and every result deterministic; more is for the
+    timed CPU baseline, where scatters become atomic adds)."""
+    lib().dso_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().dso_get_threads())
 
 
 def _f32(a):
@@ -379,10 +392,11 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
                                      view_base=b).reshape(-1, 4) for b, (H, W) in enumerate(res)])
     lo, hi = s.levels
     P = rast.shape[0]
-    X = multires_color(rast, lo, hi, s.feat, True).reshape(P, -1)
     covered = rast[:, 3] > 0
+    rast_cov = np.ascontiguousarray(rast[covered])   # background gathers are zero and carry no gradient
+    X = multires_color(rast_cov, lo, hi, s.feat, True).reshape(rast_cov.shape[0], -1)
     idx = torch.from_numpy(np.nonzero(covered)[0])
-    Xt = torch.tensor(X[covered], dtype=dtype, requires_grad=True)
+    Xt = torch.tensor(X, dtype=dtype, requires_grad=True)
     Ws = [(torch.tensor(w, dtype=dtype, requires_grad=True), torch.tensor(b, dtype=dtype, requires_grad=True))
           for w, b in s.mlp]
     h = Xt
@@ -391,7 +405,7 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
         h = torch.relu(h) if i < len(Ws) - 1 else torch.sigmoid(h)
     r3 = Xt[:, 3:4]
     blend = (1 - r3) * Xt[:, :3] + r3 * h
-    tgt = torch.tensor(s.target, dtype=dtype)
+    tgt = torch.from_numpy(np.ascontiguousarray(s.target)).to(dtype)
     color = torch.ones(P, 3, dtype=dtype).index_put((idx,), blend)
     gcol = None
     if loss == "mse":
@@ -403,9 +417,7 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
         loss = 0.8 * (tgt - color).abs().mean() + 0.2 * 0.5 * (1 - torch.stack(views).mean())
         gcol = torch.autograd.grad(loss, color, retain_graph=True)[0]
     loss.backward()
-    dX = np.zeros(X.shape, np.float32)
-    dX[covered] = Xt.grad.numpy()
-    grad_feat = multires_color_bwd(rast, lo, hi, s.feat.shape, dX, True)
+    grad_feat = multires_color_bwd(rast_cov, lo, hi, s.feat.shape, Xt.grad.numpy().astype(np.float32), True)
     color32 = color.detach().numpy().astype(np.float32)
     if gcol is not None:
         grad_color32 = gcol.numpy().astype(np.float32)

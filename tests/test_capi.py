@@ -84,3 +84,35 @@ def test_core_dropin_surface_and_type_checks(L):
     from diffsoup_amd._lib import DiffSoupError
     with pytest.raises(DiffSoupError, match="GPU tensor"):
         _core.encode_view_dir_sh2(torch.zeros(1, 2, 2, 4), torch.zeros(1, 4, 4), torch.zeros(1, 2, 2, 9))
+
+
+def test_device_code_has_no_packed_fp32(tmp_path):
+    """The library's hipcc flags keep packed fp32 (v_pk_fma_f32 and friends)
+    out of the device code: a probe whose float2 arithmetic hipcc -O3 packs
+    without the flag compiles to unpacked instructions with them."""
+    import shutil
+    import subprocess
+
+    from diffsoup_amd import build_lib
+
+    if not os.path.exists("/opt/rocm/bin/hipcc") and not shutil.which("hipcc"):
+        pytest.skip("hipcc not present")
+    src = tmp_path / "probe.hip"
+    src.write_text("#include <hip/hip_runtime.h>\n"
+                   "__global__ void k(float2* a, const float2* b, float s) {\n"
+                   "  int i = threadIdx.x; float2 x = a[i], y = b[i];\n"
+                   "  x.x = x.x * s + y.x; x.y = x.y * s + y.y; a[i] = x; }\n")
+    packed = re.compile(r"v_pk_(fma|mul|add)_f32")
+
+    def device_asm(flags, name):
+        out = tmp_path / name
+        flags = [f for f in flags if f not in ("-shared", "-fPIC")]
+        subprocess.run([build_lib.hipcc()] + flags + ["--cuda-device-only", "-S", str(src), "-o", str(out)],
+                       check=True, capture_output=True)
+        return out.read_text()
+
+    with_flag = device_asm(build_lib.HIPCC_FLAGS, "with.s")
+    assert not packed.search(with_flag)
+    i = build_lib.HIPCC_FLAGS.index("-packed-fp32-ops")
+    without = build_lib.HIPCC_FLAGS[:i - 3] + build_lib.HIPCC_FLAGS[i + 1:]
+    assert packed.search(device_asm(without, "without.s")), "the probe no longer packs: it checks nothing"

@@ -29,6 +29,16 @@ def _bf16_round(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(torch.bfloat16).to(torch.float32).numpy()
 
 
+def _bf16_rows_as_read(feat, lo):
+    """The features as the bf16-row shading kernel reads them: rounded to
+    bf16, except channels 0-3 of the first level's samples (the residual-blend
+    inputs), which it reads from the fp32 master."""
+    out = _bf16_round(feat)
+    s_lo = O.feats_at_level(lo)
+    out[:, :s_lo, :4] = feat[:, :s_lo, :4]
+    return out
+
+
 def _step(dev, s, bf16, seed):
     from diffsoup_amd import fused
     prm = fused.SoupParams.from_soup(s, dev)
@@ -60,7 +70,7 @@ def test_scaled_down_against_oracle(dev, name):
           f"mlp={s0.mlp_dims} bf16={bf16}", flush=True)
     st, loss = _step(dev, s0, bf16, seed=7)
     pos, alpha = st.pos.cpu().numpy(), st.alpha.cpu().numpy()
-    s = dataclasses.replace(s0, feat=_bf16_round(s0.feat)) if bf16 else s0
+    s = dataclasses.replace(s0, feat=_bf16_rows_as_read(s0.feat, s0.levels[0])) if bf16 else s0
     ref = O.reference_step(s, 7, pos=pos, alpha=alpha, color_in=st.color.cpu().numpy(),
                            grad_color_in=st.grad_color.cpu().numpy())
     np.testing.assert_array_equal(st.rast.cpu().numpy().view(np.uint32), ref["rast"].view(np.uint32))
@@ -87,6 +97,10 @@ def _view(s, b):
 
 @pytest.mark.parametrize("name", ["uneven", "bf16_large"])
 def test_full_size_views(dev, name):
+    """Every view of the full-size scene bit-exact with the oracle, and view 0's
+    colour against float64 (bf16: on the features as the kernel reads them;
+    the end-to-end error against the unrounded model is in
+    test_gpu_precision.py)."""
     make = {"uneven": lambda: scenes.batched_uneven_soup(0), "bf16_large": lambda: scenes.bf16_large_soup(0)}[name]
     bf16 = name == "bf16_large"
     s0 = make()
@@ -103,8 +117,46 @@ def test_full_size_views(dev, name):
         np.testing.assert_array_equal(rast[off[b]:off[b + 1]].view(np.uint32), ref.reshape(-1, 4).view(np.uint32),
                                       err_msg=f"view {b}")
     # colour of the first view against float64 (the step's view 0 draws the mask of view 0)
-    s = dataclasses.replace(s0, feat=_bf16_round(s0.feat)) if bf16 else s0
+    s = dataclasses.replace(s0, feat=_bf16_rows_as_read(s0.feat, s0.levels[0])) if bf16 else s0
     ref = O.reference_step(_view(s, 0), 11, pos=pos[:1], alpha=alpha)
     err = np.abs(color[:off[1]] - ref["color"]).max()
     print(f"  view 0 colour max abs err vs fp64: {err:.3e}", flush=True)
     assert err <= (1e-3 if bf16 else 1e-5)
+
+
+def test_stress_full_size_captured(dev):
+    """The 300K-triangle stress scene at full size (64 views up to 1920x1080,
+    bf16 features and MLP): the autotuned tile, the captured step with the
+    optimiser, two replays.  Loss and every gradient finite, the optimiser
+    moved the parameters, and the raster of four views spread over the
+    batch bit-exact with the oracle under the replay's seed."""
+    from diffsoup_amd import fused
+    s = scenes.stress_soup(0)
+    print(f"stress: T={s.T} views={s.B} pixels={s.pixels}", flush=True)
+    mvp = torch.from_numpy(s.mvp).to(dev)
+    target = torch.from_numpy(s.target).to(dev)
+    st = fused.FusedStep(s.resolutions, s.T, s.verts.shape[0], s.alpha_level, s.levels, s.feat.shape[2],
+                         s.mlp_dims[1], bf16=True, bf16_features=True, device=dev)
+    params = fused.SoupParams.from_soup(s, dev).with_bf16_features()
+    updated = fused.SoupParams.from_soup(s, dev).with_bf16_features()
+    tune = st.autotune(params, mvp, target, tiles=(8, 16, 32), reps=1, cache=None)
+    print(f"  autotune {tune} -> tile {st.tile}", flush=True)
+    opt = fused.soup_optimizer(st, params, out=updated)
+    cap = fused.CapturedStep(st, params, mvp, target, opt, seed=500)
+    for i in range(2):
+        loss = float(cap.replay().item())
+        torch.cuda.synchronize()
+        print(f"  replay {i}: loss {loss:.6f}", flush=True)
+        assert np.isfinite(loss) and loss > 0
+    for name in ("grad_feat", "grad_mlp", "grad_verts", "grad_alpha_logits", "color"):
+        assert bool(torch.isfinite(getattr(st, name)).all()), name
+    assert float((updated.feat - params.feat).abs().max()) > 0
+    assert float((updated.verts - params.verts).abs().max()) > 0
+    pos, alpha, rast = st.pos.cpu().numpy(), st.alpha.cpu().numpy(), st.rast.cpu().numpy()
+    off = np.cumsum([0] + [h * w for h, w in s.resolutions])
+    for b in (0, 21, 42, 63):
+        h, w = s.resolutions[b]
+        ref = O.rasterize((h, w), pos[b:b + 1], s.tri, s.alpha_level, alpha, True, 501, view_base=b)
+        np.testing.assert_array_equal(rast[off[b]:off[b + 1]].view(np.uint32), ref.reshape(-1, 4).view(np.uint32),
+                                      err_msg=f"view {b}")
+        print(f"  view {b} ({h}x{w}) raster bit-exact", flush=True)
