@@ -12,6 +12,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "diffsoup_hip.h"
@@ -159,6 +161,67 @@ int ds_rasterize_fwd(int B, const int* H, const int* W, int V, const float* pos,
     return hip_status(ds::launch_raster_resolve(v, T, a, stochastic ? ds::kMaskHash : ds::kMaskDeterministic,
                                                 ds::MaskSeed{seed, nullptr}, rast, w, s),
                       "raster resolve");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free rasterization: the workspace of ds_rasterize_fwd lives in a
+// file-scope device array of DS_NOALLOC_SCRATCH_BYTES, one copy per GPU held
+// until the process ends.  Host code never uses the array's name as an
+// address: each device's address comes from hipGetSymbolAddress, taken once
+// per device (with that device current) on the first call there.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[DS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_status(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(DS_ERR_INVALID, "device %d beyond %d devices", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return hip_status(e == hipSuccess ? hipErrorInvalidSymbol : e,
+                                                      "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return DS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ds_rasterize_fwd_noalloc(int B, const int* H, const int* W, int V, const float* pos, int T, const int* tri,
+                             int level, const float* alpha_src, int stochastic, uint64_t seed, int tile, float* rast,
+                             void* stream) {
+    ds::Views v;
+    int rc = make_views(B, H, W, tile, v);
+    if (rc != DS_OK) return rc;
+    if (T < 0) return fail(DS_ERR_INVALID, "negative T (%d)", T);
+    // everything but the bins, then as many bin entries as the rest holds
+    const size_t fixed = ds::raster_workspace_bytes(B, T, v.tile_base[B], 1) - 256;
+    if (fixed + 256 > (size_t)DS_NOALLOC_SCRATCH_BYTES)
+        return fail(DS_ERR_INVALID,
+                    "ds_rasterize_fwd_noalloc: %d views x %d triangles need %zu bytes of scratch before any bin "
+                    "entry; the fixed scratch holds %zu (use ds_rasterize_fwd with a workspace)",
+                    B, T, fixed, (size_t)DS_NOALLOC_SCRATCH_BYTES);
+    const int64_t capacity =
+        std::min<int64_t>((int64_t)((DS_NOALLOC_SCRATCH_BYTES - fixed) / sizeof(int)), INT32_MAX);
+    void* ptr = nullptr;
+    rc = scratch_on_current_device(&ptr);
+    if (rc != DS_OK) return rc;
+    return ds_rasterize_fwd(B, H, W, V, pos, T, tri, level, alpha_src, stochastic, seed, tile, rast, ptr,
+                            DS_NOALLOC_SCRATCH_BYTES, capacity, stream);
 }
 
 int ds_opacity_aux_bwd(int B, const int* H, const int* W, int V, const float* pos, int T,

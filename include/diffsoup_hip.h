@@ -85,6 +85,26 @@ int ds_rasterize_fwd(int B, const int* H, const int* W, int V, const float* pos,
                      uint64_t seed, int tile, float* rast, void* workspace,
                      size_t workspace_bytes, int64_t capacity, void* stream);
 
+/* Bytes of the fixed device scratch behind ds_rasterize_fwd_noalloc. */
+#define DS_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* ds_rasterize_fwd without a workspace argument, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The workspace
+ * is a fixed device array of DS_NOALLOC_SCRATCH_BYTES (64 MiB) per GPU,
+ * held until the process ends; the call allocates nothing and takes the
+ * (triangle, tile) bin capacity from what the records and tile tables leave
+ * of it (bins that overflow it still give the exact image through the
+ * slower path).  Batches whose records and tile tables alone exceed it
+ * (views x triangles x 64 B + 12 B per tile) are rejected with
+ * DS_ERR_INVALID.  The scratch is that of the device current at the call;
+ * every call on one GPU shares it, so calls that overlap on one GPU (two
+ * streams, two threads) race.  The stats block it leaves is not readable
+ * through this entry point. */
+int ds_rasterize_fwd_noalloc(int B, const int* H, const int* W, int V, const float* pos, int T,
+                             const int* tri, int level, const float* alpha_src, int stochastic,
+                             uint64_t seed, int tile, float* rast, void* stream);
+
 /* Opacity auxiliary gradient: regenerates every fragment, and for those not
  * strictly behind the visible surface adds the score-function gradient
  * +l/max(a,1e-5) (visible triangle) or -l/max(1-a,1e-5) (others) onto
