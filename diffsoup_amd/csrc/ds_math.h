@@ -186,6 +186,16 @@ __host__ __device__ __forceinline__ Corners lattice_corners(float b0, float b1, 
     return c;
 }
 
+// Decode a lattice index into (x, y) with x + y = n, index = n(n+1)/2 + y
+// (src/cuda/multires.cu:313-325).
+__host__ __device__ __forceinline__ void lattice_xy(uint32_t k, uint32_t& x, uint32_t& y) {
+    const float fi = static_cast<float>(k);
+    const uint32_t n = static_cast<uint32_t>(floorf((sqrtf(8.f * fi + 1.f) - 1.f) * 0.5f));
+    const uint32_t Tn = (n * (n + 1u)) >> 1;
+    y = k - Tn;
+    x = n - y;
+}
+
 // ---------------------------------------------------------------------------
 // Stochastic-mask threshold: a counter-based hash of (seed, view, triangle,
 // row, column) mapped to a 24-bit uniform in [0, 1).  A fragment is kept when
@@ -217,6 +227,47 @@ __host__ __device__ __forceinline__ float mask_threshold_in_view(uint64_t view_h
 __host__ __device__ __forceinline__ float mask_threshold(uint64_t seed, uint32_t view, uint32_t tri,
                                                          uint32_t y, uint32_t x) {
     return mask_threshold_in_view(mask_view_hash(seed, view), tri, y, x);
+}
+
+// ---------------------------------------------------------------------------
+// Degree-2 real spherical-harmonic basis of a pixel's view direction.  The
+// direction points from the pixel ray's far-plane point to its near-plane
+// point (toward the camera), both unprojected with the view's row-major
+// inverse MVP; it is normalised with a correctly rounded 1 / sqrt.
+// Reference: src/cuda/encoding.cu:9-85 (constants and sign pattern :56-85).
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ void sh2_unproject(const float* m, float x, float y, float z, float out[3]) {
+    const float cx = m[0] * x + m[1] * y + m[2] * z + m[3];
+    const float cy = m[4] * x + m[5] * y + m[6] * z + m[7];
+    const float cz = m[8] * x + m[9] * y + m[10] * z + m[11];
+    const float cw = m[12] * x + m[13] * y + m[14] * z + m[15];
+    const float iw = 1.0f / cw;
+    out[0] = cx * iw; out[1] = cy * iw; out[2] = cz * iw;
+}
+
+__host__ __device__ __forceinline__ void sh2_basis(const float* inv_mvp, float ndc_x, float ndc_y, float e[9]) {
+    // 1/(2 sqrt(pi)), sqrt(3)/(2 sqrt(pi)), sqrt(15)/(2 sqrt(pi)), sqrt(5)/(4 sqrt(pi)), sqrt(15)/(4 sqrt(pi))
+    constexpr float kY00 = 0.28209479177387814f;
+    constexpr float kY1 = 0.4886025119029199f;
+    constexpr float kY2a = 1.0925484305920792f;
+    constexpr float kY2b = 0.31539156525252005f;
+    constexpr float kY2c = 0.5462742152960396f;
+    float wn[3], wf[3];
+    sh2_unproject(inv_mvp, ndc_x, ndc_y, -1.0f, wn);
+    sh2_unproject(inv_mvp, ndc_x, ndc_y, 1.0f, wf);
+    const float dx = wn[0] - wf[0], dy = wn[1] - wf[1], dz = wn[2] - wf[2];
+    const float il = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    const float X = dx * il, Y = dy * il, Z = dz * il;
+    const float xx = X * X, yy = Y * Y, zz = Z * Z;
+    e[0] = kY00;
+    e[1] = -kY1 * Y;
+    e[2] = kY1 * Z;
+    e[3] = -kY1 * X;
+    e[4] = kY2a * (X * Y);
+    e[5] = -kY2a * (Y * Z);
+    e[6] = kY2b * (2.0f * zz - xx - yy);
+    e[7] = -kY2a * (X * Z);
+    e[8] = kY2c * (xx - yy);
 }
 
 }  // namespace ds

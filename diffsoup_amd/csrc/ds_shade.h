@@ -35,13 +35,18 @@ struct ShadeArgs {
     // there (and writes no colour, gradient, L1 or loss) and runs the adjoint.
     const float* ext_grad;
     int fwd_only;
+    // SH2 inputs (configurations compiled with them): the views, to find a
+    // covered pixel's view, row and column, and their inverse MVPs [B,4,4].
+    Views views;
+    const float* inv_mvp;
 };
 
-bool shade_supported(int nl, int c, int dh);
-size_t shade_blob_floats(int nl, int c, int dh);
-int shade_nparam(int nl, int c, int dh);
-hipError_t launch_shade_blobs(int nl, int c, int dh, bool bf16, const float* w1, const float* w2,
+bool shade_supported(int nl, int c, int dh, bool sh);
+size_t shade_blob_floats(int nl, int c, int dh, bool sh);
+int shade_nparam(int nl, int c, int dh, bool sh);
+hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const float* w1, const float* w2,
                               const float* w3, float* blobs, hipStream_t s);
-hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s);
+hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
+                        hipStream_t s);
 
 }  // namespace ds

@@ -88,15 +88,6 @@ __global__ void k_color_bwd(long long P, const float4* __restrict__ rast, int lo
     }
 }
 
-// Decode a lattice index into (x, y) with x + y = n, index = n(n+1)/2 + y
-// (src/cuda/multires.cu:313-325).
-__device__ __forceinline__ void lattice_xy(uint32_t k, uint32_t& x, uint32_t& y) {
-    const float fi = static_cast<float>(k);
-    const uint32_t n = static_cast<uint32_t>(floorf((sqrtf(8.f * fi + 1.f) - 1.f) * 0.5f));
-    const uint32_t Tn = (n * (n + 1u)) >> 1;
-    y = k - Tn;
-    x = n - y;
-}
 
 __global__ void k_accum_fwd(int T, int lo, int hi, int target, int S_tot, int S_T, int C,
                             const float* __restrict__ feat, float* __restrict__ out) {

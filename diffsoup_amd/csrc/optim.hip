@@ -33,7 +33,10 @@ struct Group {
     long long units;  // elements (Adam) or vectors (VectorAdam)
     int dim;          // 0: Adam; k > 0: VectorAdam over vectors of k floats
     double lr, b1, b2;     // bias corrections are formed in double
-    float b1f, b2f, w1, w2, eps, lrf;  // fp32 forms: beta, 1 - beta (rounded from double), eps, lr
+    float b1f, b2f, w1, w2, eps;  // fp32 forms: beta, 1 - beta (rounded from double), eps
+    const float* lr_mult;  // optional device multiplier of lr
+    long long decay_steps; // > 0: exponential decay of lr with the iteration
+    double decay_final;
     long long block_base;  // first chunk of this group
 };
 
@@ -86,15 +89,24 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                                                    const long long* __restrict__ step_before) {
     // per group, once per block: the bias corrections (in double, as torch
     // forms beta ** step on the host) and the fp32 factors derived from them
-    __shared__ float coef[kMaxGroups][4];   // lr / bc1, sqrt(bc2), bc1, bc2
+    __shared__ float coef[kMaxGroups][5];   // lr / bc1, sqrt(bc2), bc1, bc2, lr
     if (threadIdx.x < (unsigned)tab.n) {
         const Group& G = tab.grp[threadIdx.x];
         const double t = static_cast<double>(*step_before + 1);
         const double bc1 = 1.0 - pow(G.b1, t), bc2 = 1.0 - pow(G.b2, t);
-        coef[threadIdx.x][0] = static_cast<float>(G.lr / bc1);
+        // the learning rate of this iteration: the schedule's multiplier
+        // final ^ (i / steps), i = t clamped to [1, steps], then the caller's
+        double lr = G.lr;
+        if (G.decay_steps > 0) {
+            const double n = static_cast<double>(G.decay_steps);
+            lr *= pow(G.decay_final, fmin(fmax(t, 1.0), n) / n);
+        }
+        if (G.lr_mult) lr *= static_cast<double>(*G.lr_mult);
+        coef[threadIdx.x][0] = static_cast<float>(lr / bc1);
         coef[threadIdx.x][1] = static_cast<float>(sqrt(bc2));
         coef[threadIdx.x][2] = static_cast<float>(bc1);
         coef[threadIdx.x][3] = static_cast<float>(bc2);
+        coef[threadIdx.x][4] = static_cast<float>(lr);
     }
     __syncthreads();
     for (long long c = blockIdx.x; c < n_chunks; c += gridDim.x) {
@@ -162,7 +174,7 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
         } else {
             // VectorAdam: g1 = b1 g1 + (1 - b1) g; g2 = b2 g2 + (1 - b2) |g|^2;
             // p -= lr * (g1 / bc1) / (sqrt(g2 / bc2) + 1e-8)
-            const float ibc1 = coef[gi][2], ibc2 = coef[gi][3];
+            const float ibc1 = coef[gi][2], ibc2 = coef[gi][3], lrf = coef[gi][4];
             const int k = G.dim;
             for (long long u = u0 + threadIdx.x; u < u1; u += kThreads) {
                 const long long base = u * k;
@@ -178,7 +190,7 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                     const float g = G.g[base + j];
                     const float g1 = G.m[base + j] * G.b1f + G.w1 * g;
                     G.m[base + j] = g1;
-                    const float o = G.p[base + j] - G.lrf * ((g1 / ibc1) / den);
+                    const float o = G.p[base + j] - lrf * ((g1 / ibc1) / den);
                     G.out[base + j] = o;
                     if (G.out16) G.out16[base + j] = static_cast<unsigned short>(bf16_bits(o));
                 }
@@ -227,7 +239,11 @@ extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* 
         G.w1 = static_cast<float>(1.0 - s.beta1);
         G.w2 = static_cast<float>(1.0 - s.beta2);
         G.eps = static_cast<float>(s.eps);
-        G.lrf = static_cast<float>(s.lr);
+        G.lr_mult = s.lr_mult;
+        G.decay_steps = s.decay_steps;
+        G.decay_final = s.decay_final;
+        if (s.decay_steps > 0 && !(s.decay_final > 0.0))
+            return optim_fail(DS_ERR_INVALID, "adam: decay_final must be > 0 with decay_steps");
         G.block_base = blocks;
         blocks += (G.units + kChunk - 1) / kChunk;
     }

@@ -32,6 +32,7 @@
 #include "ds_math.h"
 #include "ds_shade.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace ds {
@@ -62,12 +63,19 @@ __device__ __forceinline__ float row16_sum(float x) {
 }
 
 
-// Compile-time shape of one shading configuration.
-template <int NL_, int C_, int DH_, bool BF16_>
+// Compile-time shape of one shading configuration: NL levels of C channels
+// (concatenated), optionally followed by the 9 SH2 values of the pixel's
+// view direction (SH), into an MLP of hidden width DH.  In LDS a sample's
+// channels are padded to CP, a multiple of 4, so every 4-channel block is one
+// 16-byte access; a channel count that is not a multiple of 4 needs a single
+// level (the reference's accumulated features: 7 channels + SH2 = 16 inputs).
+template <int NL_, int C_, int DH_, bool BF16_, bool SH_ = false>
 struct Cfg {
     static constexpr int NL = NL_, C = C_, DH = DH_;
-    static constexpr bool BF16 = BF16_;
-    static constexpr int DIN = NL * C;
+    static constexpr bool BF16 = BF16_, SH = SH_;
+    static constexpr int CP = (C + 3) & ~3;
+    static constexpr int DF = NL * C;            // feature inputs
+    static constexpr int DIN = DF + (SH ? 9 : 0);
     static constexpr int NI = (DIN + 15) / 16;  // input tiles
     static constexpr int NH = DH / 16;          // hidden tiles
     static constexpr int DINP = 16 * NI;
@@ -95,7 +103,7 @@ struct Cfg {
     static constexpr int STAGE_ = region(DH) + region(DINP) > 2 * region(DH) ? region(DH) + region(DINP)
                                                                               : 2 * region(DH);
     static constexpr int STAGE = STAGE_ > region(16) + region(DH) ? STAGE_ : region(16) + region(DH);
-    static_assert(C % 4 == 0, "channels per level must be a multiple of 4");
+    static_assert(C % 4 == 0 || NL == 1, "channels per level must be a multiple of 4 with several levels");
     static_assert(DH % 16 == 0, "hidden width must be a multiple of 16");
 };
 
@@ -126,7 +134,7 @@ struct Plan {
     __host__ __device__ static constexpr int base_bytes(int rowp) {
         return frow_bytes(rowp) + 4 * rowp + stage_bytes();
     }
-    static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::C) + 3) & ~3;
+    static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::CP) + 3) & ~3;
     static constexpr int LDS_MAX = 160 * 1024, BLOB = 4 * K::BLOB_ELEMS;
     // preference, for rows starting at level 0: operands and claim table in
     // LDS; claim table only; operands only; neither
@@ -252,12 +260,38 @@ __device__ __forceinline__ void dense(const float* blobs, int base, int lane, co
             out[m] = Mma<K::BF16>::tile(blob_at<K>(blobs, base, m, NIN, t), lane, in[t], out[m]);
 }
 
+// The 4 channels c .. c+3 of one sample straight from global memory (DIRECT
+// rows); channels past C read as 0.
+template <int C>
+__device__ __forceinline__ float4 ld4_global(const float* p, int c) {
+    if constexpr (C % 4 == 0) {
+        return *reinterpret_cast<const float4*>(p);
+    } else {
+        return make_float4(c < C ? p[0] : 0.f, c + 1 < C ? p[1] : 0.f, c + 2 < C ? p[2] : 0.f,
+                           c + 3 < C ? p[3] : 0.f);
+    }
+}
+
+// e[k] for a lane-varying k in [0, 9) without dynamic register indexing.
+__device__ __forceinline__ float pick9(const float (&e)[9], int k) {
+    float v = e[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i) v = k == i ? e[i] : v;
+    return v;
+}
+
 // SPLIT selects the two-pass forms (a.fwd_only, a.ext_grad); the fused
-// single-pass instance carries neither test.
-template <class K, bool FB, bool SPLIT>
+// single-pass instance carries neither test.  DIRECT rows are not staged: a
+// triangle's row that does not fit the LDS plan (single-level rows at high
+// levels, e.g. 561 samples at level 5) is gathered from global memory, where
+// a wave's consecutive pixels of one triangle hit the same cache lines, and
+// its gradient is added with global atomics into the zeroed table.
+template <class K, bool FB, bool SPLIT, bool DIRECT>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock, K::MIN_BLOCKS)
 k_shade(ShadeArgs a, int rowp) {
-    constexpr int NI = K::NI, NH = K::NH, C = K::C, DIN = K::DIN, DH = K::DH;
+    constexpr int NI = K::NI, NH = K::NH, C = K::C, CP = K::CP, DF = K::DF, DIN = K::DIN, DH = K::DH;
+    static_assert(!FB || C == CP, "bf16 rows need whole 4-channel blocks");
+    static_assert(!DIRECT || K::NL == 1, "direct rows are compiled for single-level features");
     using P = Plan<K, FB>;
     using FT = typename P::FT;
     using ST = typename K::ST;
@@ -363,7 +397,16 @@ k_shade(ShadeArgs a, int rowp) {
         pos = end;
         float4* g4 = reinterpret_cast<float4*>(grow);
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if constexpr (FB) {
+        if constexpr (DIRECT) {
+            // nothing staged
+        } else if constexpr (C != CP) {
+            // channels padded to CP in LDS (the padding stays zero)
+            const float* src = a.feat + (long long)t * row;
+            for (int i = lane; i < a.S_tot * CP; i += kWave) {
+                const int sm = i / CP, c = i - sm * CP;
+                frow[i] = c < C ? src[sm * C + c] : 0.f;
+            }
+        } else if constexpr (FB) {
             // bf16 features stay bf16 in the row: 4 values per 8-byte copy
             const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row);
             uint2* f2 = reinterpret_cast<uint2*>(frow);
@@ -403,14 +446,25 @@ k_shade(ShadeArgs a, int rowp) {
             const float4 next = nv ? recs4[2 * (beg + idx + 16)] : make_float4(0.f, 0.f, 0.f, 0.f);
             const float4 ntg = g == 0 && nv ? recs4[2 * (beg + idx + 16) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
 
-            // ---- gather from the LDS row: lane (g, j) holds features 16 tt + 4 g + r of pixel j
+            // ---- the view direction's SH2 values of pixel j (the inputs after the features)
+            float shv[9];
+            if constexpr (K::SH) {
+                const Views& vw = a.views;
+                int b = 0, base = 0, Wb = vw.W[0], Hb = vw.H[0];
+                for (int k = 1; k < vw.n; ++k)
+                    if (p >= vw.pix_base[k]) { b = k; base = vw.pix_base[k]; Wb = vw.W[k]; Hb = vw.H[k]; }
+                const int local = p - base, py = local / Wb, px = local - py * Wb;
+                sh2_basis(a.inv_mvp + 16 * b, pixel_ndc(px, Wb), pixel_ndc(py, Hb), shv);
+            }
+
+            // ---- gather from the LDS row: lane (g, j) holds inputs 16 tt + 4 g + r of pixel j
             f32x4 x[NI];
             Corners crs[NI];
             for (int tt = 0; tt < NI; ++tt) {
                 x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
                 crs[tt] = Corners{{0u, 0u, 0u}, {0.f, 0.f, 0.f}};
                 const int f0 = 16 * tt + 4 * g;
-                if (valid && f0 < DIN) {
+                if (valid && f0 < DF) {
                     const int lv = f0 / C, c = f0 % C;
                     const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
                     crs[tt] = cr;
@@ -425,11 +479,16 @@ k_shade(ShadeArgs a, int rowp) {
                         v0 = *reinterpret_cast<const float4*>(fg + cr.idx[0] * C);
                         v1 = *reinterpret_cast<const float4*>(fg + cr.idx[1] * C);
                         v2 = *reinterpret_cast<const float4*>(fg + cr.idx[2] * C);
+                    } else if constexpr (DIRECT) {
+                        const float* fg = a.feat + (long long)t * row + loff[lv] * C + c;
+                        v0 = ld4_global<C>(fg + cr.idx[0] * C, c);
+                        v1 = ld4_global<C>(fg + cr.idx[1] * C, c);
+                        v2 = ld4_global<C>(fg + cr.idx[2] * C, c);
                     } else {
-                        const FT* fl = frow + loff[lv] * C + c;
-                        v0 = ld4(fl + cr.idx[0] * C);
-                        v1 = ld4(fl + cr.idx[1] * C);
-                        v2 = ld4(fl + cr.idx[2] * C);
+                        const FT* fl = frow + loff[lv] * CP + c;
+                        v0 = ld4(fl + cr.idx[0] * CP);
+                        v1 = ld4(fl + cr.idx[1] * CP);
+                        v2 = ld4(fl + cr.idx[2] * CP);
                     }
                     float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
                     e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
@@ -437,6 +496,12 @@ k_shade(ShadeArgs a, int rowp) {
                     e2 += cr.w[0] * v0.z; e2 += cr.w[1] * v1.z; e2 += cr.w[2] * v2.z;
                     e3 += cr.w[0] * v0.w; e3 += cr.w[1] * v1.w; e3 += cr.w[2] * v2.w;
                     x[tt] = f32x4{e0, e1, e2, e3};
+                }
+                if constexpr (K::SH) {
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = f0 + r - DF;
+                        if (valid && k >= 0 && k < 9) x[tt][r] = pick9(shv, k);
+                    }
                 }
             }
 
@@ -520,9 +585,23 @@ k_shade(ShadeArgs a, int rowp) {
             // atomics afterwards.
             for (int tt = 0; tt < NI; ++tt) {
                 const int f0 = 16 * tt + 4 * g;
-                const bool in = valid && f0 < DIN;
+                const bool in = valid && f0 < DF;
                 const int lv = in ? f0 / C : 0, c = f0 % C;
                 const Corners& cr = crs[tt];
+                if constexpr (K::SH || C != CP) {
+                    // only feature inputs scatter (not the SH2 values; no padding channel)
+                    for (int r = 0; r < 4; ++r)
+                        if (f0 + r >= DF || c + r >= C) dx[tt][r] = 0.f;
+                }
+                if constexpr (DIRECT) {
+                    if (in) {
+                        float* gg = a.grad_feat + (long long)t * row + loff[lv] * C + c;
+                        for (int k = 0; k < 3; ++k)
+                            for (int r = 0; r < 4; ++r)
+                                if (c + r < C) unsafeAtomicAdd(gg + cr.idx[k] * C + r, cr.w[k] * dx[tt][r]);
+                    }
+                    continue;
+                }
                 const bool coarse = tt == 0 && a.lo + lv <= 1;
                 if (in && coarse) {
                     for (int vtx = 0; vtx < 6; ++vtx) {
@@ -537,14 +616,14 @@ k_shade(ShadeArgs a, int rowp) {
                     // no room for the claim table: LDS atomics
                     if (fine)
                         for (int k = 0; k < 3; ++k) {
-                            float* gl = grow + grad_slot<C>(loff[lv] + cr.idx[k], c);
+                            float* gl = grow + grad_slot<CP>(loff[lv] + cr.idx[k], c);
                             for (int r = 0; r < 4; ++r) atomicAdd(gl + r, cr.w[k] * dx[tt][r]);
                         }
                     continue;
                 }
                 if (__ballot(fine) == 0ull) continue;   // wave-uniform
                 int slot[3];
-                for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<C>(loff[lv] + cr.idx[k], c) : 0;
+                for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<CP>(loff[lv] + cr.idx[k], c) : 0;
                 if (fine)
                     for (int k = 0; k < 3; ++k)
                         __hip_atomic_store(claim + (slot[k] >> 2), lane, __ATOMIC_RELAXED,
@@ -616,12 +695,18 @@ k_shade(ShadeArgs a, int rowp) {
             continue;
         }
 
+        if constexpr (DIRECT) {   // the gradient went straight to global memory
+            t = t_next;
+            te = te_next;
+            continue;
+        }
+
         // ---- the coarse-level sums: reduce over the 16 pixel lanes of each
         // lane group, then one lane per group adds them to the LDS row
         {
             const int f0 = 4 * g;
             const int lv = f0 / C, c = f0 % C;
-            const bool coarse = f0 < DIN && a.lo + lv <= 1;
+            const bool coarse = f0 < DF && a.lo + lv <= 1;
             const int S = coarse ? static_cast<int>(feats_at_level(a.lo + lv)) : 0;
             for (int vtx = 0; vtx < 6; ++vtx)
                 for (int r = 0; r < 4; ++r)
@@ -630,7 +715,7 @@ k_shade(ShadeArgs a, int rowp) {
             if (coarse && j == 0) {
                 for (int vtx = 0; vtx < 6; ++vtx)
                     if (vtx < S) {
-                        float* gl = grow + grad_slot<C>(loff[lv] + vtx, c);
+                        float* gl = grow + grad_slot<CP>(loff[lv] + vtx, c);
                         for (int r = 0; r < 4; ++r) gl[r] += racc[vtx][r];
                     }
             }
@@ -638,15 +723,17 @@ k_shade(ShadeArgs a, int rowp) {
         }
 
         // ---- the triangle's feature gradient row
-        if (owned) {
+        if (owned && C == CP) {
             float4* dst = reinterpret_cast<float4*>(a.grad_feat + (long long)t * row);
             for (int i = lane; i < row / 4; i += kWave)
-                dst[i] = *reinterpret_cast<const float4*>(grow + grad_slot<C>(4 * i / C, 4 * i % C));
+                dst[i] = *reinterpret_cast<const float4*>(grow + grad_slot<CP>(4 * i / C, 4 * i % C));
         } else {
             float* dst = a.grad_feat + (long long)t * row;
             for (int i = lane; i < row; i += kWave) {
-                const float gv = grow[grad_slot<C>(i / C, i % C & ~3) + (i & 3)];
-                if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
+                const int sm = i / C, c = i - sm * C;
+                const float gv = grow[grad_slot<CP>(sm, c & ~3) + (c & 3)];
+                if (owned) dst[i] = gv;
+                else if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
             }
         }
         wave_lds_sync();
@@ -747,30 +834,40 @@ __global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ 
     }
 }
 
-template <class K, bool FB, bool SPLIT>
+template <class K, bool FB, bool SPLIT, bool DIRECT>
 hipError_t shade_launch(const ShadeArgs& a, int nblocks, int rowp, size_t bytes, hipStream_t s) {
     static size_t opted_in = 65536;   // per instance: more LDS needs the kernel's opt-in
     if (bytes > opted_in) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K, FB, SPLIT>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K, FB, SPLIT, DIRECT>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  static_cast<int>(bytes));
         if (e != hipSuccess) return e;
         opted_in = bytes;
     }
-    k_shade<K, FB, SPLIT><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
+    k_shade<K, FB, SPLIT, DIRECT><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
     return hipGetLastError();
 }
 
+template <class K, bool FB, bool DIRECT>
+hipError_t shade_forms(const ShadeArgs& a, int nblocks, int rowp, size_t bytes, hipStream_t s) {
+    if (a.fwd_only || a.ext_grad) return shade_launch<K, FB, true, DIRECT>(a, nblocks, rowp, bytes, s);
+    return shade_launch<K, FB, false, DIRECT>(a, nblocks, rowp, bytes, s);
+}
+
+// Staged rows when the LDS plan fits; single-level rows that do not fit are
+// gathered and scattered in global memory instead (DIRECT).
 template <class K, bool FB>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     using P = Plan<K, FB>;
-    const int rowp = (a.S_tot * K::C + 3) & ~3;
-    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
+    const int rowp = (a.S_tot * K::CP + 3) & ~3;
     const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
-    const size_t bytes = lds > red ? lds : red;
-    if (bytes > 160 * 1024) return hipErrorInvalidValue;
-    if (a.fwd_only || a.ext_grad) return shade_launch<K, FB, true>(a, nblocks, rowp, bytes, s);
-    return shade_launch<K, FB, false>(a, nblocks, rowp, bytes, s);
+    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
+    if (std::max(lds, red) <= 160 * 1024) return shade_forms<K, FB, false>(a, nblocks, rowp, std::max(lds, red), s);
+    if constexpr (K::NL == 1 && !FB) {
+        const size_t lds0 = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(0);
+        if (std::max(lds0, red) <= 160 * 1024) return shade_forms<K, FB, true>(a, nblocks, 0, std::max(lds0, red), s);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <class K>
@@ -780,50 +877,68 @@ hipError_t blobs_dispatch(const float* w1, const float* w2, const float* w3, flo
     return hipGetLastError();
 }
 
-// The compiled configurations: (levels, channels per level, hidden width).
+// One configuration: fp32 MLP, bf16 MLP on fp32 rows, or bf16 MLP on bf16
+// rows (whole 4-channel blocks only).
+template <int NL, int C, int DH, bool SH>
+hipError_t shade_config(bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s) {
+    if (!bf16) return shade_dispatch<Cfg<NL, C, DH, false, SH>, false>(a, nblocks, s);
+    if (a.feat_bf16) {
+        if constexpr (C % 4 == 0) return shade_dispatch<Cfg<NL, C, DH, true, SH>, true>(a, nblocks, s);
+        return hipErrorInvalidValue;
+    }
+    return shade_dispatch<Cfg<NL, C, DH, true, SH>, false>(a, nblocks, s);
+}
+
+// The compiled configurations: (levels, channels per level, hidden width,
+// SH2 inputs after the features).  (1, 7, 16, SH2) is the reference's own
+// network: 7 accumulated feature channels + 9 SH2 values into 16 -> 16 -> 16
+// -> 3 (examples/02_synthetic.py:290,305).
 #define DS_SHADE_CONFIGS(X) \
-    X(1, 8, 16)             \
-    X(4, 8, 32)             \
-    X(1, 8, 32)             \
-    X(4, 4, 32)             \
-    X(2, 8, 32)             \
-    X(5, 16, 64)
+    X(1, 8, 16, false)      \
+    X(4, 8, 32, false)      \
+    X(1, 8, 32, false)      \
+    X(4, 4, 32, false)      \
+    X(2, 8, 32, false)      \
+    X(5, 16, 64, false)     \
+    X(1, 7, 16, true)
 
 }  // namespace
 
-bool shade_supported(int nl, int c, int dh) {
-#define X(NL, C, DH) if (nl == NL && c == C && dh == DH) return true;
+bool shade_supported(int nl, int c, int dh, bool sh) {
+#define X(NL, C, DH, SH) if (nl == NL && c == C && dh == DH && sh == SH) return true;
     DS_SHADE_CONFIGS(X)
 #undef X
     return false;
 }
 
-size_t shade_blob_floats(int nl, int c, int dh) {
-#define X(NL, C, DH) if (nl == NL && c == C && dh == DH) return Cfg<NL, C, DH, false>::BLOB_ELEMS;
+size_t shade_blob_floats(int nl, int c, int dh, bool sh) {
+#define X(NL, C, DH, SH) \
+    if (nl == NL && c == C && dh == DH && sh == SH) return Cfg<NL, C, DH, false, SH>::BLOB_ELEMS;
     DS_SHADE_CONFIGS(X)
 #undef X
     return 0;
 }
 
-int shade_nparam(int nl, int c, int dh) { return dh * nl * c + dh * dh + 3 * dh + dh + dh + 3 + 1; }
+int shade_nparam(int nl, int c, int dh, bool sh) {
+    const int din = nl * c + (sh ? 9 : 0);
+    return dh * din + dh * dh + 3 * dh + dh + dh + 3 + 1;
+}
 
-hipError_t launch_shade_blobs(int nl, int c, int dh, bool bf16, const float* w1, const float* w2,
+hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const float* w1, const float* w2,
                               const float* w3, float* blobs, hipStream_t s) {
-#define X(NL, C, DH)                                                                   \
-    if (nl == NL && c == C && dh == DH)                                                \
-        return bf16 ? blobs_dispatch<Cfg<NL, C, DH, true>>(w1, w2, w3, blobs, s)       \
-                    : blobs_dispatch<Cfg<NL, C, DH, false>>(w1, w2, w3, blobs, s);
+#define X(NL, C, DH, SH)                                                                    \
+    if (nl == NL && c == C && dh == DH && sh == SH)                                         \
+        return bf16 ? blobs_dispatch<Cfg<NL, C, DH, true, SH>>(w1, w2, w3, blobs, s)        \
+                    : blobs_dispatch<Cfg<NL, C, DH, false, SH>>(w1, w2, w3, blobs, s);
     DS_SHADE_CONFIGS(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_shade(int nl, int c, int dh, bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s) {
-#define X(NL, C, DH)                                                                   \
-    if (nl == NL && c == C && dh == DH)                                                \
-        return !bf16        ? shade_dispatch<Cfg<NL, C, DH, false>, false>(a, nblocks, s)  \
-               : a.feat_bf16 ? shade_dispatch<Cfg<NL, C, DH, true>, true>(a, nblocks, s)    \
-                             : shade_dispatch<Cfg<NL, C, DH, true>, false>(a, nblocks, s);
+hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
+                        hipStream_t s) {
+#define X(NL, C, DH, SH) \
+    if (nl == NL && c == C && dh == DH && sh == SH) return shade_config<NL, C, DH, SH>(bf16, a, nblocks, s);
     DS_SHADE_CONFIGS(X)
 #undef X
     return hipErrorInvalidValue;

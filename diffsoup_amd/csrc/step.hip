@@ -28,14 +28,25 @@
 namespace ds {
 namespace {
 
-// The step's prologue, grid-stride: sigmoid of the opacity logits, zeroed
-// gradient accumulators and per-triangle counts, clip = M_b [x, y, z, 1] for
-// every (view, vertex) (row-major M, examples/utils.py:412-426, summed in the
-// order ((m0 x + m1 y) + m2 z) + m3), and the raster bins' cleared tile
-// counters and stats block.
+// Zero n floats at p (16-byte aligned) from a grid-stride loop: float4 body,
+// scalar tail.
+__device__ __forceinline__ void zero_floats(float* p, long long n, long long i0, long long stride) {
+    if (!p) return;
+    float4* p4 = reinterpret_cast<float4*>(p);
+    for (long long i = i0; i < n / 4; i += stride) p4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (long long i = (n / 4) * 4 + i0; i < n; i += stride) p[i] = 0.f;
+}
+
+// The step's prologue, grid-stride: sigmoid of the opacity logits (when
+// logits is given; the accumulated opacities come from k_accum_sigmoid
+// instead), zeroed gradient accumulators and per-triangle counts, clip =
+// M_b [x, y, z, 1] for every (view, vertex) (row-major M,
+// examples/utils.py:412-426, summed in the order ((m0 x + m1 y) + m2 z) + m3),
+// and the raster bins' cleared tile counters and stats block.
 __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict__ alpha, long long n_alpha,
                             float* __restrict__ galpha, float* __restrict__ gpos, long long n_pos,
                             int* __restrict__ tri_count, int T, float* __restrict__ gfeat, long long n_feat,
+                            float* __restrict__ z1, long long n_z1, float* __restrict__ z2, long long n_z2,
                             int B, int V, const float* __restrict__ verts, const float* __restrict__ mvp,
                             float4* __restrict__ pos, int* __restrict__ tile_count, int n_tiles,
                             long long* __restrict__ stats) {
@@ -44,11 +55,12 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
     for (long long i = i0; i < n_alpha; i += stride) {
         // evaluated in double and rounded once, so the opacity is the
         // correctly rounded sigmoid (the last bit decides mask ties)
-        alpha[i] = static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(logits[i]))));
+        if (logits) alpha[i] = static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(logits[i]))));
         galpha[i] = 0.f;
     }
-    float4* gf4 = reinterpret_cast<float4*>(gfeat);
-    for (long long i = i0; i < n_feat / 4; i += stride) gf4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    zero_floats(gfeat, n_feat, i0, stride);
+    zero_floats(z1, n_z1, i0, stride);
+    zero_floats(z2, n_z2, i0, stride);
     float4* gp4 = reinterpret_cast<float4*>(gpos);
     for (long long i = i0; i < n_pos / 4; i += stride) gp4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (long long i = i0; i < T; i += stride) tri_count[i] = 0;
@@ -63,6 +75,89 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
     }
     for (long long i = i0; i < n_tiles; i += stride) tile_count[i] = 0;
     if (i0 < kStatCount) stats[i0] = 0;
+}
+
+// One accumulation of the reference's composition: the level-summed lattice
+// function of src [T, S_tot(lo..hi), C] evaluated at the vertices of the
+// target lattice (accumulate_to_level, python/diffsoup/multires.py:279-339,
+// src/cuda/multires.cu:314-389), then its sigmoid (examples/02_synthetic.py:
+// 355-356): dst [T, S_T, C].  The backward reads dst (the sigmoid) and the
+// gradient of dst and scatters scale * g * s (1 - s) onto src's lattices.
+struct AccumJob {
+    const float* src;
+    float* dst;
+    const float* grad_dst;   // backward
+    float* grad_src;         // backward (zeroed, accumulated)
+    float scale;             // backward
+    int lo, hi, target, C, S_tot, S_T;
+};
+
+__device__ __forceinline__ void accum_point(const AccumJob& J, long long lid, int& t, float& b0, float& b1) {
+    t = static_cast<int>(lid / J.S_T);
+    const uint32_t k = static_cast<uint32_t>(lid - (long long)t * J.S_T);
+    uint32_t x, y;
+    lattice_xy(k, x, y);
+    const float Rf = static_cast<float>(1u << J.target);
+    b0 = static_cast<float>(x) / Rf;
+    b1 = static_cast<float>(y) / Rf;
+}
+
+// Forward of both accumulations (features, then opacities), grid-stride over
+// the target samples of both; each sample's channels are summed in the order
+// of ds_accumulate_fwd and the sigmoid is evaluated in double and rounded once.
+__global__ void k_accum_sigmoid(int T, AccumJob f, AccumJob a) {
+    const long long nf = (long long)T * f.S_T, n = nf + (long long)T * a.S_T;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n; id += stride) {
+        const bool is_f = id < nf;
+        const AccumJob& J = is_f ? f : a;
+        const long long lid = is_f ? id : id - nf;
+        int t;
+        float b0, b1;
+        accum_point(J, lid, t, b0, b1);
+        const float* src = J.src + (long long)t * J.S_tot * J.C;
+        for (int j = 0; j < J.C; ++j) {
+            float v = 0.f;
+            uint32_t off = 0;
+            for (int l = J.lo; l <= J.hi; ++l) {
+                const Corners c = lattice_corners(b0, b1, static_cast<uint32_t>(l));
+                const float* fl = src + (long long)off * J.C;
+                v += c.w[0] * fl[c.idx[0] * J.C + j];
+                v += c.w[1] * fl[c.idx[1] * J.C + j];
+                v += c.w[2] * fl[c.idx[2] * J.C + j];
+                off += feats_at_level(static_cast<uint32_t>(l));
+            }
+            J.dst[lid * J.C + j] = static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(v))));
+        }
+    }
+}
+
+// Backward of both: grad_src += scale * grad_dst * s (1 - s) scattered through
+// the transpose of the accumulation (ds_accumulate_bwd's atomics).
+__global__ void k_accum_sigmoid_bwd(int T, AccumJob f, AccumJob a) {
+    const long long nf = (long long)T * f.S_T, n = nf + (long long)T * a.S_T;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n; id += stride) {
+        const bool is_f = id < nf;
+        const AccumJob& J = is_f ? f : a;
+        const long long lid = is_f ? id : id - nf;
+        int t;
+        float b0, b1;
+        accum_point(J, lid, t, b0, b1);
+        float* gsrc = J.grad_src + (long long)t * J.S_tot * J.C;
+        uint32_t off = 0;
+        for (int l = J.lo; l <= J.hi; ++l) {
+            const Corners c = lattice_corners(b0, b1, static_cast<uint32_t>(l));
+            float* dst = gsrc + (long long)off * J.C;
+            for (int j = 0; j < J.C; ++j) {
+                const float sg = J.dst[lid * J.C + j];
+                const float g = J.scale * J.grad_dst[lid * J.C + j] * (sg * (1.0f - sg));
+                if (g == 0.f) continue;
+                for (int i = 0; i < 3; ++i) unsafeAtomicAdd(&dst[c.idx[i] * J.C + j], g * c.w[i]);
+            }
+            off += feats_at_level(static_cast<uint32_t>(l));
+        }
+    }
 }
 
 // The step's closing reductions in one launch of 1024-thread blocks:
@@ -177,10 +272,16 @@ inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct StepLayout {
     size_t raster, gpos, galpha, tile_list, tile_ncov, tri_count, tri_off, list, scan, bgloss, blobs, partials,
-        tile_px_off, px_list, scan2, photo,
+        tile_px_off, px_list, scan2, photo, feat_acc, gfeat_acc,
         total;
     int n_tiles, nblocks, nparam;
+    int nl;          // levels the shading kernel gathers (1 with accumulate)
+    bool sh;         // SH2 inputs
     long long P;
+    long long S_src;   // samples per triangle of d->feat
+    long long S_shade; // samples per triangle of the table the shading reads
+    long long Sa_src;  // samples per triangle of d->alpha_logits
+    long long Sa;      // opacity samples at alpha_level
 };
 
 int shade_blocks(long long P) {
@@ -191,17 +292,22 @@ int shade_blocks(long long P) {
 }
 
 bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
-    const int nl = d->feat_hi - d->feat_lo + 1;
-    if (!shade_supported(nl, d->channels, d->hidden)) return false;
+    const bool acc = d->accumulate != 0;
+    L.nl = acc ? 1 : d->feat_hi - d->feat_lo + 1;
+    L.sh = d->sh2 != 0;
+    if (!shade_supported(L.nl, d->channels, d->hidden, L.sh)) return false;
     L.n_tiles = v.tile_base[v.n];
     L.P = v.pix_base[v.n];
     L.nblocks = shade_blocks(L.P);
-    L.nparam = shade_nparam(nl, d->channels, d->hidden);
-    const long long Sa = feats_at_level(d->alpha_level);
+    L.nparam = shade_nparam(L.nl, d->channels, d->hidden, L.sh);
+    L.Sa = feats_at_level(d->alpha_level);
+    L.Sa_src = acc ? feats_in_levels(d->alpha_lo, d->alpha_level) : L.Sa;
+    L.S_src = feats_in_levels(d->feat_lo, d->feat_hi);
+    L.S_shade = acc ? feats_at_level(d->feat_hi) : L.S_src;
     size_t o = 0;
     L.raster = o; o += up256(raster_workspace_bytes(d->B, d->T, L.n_tiles, d->capacity));
     L.gpos = o; o += up256(sizeof(float) * 4 * (size_t)d->B * d->V);
-    L.galpha = o; o += up256(sizeof(float) * (size_t)d->T * Sa);
+    L.galpha = o; o += up256(sizeof(float) * (size_t)d->T * L.Sa);
     const size_t nt2 = (size_t)d->tile * d->tile;
     L.tile_list = o; o += up256(sizeof(int) * 2 * (size_t)L.n_tiles * nt2);
     L.tile_ncov = o; o += up256(sizeof(int) * (size_t)L.n_tiles);
@@ -213,8 +319,14 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.px_list = o; o += up256(sizeof(int4) * (size_t)L.P);
     L.scan2 = o; o += up256(scan_tmp_bytes(L.n_tiles));
     L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
-    L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(nl, d->channels, d->hidden));
+    L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(L.nl, d->channels, d->hidden, L.sh));
     L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
+    L.feat_acc = L.gfeat_acc = o;
+    if (acc) {
+        const size_t nacc = sizeof(float) * (size_t)d->T * L.S_shade * d->channels;
+        L.feat_acc = o; o += up256(nacc);
+        L.gfeat_acc = o; o += up256(nacc);
+    }
     L.photo = o;
     if (d->loss_kind == DS_LOSS_L1_SSIM) {
         const size_t nf = photo_loss_floats(v, 3);
@@ -225,42 +337,53 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     return true;
 }
 
+// Everything one enqueue of the step needs: the views, the workspace layout
+// and its carve-up.
+struct Step {
+    Views v;
+    StepLayout L;
+    RasterWorkspace rw;
+    float *gpos, *galpha, *bgloss, *blobs, *partials, *feat_acc, *gfeat_acc, *photo;
+    int2* tile_list;
+    int *tile_ncov, *tri_count, *tri_off, *tile_px_off;
+    PixRec* list;
+    int4* px_list;
+    void *scan_tmp, *scan_tmp2;
+    float inv_n;   // 1 / (3 P): the MSE mean and the opacity term's 1 / numel(color)
+    hipStream_t s;
+};
+
+int step_prepare(const ds_step_desc* d, void* stream, Step& st);
+int enqueue_forward(const ds_step_desc* d, Step& st);
+int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ext_grad);
+int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, bool write_loss);
+
 }  // namespace
 
 // Exported through capi.hip's error helpers.
 int step_make_views(const ds_step_desc* d, Views& v);
 int step_fail(int code, const char* msg);
 
-}  // namespace ds
+namespace {
 
-extern "C" {
-
-int64_t ds_step_mlp_params(int levels, int channels, int hidden) {
-    return ds::shade_nparam(levels, channels, hidden);
-}
-
-size_t ds_step_workspace_size(const ds_step_desc* d) {
-    ds::Views v;
-    if (!d || ds::step_make_views(d, v) != DS_OK) return 0;
-    ds::StepLayout L;
-    if (!ds::layout(d, v, L)) return 0;
-    return L.total;
-}
-
-int ds_train_step(const ds_step_desc* d, void* stream) {
-    using namespace ds;
+int step_prepare(const ds_step_desc* d, void* stream, Step& st) {
     if (!d) return step_fail(DS_ERR_INVALID, "null step descriptor");
-    Views v;
-    int rc = step_make_views(d, v);
+    int rc = step_make_views(d, st.v);
     if (rc != DS_OK) return rc;
     const int nl = d->feat_hi - d->feat_lo + 1;
     if (d->feat_lo < 0 || nl < 1 || d->feat_hi > 10 || d->alpha_level < 0 || d->alpha_level > 10)
         return step_fail(DS_ERR_INVALID, "level range invalid");
+    if (d->accumulate && (d->alpha_lo < 0 || d->alpha_lo > d->alpha_level))
+        return step_fail(DS_ERR_INVALID, "alpha_lo must be in [0, alpha_level]");
+    if (d->accumulate && d->feat_bf16)
+        return step_fail(DS_ERR_INVALID, "accumulated features are fp32: feat_bf16 must be null");
+    if (d->sh2 && !d->inv_mvp) return step_fail(DS_ERR_INVALID, "sh2 needs inv_mvp");
     if (d->T < 0 || d->V < 0 || d->T > (1 << 24) - 1) return step_fail(DS_ERR_INVALID, "bad T or V");
     if (d->loss_kind != DS_LOSS_MSE && d->loss_kind != DS_LOSS_L1_SSIM)
         return step_fail(DS_ERR_INVALID, "loss_kind must be DS_LOSS_MSE or DS_LOSS_L1_SSIM");
-    StepLayout L;
-    if (!layout(d, v, L)) return step_fail(DS_ERR_UNSUPPORTED, "no compiled shading kernel for this (levels, channels, hidden)");
+    StepLayout& L = st.L;
+    if (!layout(d, st.v, L))
+        return step_fail(DS_ERR_UNSUPPORTED, "no compiled shading kernel for this (levels, channels, hidden, sh2)");
     if (!d->workspace || d->workspace_bytes < L.total)
         return step_fail(DS_ERR_WORKSPACE, "workspace smaller than ds_step_workspace_size");
     if (reinterpret_cast<uintptr_t>(d->workspace) & 255u) return step_fail(DS_ERR_INVALID, "workspace must be 256-byte aligned");
@@ -278,116 +401,225 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
          reinterpret_cast<uintptr_t>(feat_in) | reinterpret_cast<uintptr_t>(d->grad_feat)) & 15u)
         return step_fail(DS_ERR_INVALID, "pos, rast, feat (or feat_bf16) and grad_feat must be 16-byte aligned");
 
-    hipStream_t s = static_cast<hipStream_t>(stream);
+    st.s = static_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(d->workspace);
-    RasterWorkspace rw = raster_workspace_carve(ws + L.raster, d->B, d->T, L.n_tiles, d->capacity);
-    float* gpos = reinterpret_cast<float*>(ws + L.gpos);
-    float* galpha = reinterpret_cast<float*>(ws + L.galpha);
-    int2* tile_list = reinterpret_cast<int2*>(ws + L.tile_list);
-    int* tile_ncov = reinterpret_cast<int*>(ws + L.tile_ncov);
-    int* tri_count = reinterpret_cast<int*>(ws + L.tri_count);
-    int* tri_off = reinterpret_cast<int*>(ws + L.tri_off);
-    PixRec* list = reinterpret_cast<PixRec*>(ws + L.list);
-    void* scan_tmp = ws + L.scan;
-    int* tile_px_off = reinterpret_cast<int*>(ws + L.tile_px_off);
-    int4* px_list = reinterpret_cast<int4*>(ws + L.px_list);
-    void* scan_tmp2 = ws + L.scan2;
-    float* bgloss = reinterpret_cast<float*>(ws + L.bgloss);
-    float* blobs = reinterpret_cast<float*>(ws + L.blobs);
-    float* partials = reinterpret_cast<float*>(ws + L.partials);
+    st.rw = raster_workspace_carve(ws + L.raster, d->B, d->T, L.n_tiles, d->capacity);
+    st.gpos = reinterpret_cast<float*>(ws + L.gpos);
+    st.galpha = reinterpret_cast<float*>(ws + L.galpha);
+    st.tile_list = reinterpret_cast<int2*>(ws + L.tile_list);
+    st.tile_ncov = reinterpret_cast<int*>(ws + L.tile_ncov);
+    st.tri_count = reinterpret_cast<int*>(ws + L.tri_count);
+    st.tri_off = reinterpret_cast<int*>(ws + L.tri_off);
+    st.list = reinterpret_cast<PixRec*>(ws + L.list);
+    st.scan_tmp = ws + L.scan;
+    st.tile_px_off = reinterpret_cast<int*>(ws + L.tile_px_off);
+    st.px_list = reinterpret_cast<int4*>(ws + L.px_list);
+    st.scan_tmp2 = ws + L.scan2;
+    st.bgloss = reinterpret_cast<float*>(ws + L.bgloss);
+    st.blobs = reinterpret_cast<float*>(ws + L.blobs);
+    st.partials = reinterpret_cast<float*>(ws + L.partials);
+    st.feat_acc = d->accumulate ? reinterpret_cast<float*>(ws + L.feat_acc) : nullptr;
+    st.gfeat_acc = d->accumulate ? reinterpret_cast<float*>(ws + L.gfeat_acc) : nullptr;
+    st.photo = reinterpret_cast<float*>(ws + L.photo);
+    st.inv_n = 1.0f / (3.0f * static_cast<float>(L.P));
+    return DS_OK;
+}
 
-    const long long Sa = feats_at_level(d->alpha_level);
-    const long long S_tot = feats_in_levels(d->feat_lo, d->feat_hi);
-    const long long n_alpha = (long long)d->T * Sa;
-    const long long n_feat = (long long)d->T * S_tot * d->channels;
-    const long long n_pos = 4LL * d->B * d->V;
-    const float inv_n = 1.0f / (3.0f * static_cast<float>(L.P));
-
-    hipError_t e;
 #define DS_TRY(expr, what) \
     if ((e = (expr)) != hipSuccess) return step_fail(DS_ERR_HIP, what);
-    auto mark = [&](int stage, bool begin) -> hipError_t {
-        if (d->timed_stage != stage) return hipSuccess;
-        void* ev = begin ? d->ev_begin : d->ev_end;
-        return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
-    };
 
+hipError_t mark(const ds_step_desc* d, hipStream_t s, int stage, bool begin) {
+    if (d->timed_stage != stage) return hipSuccess;
+    void* ev = begin ? d->ev_begin : d->ev_end;
+    return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
+}
+
+AccumJob feat_job(const ds_step_desc* d, const Step& st) {
+    return AccumJob{d->feat, st.feat_acc, st.gfeat_acc, d->grad_feat, 1.0f, d->feat_lo, d->feat_hi, d->feat_hi,
+                    d->channels, static_cast<int>(st.L.S_src), static_cast<int>(st.L.S_shade)};
+}
+
+AccumJob alpha_job(const ds_step_desc* d, const Step& st) {
+    return AccumJob{d->alpha_logits, d->alpha, st.galpha, d->grad_alpha_logits, st.inv_n, d->alpha_lo,
+                    d->alpha_level, d->alpha_level, 1, static_cast<int>(st.L.Sa_src), static_cast<int>(st.L.Sa)};
+}
+
+// Prologue, binning, resolve (with background shading) and the covered-pixel
+// lists: everything before the shading kernel.
+int enqueue_forward(const ds_step_desc* d, Step& st) {
+    const StepLayout& L = st.L;
+    const hipStream_t s = st.s;
+    const bool acc = d->accumulate != 0;
+    const long long n_alpha = (long long)d->T * L.Sa;
+    const long long n_feat = (long long)d->T * L.S_shade * d->channels;   // the table the shading writes
+    const long long n_src = (long long)d->T * L.S_src * d->channels;
+    const long long n_asrc = (long long)d->T * L.Sa_src;
+    const long long n_pos = 4LL * d->B * d->V;
+    hipError_t e;
     const long long prep_n = std::max(std::max(std::max(n_alpha, (long long)d->T), (long long)L.n_tiles),
-                                      std::max(std::max(n_pos / 4, n_feat / 4), std::max((long long)d->B * d->V,
-                                                                                         (long long)kStatCount)));
-    DS_TRY(mark(DS_STAGE_PREP, true), "event");
-    k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
-        d->alpha_logits, d->alpha, n_alpha, galpha, gpos, n_pos, tri_count, d->T, d->grad_feat, n_feat, d->B, d->V,
-        d->verts, d->mvp, reinterpret_cast<float4*>(d->pos), rw.tile_count, L.n_tiles, rw.stats);
+                                      std::max(std::max(n_pos / 4, n_feat / 4),
+                                               std::max((long long)d->B * d->V, (long long)kStatCount)));
+    DS_TRY(mark(d, s, DS_STAGE_PREP, true), "event");
+    k_step_prep<<<std::min<unsigned>(blocks_for(std::max(prep_n, acc ? n_src / 4 : 0LL), 256), 4096), 256, 0, s>>>(
+        acc ? nullptr : d->alpha_logits, d->alpha, n_alpha, st.galpha, st.gpos, n_pos, st.tri_count, d->T,
+        acc ? st.gfeat_acc : d->grad_feat, n_feat, acc ? d->grad_feat : nullptr, acc ? n_src : 0,
+        acc ? d->grad_alpha_logits : nullptr, acc ? n_asrc : 0, d->B, d->V, d->verts, d->mvp,
+        reinterpret_cast<float4*>(d->pos), st.rw.tile_count, L.n_tiles, st.rw.stats);
     DS_TRY(hipGetLastError(), "step prep");
-    DS_TRY(mark(DS_STAGE_PREP, false), "event");
-    DS_TRY(mark(DS_STAGE_BLOBS, true), "event");
-    DS_TRY(launch_shade_blobs(nl, d->channels, d->hidden, d->bf16 != 0, d->w1, d->w2, d->w3, blobs, s), "blobs");
-    DS_TRY(mark(DS_STAGE_BLOBS, false), "event");
-    DS_TRY(mark(DS_STAGE_PROJECT, true), "event");   // projection runs in the prologue
-    DS_TRY(mark(DS_STAGE_PROJECT, false), "event");
-    DS_TRY(mark(DS_STAGE_BIN, true), "event");
-    DS_TRY(launch_raster_bin(v, d->V, d->pos, d->T, d->tri, rw, s, /*clear=*/false), "raster bin");
-    DS_TRY(mark(DS_STAGE_BIN, false), "event");
-    AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(Sa)};
-    BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, bgloss, tile_list, tile_ncov, tri_count,
-                       2.0f * inv_n, d->background};
-    DS_TRY(mark(DS_STAGE_RESOLVE, true), "event");
-    const MaskSeed seed{d->seed, reinterpret_cast<const unsigned long long*>(d->seed_device)};
-    DS_TRY(launch_raster_resolve_shade(v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, seed,
-                                       d->rast, rw, bg, s),
-           "resolve");
-    DS_TRY(mark(DS_STAGE_RESOLVE, false), "event");
-    DS_TRY(launch_exclusive_scan_pair(tri_count, d->T, tri_off, scan_tmp, tile_ncov, L.n_tiles, tile_px_off,
-                                      scan_tmp2, s),
-           "triangle / tile scans");
-    DS_TRY(launch_fill_triangle_lists(v, tile_list, tile_ncov, d->rast, d->target, tri_off, list, tile_px_off,
-                                      px_list, s),
-           "triangle lists");
-    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), tri_off, list, d->T, d->feat,
-                 reinterpret_cast<const unsigned short*>(d->feat_bf16), d->grad_feat,
-                 static_cast<int>(S_tot), d->feat_lo, blobs, d->b1, d->b2, d->b3, d->color,
-                 d->grad_color, d->pix_loss, 2.0f * inv_n, partials};
-    const bool photo = d->loss_kind == DS_LOSS_L1_SSIM;
-    DS_TRY(mark(DS_STAGE_SHADE, true), "event");
-    if (photo) {
-        // the loss couples neighbouring pixels: colour first, then the
-        // L1 + SSIM gradient of the whole image, then the adjoint on it
-        sa.fwd_only = 1;
-        DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade (colour)");
-        DS_TRY(launch_photo_loss(v, 3, d->target, d->color, reinterpret_cast<float*>(ws + L.photo), 0.8f, 0.2f,
-                                 d->grad_color, d->loss, s),
-               "L1 + SSIM loss");
-        sa.fwd_only = 0;
-        sa.ext_grad = d->grad_color;
+    if (acc) {
+        const long long n = (long long)d->T * (L.S_shade + L.Sa);
+        k_accum_sigmoid<<<std::min<unsigned>(blocks_for(n, 256), 8192), 256, 0, s>>>(d->T, feat_job(d, st),
+                                                                                     alpha_job(d, st));
+        DS_TRY(hipGetLastError(), "accumulate + sigmoid");
     }
-    DS_TRY(launch_shade(nl, d->channels, d->hidden, d->bf16 != 0, sa, L.nblocks, s), "shade");
-    DS_TRY(mark(DS_STAGE_SHADE, false), "event");
-    DS_TRY(mark(DS_STAGE_EDGE, true), "event");
-    DS_TRY(launch_edge_grad_sorted(v, 3, d->color, d->grad_color, d->rast, d->V, d->pos, gpos, d->T, d->tri,
-                                   rw.recs, px_list, tile_px_off + L.n_tiles, L.P, s),
+    DS_TRY(mark(d, s, DS_STAGE_PREP, false), "event");
+    DS_TRY(mark(d, s, DS_STAGE_BLOBS, true), "event");
+    DS_TRY(launch_shade_blobs(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, d->w1, d->w2, d->w3, st.blobs, s),
+           "blobs");
+    DS_TRY(mark(d, s, DS_STAGE_BLOBS, false), "event");
+    DS_TRY(mark(d, s, DS_STAGE_PROJECT, true), "event");   // projection runs in the prologue
+    DS_TRY(mark(d, s, DS_STAGE_PROJECT, false), "event");
+    DS_TRY(mark(d, s, DS_STAGE_BIN, true), "event");
+    DS_TRY(launch_raster_bin(st.v, d->V, d->pos, d->T, d->tri, st.rw, s, /*clear=*/false), "raster bin");
+    DS_TRY(mark(d, s, DS_STAGE_BIN, false), "event");
+    AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(L.Sa)};
+    BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, st.bgloss, st.tile_list, st.tile_ncov,
+                       st.tri_count, 2.0f * st.inv_n, d->background};
+    DS_TRY(mark(d, s, DS_STAGE_RESOLVE, true), "event");
+    const MaskSeed seed{d->seed, reinterpret_cast<const unsigned long long*>(d->seed_device)};
+    DS_TRY(launch_raster_resolve_shade(st.v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, seed,
+                                       d->rast, st.rw, bg, s),
+           "resolve");
+    DS_TRY(mark(d, s, DS_STAGE_RESOLVE, false), "event");
+    DS_TRY(launch_exclusive_scan_pair(st.tri_count, d->T, st.tri_off, st.scan_tmp, st.tile_ncov, L.n_tiles,
+                                      st.tile_px_off, st.scan_tmp2, s),
+           "triangle / tile scans");
+    DS_TRY(launch_fill_triangle_lists(st.v, st.tile_list, st.tile_ncov, d->rast, d->target, st.tri_off, st.list,
+                                      st.tile_px_off, st.px_list, s),
+           "triangle lists");
+    return DS_OK;
+}
+
+// One shading launch: fused (colour, MSE gradient, adjoint), colour only, or
+// the adjoint on an external colour gradient.
+int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ext_grad) {
+    const StepLayout& L = st.L;
+    const bool acc = d->accumulate != 0;
+    ShadeArgs sa{reinterpret_cast<const float4*>(d->rast), st.tri_off, st.list, d->T, acc ? st.feat_acc : d->feat,
+                 acc ? nullptr : reinterpret_cast<const unsigned short*>(d->feat_bf16),
+                 acc ? st.gfeat_acc : d->grad_feat, static_cast<int>(L.S_shade), acc ? d->feat_hi : d->feat_lo,
+                 st.blobs, d->b1, d->b2, d->b3, d->color, d->grad_color, d->pix_loss, 2.0f * st.inv_n,
+                 st.partials, ext_grad, fwd_only, st.v, d->inv_mvp};
+    hipError_t e;
+    DS_TRY(launch_shade(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, sa, L.nblocks, st.s), "shade");
+    return DS_OK;
+}
+
+// Edge gradients, opacity gradient, the closing reductions and (accumulate)
+// the backward of the accumulations: everything after the shading adjoint.
+int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, bool write_loss) {
+    const StepLayout& L = st.L;
+    const hipStream_t s = st.s;
+    const bool acc = d->accumulate != 0;
+    hipError_t e;
+    DS_TRY(mark(d, s, DS_STAGE_EDGE, true), "event");
+    DS_TRY(launch_edge_grad_sorted(st.v, 3, d->color, grad_color, d->rast, d->V, d->pos, st.gpos, d->T, d->tri,
+                                   st.rw.recs, st.px_list, st.tile_px_off + L.n_tiles, L.P, s),
            "edge grad");
-    DS_TRY(mark(DS_STAGE_EDGE, false), "event");
-    DS_TRY(mark(DS_STAGE_AUX, true), "event");
-    DS_TRY(launch_opacity_aux(v, d->T, a, d->rast, d->pix_loss, galpha, rw, s), "opacity aux");
-    DS_TRY(mark(DS_STAGE_AUX, false), "event");
-    DS_TRY(mark(DS_STAGE_FINAL, true), "event");
+    DS_TRY(mark(d, s, DS_STAGE_EDGE, false), "event");
+    DS_TRY(mark(d, s, DS_STAGE_AUX, true), "event");
+    AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(L.Sa)};
+    DS_TRY(launch_opacity_aux(st.v, d->T, a, d->rast, d->pix_loss, st.galpha, st.rw, s), "opacity aux");
+    DS_TRY(mark(d, s, DS_STAGE_AUX, false), "event");
+    DS_TRY(mark(d, s, DS_STAGE_FINAL, true), "event");
     {
+        const long long n_alpha = acc ? 0 : (long long)d->T * L.Sa;   // accumulate: the sigmoid backward follows
         const long long nf = std::max<long long>(d->V, n_alpha);
         const unsigned R = static_cast<unsigned>((L.nparam + 63) / 64);
         k_epilogue<<<R + 2 + blocks_for(nf, 1024), 1024, 0, s>>>(
-            partials, L.nblocks, L.nparam, d->grad_mlp, bgloss, L.n_tiles, inv_n, photo ? nullptr : d->loss, d->B,
-            d->V, d->mvp,
-            reinterpret_cast<const float4*>(gpos), d->grad_verts, n_alpha, d->alpha, galpha, d->grad_alpha_logits,
-            rw.tile_cov, L.n_tiles, rw.stats,
+            st.partials, L.nblocks, L.nparam, d->grad_mlp, st.bgloss, L.n_tiles, st.inv_n,
+            write_loss ? d->loss : nullptr, d->B, d->V, d->mvp, reinterpret_cast<const float4*>(st.gpos),
+            d->grad_verts, n_alpha, d->alpha, st.galpha, d->grad_alpha_logits, st.rw.tile_cov, L.n_tiles,
+            st.rw.stats,
             d->seed_device && d->advance_seed
                 ? const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(d->seed_device))
                 : nullptr);
         DS_TRY(hipGetLastError(), "epilogue");
+        if (acc) {
+            const long long n = (long long)d->T * (L.S_shade + L.Sa);
+            k_accum_sigmoid_bwd<<<std::min<unsigned>(blocks_for(n, 256), 8192), 256, 0, s>>>(d->T, feat_job(d, st),
+                                                                                             alpha_job(d, st));
+            DS_TRY(hipGetLastError(), "accumulate + sigmoid backward");
+        }
     }
-    DS_TRY(mark(DS_STAGE_FINAL, false), "event");
-#undef DS_TRY
+    DS_TRY(mark(d, s, DS_STAGE_FINAL, false), "event");
     return DS_OK;
+}
+
+#undef DS_TRY
+
+}  // namespace
+}  // namespace ds
+
+extern "C" {
+
+int64_t ds_step_mlp_params(int levels, int channels, int hidden) {
+    return ds::shade_nparam(levels, channels, hidden, false);
+}
+
+int64_t ds_step_mlp_params_sh2(int levels, int channels, int hidden, int sh2) {
+    return ds::shade_nparam(levels, channels, hidden, sh2 != 0);
+}
+
+size_t ds_step_workspace_size(const ds_step_desc* d) {
+    ds::Views v;
+    if (!d || ds::step_make_views(d, v) != DS_OK) return 0;
+    ds::StepLayout L;
+    if (!ds::layout(d, v, L)) return 0;
+    return L.total;
+}
+
+int ds_train_step(const ds_step_desc* d, void* stream) {
+    using namespace ds;
+    Step st;
+    int rc = step_prepare(d, stream, st);
+    if (rc != DS_OK) return rc;
+    if ((rc = enqueue_forward(d, st)) != DS_OK) return rc;
+    hipError_t e;
+    if ((e = mark(d, st.s, DS_STAGE_SHADE, true)) != hipSuccess) return step_fail(DS_ERR_HIP, "event");
+    if (d->loss_kind == DS_LOSS_L1_SSIM) {
+        // the loss couples neighbouring pixels: colour first, then the
+        // L1 + SSIM gradient of the whole image, then the adjoint on it
+        if ((rc = enqueue_shade(d, st, 1, nullptr)) != DS_OK) return rc;
+        if ((e = launch_photo_loss(st.v, 3, d->target, d->color, st.photo, 0.8f, 0.2f, d->grad_color, d->loss,
+                                   st.s)) != hipSuccess)
+            return step_fail(DS_ERR_HIP, "L1 + SSIM loss");
+        if ((rc = enqueue_shade(d, st, 0, d->grad_color)) != DS_OK) return rc;
+    } else {
+        if ((rc = enqueue_shade(d, st, 0, nullptr)) != DS_OK) return rc;
+    }
+    if ((e = mark(d, st.s, DS_STAGE_SHADE, false)) != hipSuccess) return step_fail(DS_ERR_HIP, "event");
+    return enqueue_backward(d, st, d->grad_color, d->loss_kind != DS_LOSS_L1_SSIM);
+}
+
+int ds_step_forward(const ds_step_desc* d, void* stream) {
+    using namespace ds;
+    Step st;
+    int rc = step_prepare(d, stream, st);
+    if (rc != DS_OK) return rc;
+    if ((rc = enqueue_forward(d, st)) != DS_OK) return rc;
+    return enqueue_shade(d, st, 1, nullptr);
+}
+
+int ds_step_backward(const ds_step_desc* d, const float* grad_color, void* stream) {
+    using namespace ds;
+    if (!grad_color) return step_fail(DS_ERR_INVALID, "null grad_color");
+    if (reinterpret_cast<uintptr_t>(grad_color) & 3u) return step_fail(DS_ERR_INVALID, "grad_color misaligned");
+    Step st;
+    int rc = step_prepare(d, stream, st);
+    if (rc != DS_OK) return rc;
+    if ((rc = enqueue_shade(d, st, 0, grad_color)) != DS_OK) return rc;
+    return enqueue_backward(d, st, grad_color, false);
 }
 
 int ds_autotune_step(ds_step_desc* d, const int* tiles, int n_tiles, int reps, float* ms, void* stream) {

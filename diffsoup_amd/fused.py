@@ -47,6 +47,8 @@ class StepDesc(ctypes.Structure):
         ("timed_stage", ctypes.c_int), ("ev_begin", ctypes.c_void_p), ("ev_end", ctypes.c_void_p),
         ("seed_device", ctypes.c_void_p), ("feat_bf16", ctypes.c_void_p),
         ("loss_kind", ctypes.c_int), ("advance_seed", ctypes.c_int),
+        ("accumulate", ctypes.c_int), ("alpha_lo", ctypes.c_int), ("sh2", ctypes.c_int),
+        ("inv_mvp", ctypes.c_void_p),
     ]
 
 
@@ -58,6 +60,9 @@ STAGES = {"prep": 1, "blobs": 2, "project": 3, "bin": 4, "resolve": 5, "shade": 
 
 
 _lib.register("ds_step_mlp_params", [ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int64)
+_lib.register("ds_step_mlp_params_sh2", [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int64)
+_lib.register("ds_step_forward", [ctypes.POINTER(StepDesc), ctypes.c_void_p], ctypes.c_int)
+_lib.register("ds_step_backward", [ctypes.POINTER(StepDesc), ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int)
 _lib.register("ds_step_workspace_size", [ctypes.POINTER(StepDesc)], ctypes.c_size_t)
 _lib.register("ds_train_step", [ctypes.POINTER(StepDesc), ctypes.c_void_p], ctypes.c_int)
 _lib.register("ds_autotune_step", [ctypes.POINTER(StepDesc), ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
@@ -97,11 +102,26 @@ class SoupParams:
 
 
 class FusedStep:
-    """Pre-allocated buffers and workspace for ds_train_step on one set of views."""
+    """Pre-allocated buffers and workspace for ds_train_step on one set of views.
+
+    Two feature compositions:
+
+    * default - the features of ``levels`` (lo, hi) are concatenated into the
+      MLP input, opacities are sigmoid(alpha_logits) at ``alpha_level``;
+    * ``accumulate=True`` - the reference's own training iteration
+      (examples/02_synthetic.py:354-364): ``feat`` [T, S(lo..hi), C] and
+      ``alpha_logits`` [T, S(alpha_lo..alpha_level)] are multires sources,
+      the step evaluates sigmoid(accumulate_to_level(...)) of both at the
+      top level, gathers the C features there, and the gradients are those
+      of the sources.
+
+    ``sh2=True`` appends the 9 SH2 values of each pixel's view direction to
+    the MLP input (``run`` then takes ``inv_mvp``); the reference's network
+    is ``accumulate=True, channels=7, hidden=16, sh2=True``."""
 
     def __init__(self, resolutions, T, V, alpha_level, levels, channels, hidden, *, bf16=False,
                  bf16_features=False, stochastic=True, tile=16, background=1.0, device="cuda", capacity=None,
-                 loss="mse"):
+                 loss="mse", accumulate=False, alpha_lo=None, sh2=False):
         self.dev = torch.device(device)
         self.res = [(int(h), int(w)) for h, w in resolutions]
         self.B = len(self.res)
@@ -115,11 +135,17 @@ class FusedStep:
         if loss not in LOSS_KINDS:
             raise ValueError(f"loss must be one of {sorted(LOSS_KINDS)}")
         self.loss_kind = loss
+        self.accumulate, self.sh2 = bool(accumulate), bool(sh2)
+        self.alpha_lo = alpha_level if alpha_lo is None else int(alpha_lo)
+        if self.accumulate and self.bf16_features:
+            raise ValueError("accumulated features are fp32: bf16_features needs accumulate=False")
         self.Sa = feats_at_level(alpha_level)
+        self.Sa_src = feats_in_levels(self.alpha_lo, alpha_level) if self.accumulate else self.Sa
         self.S_tot = feats_in_levels(self.lo, self.hi)
-        self.nl = self.hi - self.lo + 1
+        self.nl = 1 if self.accumulate else self.hi - self.lo + 1
+        self.din = self.nl * channels + (9 if self.sh2 else 0)
         L = _lib.lib()
-        self.nparam = int(L.ds_step_mlp_params(self.nl, channels, hidden))
+        self.nparam = int(L.ds_step_mlp_params_sh2(self.nl, channels, hidden, int(self.sh2)))
         f = lambda *shape: torch.empty(*shape, dtype=torch.float32, device=self.dev)
         self.pos = f(self.B, self.V, 4)
         self.alpha = f(self.T, self.Sa)
@@ -128,7 +154,7 @@ class FusedStep:
         self.grad_color = f(self.P, 3)
         self.pix_loss = f(self.P)
         self.grad_verts = f(self.V, 3)
-        self.grad_alpha_logits = f(self.T, self.Sa)
+        self.grad_alpha_logits = f(self.T, self.Sa_src)
         self.grad_feat = f(self.T, self.S_tot, self.C)
         self.grad_mlp = f(self.nparam)
         self.loss = f(1)
@@ -144,6 +170,7 @@ class FusedStep:
         self.desc.workspace = (self._ws.data_ptr() + 255) & ~255
         self.desc.workspace_bytes = nbytes
         self._captured = False   # set by CapturedStep: workspace and tile are then fixed
+        self._generation = 0     # forwards run through render()
 
     def _fill_static(self):
         d = self.desc
@@ -155,6 +182,7 @@ class FusedStep:
         d.hidden, d.bf16 = self.hidden, int(bool(self.bf16))
         d.background = self.background
         d.loss_kind = LOSS_KINDS[self.loss_kind]
+        d.accumulate, d.alpha_lo, d.sh2 = int(self.accumulate), self.alpha_lo, int(self.sh2)
         for name in ("pos", "alpha", "rast", "color", "grad_color", "pix_loss", "grad_verts",
                      "grad_alpha_logits", "grad_feat", "grad_mlp", "loss"):
             setattr(d, name, getattr(self, name).data_ptr())
@@ -162,7 +190,7 @@ class FusedStep:
 
     def mlp_grads(self):
         """Views of grad_mlp as (dW1, db1, dW2, db2, dW3, db3)."""
-        DH, DIN = self.hidden, self.nl * self.C
+        DH, DIN = self.hidden, self.din
         g = self.grad_mlp
         o = 0
         dW1 = g[o:o + DH * DIN].view(DH, DIN); o += DH * DIN
@@ -173,10 +201,21 @@ class FusedStep:
         db3 = g[o:o + 3]
         return dW1, db1, dW2, db2, dW3, db3
 
-    def _bind(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int) -> None:
+    def _bind(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int,
+              inv_mvp: torch.Tensor | None = None) -> None:
         assert mvp.shape == (self.B, 4, 4) and target.shape == (self.P, 3)
         for t in params.tensors() + [mvp, target]:
             assert t.is_contiguous() and t.dtype == torch.float32 and t.device == self.dev
+        assert params.feat.shape == (self.T, self.S_tot, self.C), "feat shape"
+        assert params.alpha_logits.shape == (self.T, self.Sa_src), "alpha_logits shape"
+        if self.sh2:
+            if inv_mvp is None:
+                raise ValueError("sh2: pass inv_mvp [B,4,4]")
+            assert inv_mvp.shape == (self.B, 4, 4) and inv_mvp.dtype == torch.float32 and inv_mvp.is_contiguous()
+            assert inv_mvp.device == self.dev
+            self.desc.inv_mvp = inv_mvp.data_ptr()
+        else:
+            self.desc.inv_mvp = None
         d = self.desc
         d.verts, d.mvp, d.tri = params.verts.data_ptr(), mvp.data_ptr(), params.tri.data_ptr()
         d.alpha_logits, d.feat = params.alpha_logits.data_ptr(), params.feat.data_ptr()
@@ -193,11 +232,31 @@ class FusedStep:
         else:
             d.feat_bf16 = None
 
-    def run(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int) -> torch.Tensor:
+    def run(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int,
+            inv_mvp: torch.Tensor | None = None) -> torch.Tensor:
         """Enqueue one forward + backward; returns the device loss tensor."""
-        self._bind(params, mvp, target, seed)
+        self._bind(params, mvp, target, seed, inv_mvp)
         _lib.check(_lib.lib().ds_train_step(ctypes.byref(self.desc), _lib.stream_ptr(self.dev)), "ds_train_step")
         return self.loss
+
+    def forward(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int,
+                inv_mvp: torch.Tensor | None = None) -> torch.Tensor:
+        """Enqueue the forward up to the colour (ds_step_forward); returns the
+        step's colour buffer [P, 3] (the target is read only by the opacity
+        term of the backward)."""
+        self._bind(params, mvp, target, seed, inv_mvp)
+        _lib.check(_lib.lib().ds_step_forward(ctypes.byref(self.desc), _lib.stream_ptr(self.dev)),
+                   "ds_step_forward")
+        return self.color
+
+    def backward(self, grad_color: torch.Tensor) -> None:
+        """Enqueue the backward of the last ``forward`` on dLoss/dcolor [P, 3]
+        (ds_step_backward): every parameter gradient lands in the step's
+        gradient buffers."""
+        assert grad_color.shape == (self.P, 3) and grad_color.dtype == torch.float32
+        assert grad_color.is_contiguous() and grad_color.device == self.dev
+        _lib.check(_lib.lib().ds_step_backward(ctypes.byref(self.desc), grad_color.data_ptr(),
+                                               _lib.stream_ptr(self.dev)), "ds_step_backward")
 
     def workspace_bytes(self, tile: int) -> int:
         d = self.desc
@@ -232,10 +291,12 @@ class FusedStep:
         name = torch.cuda.get_device_name(self.dev).replace(" ", "_")
         res = ",".join(f"{h}x{w}" for h, w in sorted(self.res))
         return (f"{name}|B{self.B}|{res}|T{self.T}|V{self.V}|a{self.alpha_level}|l{self.lo}-{self.hi}|C{self.C}"
-                f"|h{self.hidden}|{'bf16' if self.bf16 else 'fp32'}|{'f16feat' if self.bf16_features else 'f32feat'}")
+                f"|h{self.hidden}|{'bf16' if self.bf16 else 'fp32'}|{'f16feat' if self.bf16_features else 'f32feat'}"
+                f"|{'acc' if self.accumulate else 'concat'}{self.alpha_lo}|{'sh2' if self.sh2 else 'nosh'}")
 
     def autotune(self, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, tiles=(8, 16, 32),
-                 reps: int = 5, seed: int = 0, cache: str | None = "default") -> dict:
+                 reps: int = 5, seed: int = 0, cache: str | None = "default",
+                 inv_mvp: torch.Tensor | None = None) -> dict:
         """Pick the screen tile edge by timing whole steps (ds_autotune_step),
         and remember the choice in a JSON cache keyed by ``tune_key()``
         (``cache="default"``: $DIFFSOUP_AMD_TUNE_CACHE or
@@ -265,7 +326,7 @@ class FusedStep:
             self.set_tile(int(table[key]))
             return {int(table[key]): None}
         self._ensure_workspace(max(self.workspace_bytes(t) for t in tiles))
-        self._bind(params, mvp, target, seed)
+        self._bind(params, mvp, target, seed, inv_mvp)
         arr = (ctypes.c_int * len(tiles))(*tiles)
         ms = (ctypes.c_float * len(tiles))()
         _lib.check(_lib.lib().ds_autotune_step(ctypes.byref(self.desc), arr, len(tiles), int(reps), ms,
@@ -353,7 +414,7 @@ class FusedStep:
         """
         if stage != "shade":
             raise ValueError(stage)
-        din, h = self.nl * self.C, self.hidden
+        din, h = self.din, self.hidden
         per_px = 6.0 * (din * h + h * h + 3 * h) + 4.0 * 3 * din
         return per_px * self.covered_pixels()
 
@@ -363,8 +424,51 @@ class FusedStep:
         return self._ws[base:base + 64].view(torch.int64)
 
 
+class _RenderFn(torch.autograd.Function):
+    """The fused forward up to the colour and its backward (the two-pass
+    shading around any caller-side loss)."""
+
+    @staticmethod
+    def forward(ctx, step, tri, mvp, target, seed, inv_mvp, verts, alpha_logits, feat, w1, b1, w2, b2, w3, b3):
+        params = SoupParams(verts.detach(), tri, alpha_logits.detach(), feat.detach(), w1.detach(), b1.detach(),
+                            w2.detach(), b2.detach(), w3.detach(), b3.detach())
+        color = step.forward(params, mvp, target, seed, inv_mvp).clone()
+        step._generation += 1
+        ctx.step, ctx.generation = step, step._generation
+        return color
+
+    @staticmethod
+    def backward(ctx, grad_color):
+        step = ctx.step
+        if step._generation != ctx.generation:
+            raise RuntimeError("render: the FusedStep ran another forward before this backward "
+                               "(one forward in flight per step)")
+        step.backward(grad_color.contiguous())
+        dW1, db1, dW2, db2, dW3, db3 = (g.clone() for g in step.mlp_grads())
+        return (None, None, None, None, None, None, step.grad_verts.clone(), step.grad_alpha_logits.clone(),
+                step.grad_feat.clone(), dW1, db1, dW2, db2, dW3, db3)
+
+
+def render(step: FusedStep, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor, seed: int,
+           inv_mvp: torch.Tensor | None = None) -> torch.Tensor:
+    """Differentiable colour [P, 3] (views packed) of the fused step, for a
+    loss composed by the caller: ``loss_fn(render(...)).backward()`` fills
+    ``.grad`` of params.verts, alpha_logits, feat and the MLP tensors.
+
+    The forward is ds_step_forward (projection, stochastic-opacity raster,
+    gather, MLP, blend, white background); the backward is ds_step_backward
+    on the incoming colour gradient: MLP adjoint, feature scatter, edge
+    gradients (the reference's edge_grad) and the opacity term of its
+    opacity_aux_loss (weight 1, per-pixel L1 against ``target``), as in the
+    reference's composition (examples/02_synthetic.py:354-385) whose loss
+    adds ``aux_loss``.  One forward in flight per step: call backward before
+    the step's next forward."""
+    return _RenderFn.apply(step, params.tri, mvp, target, seed, inv_mvp, params.verts, params.alpha_logits,
+                           params.feat, params.w1, params.b1, params.w2, params.b2, params.w3, params.b3)
+
+
 def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5e-2, lr_verts=1e-3,
-                   lr_mlp=1e-2, out: SoupParams | None = None):
+                   lr_mlp=1e-2, out: SoupParams | None = None, decay=None):
     """The reference's optimisers for a soup (examples/02_synthetic.py:310-315):
     Adam on features and opacity logits (lr 5e-2) and on the MLP (1e-2),
     VectorAdam on vertex positions (1e-3), reading the gradients that
@@ -372,7 +476,10 @@ def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5
     ``out`` the updated parameters are written there and ``params`` stays
     unchanged (every iteration then sees the same scene, as a throughput
     measurement of a fixed configuration needs).  When the parameters carry a
-    bf16 feature copy, the update refreshes it in the same launch."""
+    bf16 feature copy, the update refreshes it in the same launch.  ``decay``
+    = (steps, final) applies the reference's learning-rate schedule
+    exp_decay_mult (examples/02_synthetic.py:341-349) on the device, so a
+    captured iteration follows it replay after replay."""
     from .optimize import FusedAdam
 
     o = out
@@ -384,7 +491,7 @@ def soup_optimizer(step: FusedStep, params: SoupParams, lr_feat=5e-2, lr_alpha=5
     dW1, db1, dW2, db2, dW3, db3 = step.mlp_grads()
     for name, g in (("w1", dW1), ("b1", db1), ("w2", dW2), ("b2", db2), ("w3", dW3), ("b3", db3)):
         groups.append(dict(param=getattr(params, name), grad=g, lr=lr_mlp, param_out=o and getattr(o, name)))
-    return FusedAdam(groups)
+    return FusedAdam(groups, decay=decay)
 
 
 class CapturedStep:
@@ -400,13 +507,13 @@ class CapturedStep:
     """
 
     def __init__(self, step: FusedStep, params: SoupParams, mvp: torch.Tensor, target: torch.Tensor,
-                 optimizer=None, seed: int = 0):
+                 optimizer=None, seed: int = 0, inv_mvp: torch.Tensor | None = None):
         self.step = step
         self.seed_dev = torch.zeros(1, dtype=torch.int64, device=step.dev)
         # one eager step first: it loads the code objects and sets kernel
         # attributes, which must not happen during capture; it writes only
         # the step's own outputs, not the parameters
-        step.run(params, mvp, target, seed)
+        step.run(params, mvp, target, seed, inv_mvp)
         torch.cuda.synchronize(step.dev)
         step.desc.seed_device = self.seed_dev.data_ptr()
         step.desc.advance_seed = 1
@@ -415,7 +522,7 @@ class CapturedStep:
         step._captured = True
         try:
             with torch.cuda.graph(self.graph):
-                step.run(params, mvp, target, 0)
+                step.run(params, mvp, target, 0, inv_mvp)
                 if optimizer is not None:
                     optimizer.step()
         finally:

@@ -52,7 +52,8 @@ class AdamGroup(ctypes.Structure):
     _fields_ = [("param", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p),
                 ("exp_avg_sq", ctypes.c_void_p), ("n", ctypes.c_int64), ("vector_dim", ctypes.c_int),
                 ("lr", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
-                ("eps", ctypes.c_double), ("param_out", ctypes.c_void_p), ("param_bf16_out", ctypes.c_void_p)]
+                ("eps", ctypes.c_double), ("param_out", ctypes.c_void_p), ("param_bf16_out", ctypes.c_void_p),
+                ("lr_mult", ctypes.c_void_p), ("decay_steps", ctypes.c_int64), ("decay_final", ctypes.c_double)]
 
 
 _lib.register("ds_adam_step", [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -73,11 +74,18 @@ class FusedAdam:
     rounded to bf16).  Gradients are read from ``grad`` (for example
     the buffers a fused training step writes), not from ``param.grad``.  The step
     counter lives in device memory, so a captured graph advances it.
-    Hyper-parameters are baked into each launch: change them with
-    ``set_lr`` and re-capture any graph that contains the step.
+
+    Learning-rate schedules that a captured graph follows: ``lr_mult``
+    (a one-element fp32 device tensor shared by every group, read when the
+    update runs: write it with ``set_lr_mult`` between replays) and
+    ``decay=(steps, final)``, the reference's exp_decay_mult
+    (examples/utils.py:435-442): iteration i (= the device step counter + 1)
+    uses lr * final ** (min(max(i, 1), steps) / steps), formed on the device.
+    The base ``lr`` itself is baked into each launch (``set_lr`` needs a
+    re-capture).
     """
 
-    def __init__(self, groups):
+    def __init__(self, groups, lr_mult=False, decay=None):
         if not 1 <= len(groups) <= 16:
             raise ValueError("FusedAdam: 1..16 parameter groups")
         self.groups = []
@@ -112,6 +120,14 @@ class FusedAdam:
                                     eps=float(g.get("eps", 1e-8))))
         self.device = self.groups[0]["param"].device
         self.step_count = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.lr_mult = torch.ones(1, dtype=torch.float32, device=self.device) if lr_mult else None
+        if decay is not None:
+            steps, final = int(decay[0]), float(decay[1])
+            if steps < 1 or not final > 0:
+                raise ValueError("FusedAdam: decay=(steps >= 1, final > 0)")
+            self.decay = (steps, final)
+        else:
+            self.decay = None
         self._build()
 
     def _build(self):
@@ -123,7 +139,16 @@ class FusedAdam:
             a.lr, (a.beta1, a.beta2), a.eps = g["lr"], g["betas"], g["eps"]
             a.param_out = None if g["param_out"] is None else g["param_out"].data_ptr()
             a.param_bf16_out = None if g["param_bf16_out"] is None else g["param_bf16_out"].data_ptr()
+            a.lr_mult = None if self.lr_mult is None else self.lr_mult.data_ptr()
+            a.decay_steps, a.decay_final = self.decay if self.decay is not None else (0, 0.0)
         self._arr = arr
+
+    def set_lr_mult(self, mult):
+        """Store the learning-rate multiplier the next update reads (a fill
+        kernel on the current stream; outside any captured graph)."""
+        if self.lr_mult is None:
+            raise RuntimeError("FusedAdam was built without lr_mult")
+        self.lr_mult.fill_(float(mult))
 
     def set_lr(self, index, lr):
         self.groups[index]["lr"] = float(lr)

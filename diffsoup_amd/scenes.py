@@ -91,6 +91,13 @@ class Soup:
     mlp: list = field(default_factory=list)   # [(W [out,in], b [out])] f32
     feature_mode: str = "concat"
     name: str = ""
+    # the reference's own composition (examples/02_synthetic.py:354-364):
+    # feat / alpha_logits are multires sources over levels / (alpha_lo,
+    # alpha_level), accumulated to the top level and passed through a sigmoid
+    accumulate: bool = False
+    alpha_lo: int = 0
+    sh2: bool = False                          # SH2 of the view direction after the features
+    inv_mvp: np.ndarray | None = None          # [B,4,4] f32 (sh2)
 
     @property
     def T(self):
@@ -274,6 +281,36 @@ def stress_soup(seed=0, **kw) -> Soup:
             "C": 16, "alpha_level": 4, "dh": 64, "layered": 0.4}
     args.update(kw)
     return unbounded_soup(seed, **args)
+
+
+def reference_training_soup(seed=0, T=3000, views=2, res=(192, 224), levels=(0, 0), alpha_levels=(0, 0),
+                            C=7, dh=16, src_std=0.5, elevation_deg=20.0) -> Soup:
+    """The reference's training configuration (examples/02_synthetic.py:
+    288-306): feat_dim 7 features and opacities as multires sources over
+    ``levels`` / ``alpha_levels`` (the driver starts at levels (0, 0) and
+    lifts to (2, 5)), accumulated to the top level and passed through a
+    sigmoid, the 9 SH2 values of the view direction after the features, and
+    ColorMLP(16, 16, n_layers=2) (kaiming hidden layers, xavier last, zero
+    biases).  The sources are drawn N(0, src_std) instead of the driver's
+    zeros so that every term is exercised; triangles as in small_random_soup
+    (vertices within 0.1 of uniform centres in [-1,1]^3), views on a
+    radius-3 orbit at fovy 50, L1 + SSIM target U[0,1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    centres = rng.uniform(-1, 1, size=(T, 1, 3))
+    tris = centres + rng.uniform(-0.1, 0.1, size=(T, 3, 3))
+    verts, tri = _soup_from_tris(tris)
+    lo, hi = levels
+    alo, ahi = alpha_levels
+    S = sum(feats_at_level(l) for l in range(lo, hi + 1))
+    Sa = sum(feats_at_level(l) for l in range(alo, ahi + 1))
+    logits = rng.normal(0, 1, size=(T, Sa)).astype(np.float32)
+    feat = rng.normal(0, src_std, size=(T, S, C)).astype(np.float32)
+    mvp = orbit_cameras(views, 3.0, [res] * views, fovy=50.0, near=0.1, far=10.0, elevation_deg=elevation_deg)
+    inv_mvp = np.linalg.inv(mvp.astype(np.float64)).astype(np.float32)
+    target = rng.uniform(0, 1, size=(views * res[0] * res[1], 3)).astype(np.float32)
+    return Soup(verts, tri, logits, ahi, feat, (lo, hi), mvp, [res] * views, target, (C + 9, dh, 3),
+                _mlp(rng, C + 9, dh, 3, init="kaiming"), "accumulate", f"reference_training_{T}",
+                accumulate=True, alpha_lo=alo, sh2=True, inv_mvp=inv_mvp)
 
 
 def project_vertices_np(verts, mvp):

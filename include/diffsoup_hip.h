@@ -241,6 +241,27 @@ typedef struct ds_step_desc {
                                     *seed_device after its last read, so that
                                     replays of a captured step draw a new mask
                                     each without a launch of their own */
+    int accumulate;              /* 0: the features of levels feat_lo..feat_hi
+                                    are concatenated into the MLP input and the
+                                    opacities are sigmoid(alpha_logits) at
+                                    alpha_level.  1: the reference's own
+                                    composition (examples/02_synthetic.py:
+                                    354-364): feat [T, S_tot(feat_lo..feat_hi),
+                                    channels] and alpha_logits [T, S_tot(
+                                    alpha_lo..alpha_level)] are the multires
+                                    sources; the step evaluates
+                                    sigmoid(accumulate_to_level(...)) of both at
+                                    feat_hi / alpha_level, gathers the features
+                                    at level feat_hi (channels inputs), and
+                                    grad_feat / grad_alpha_logits are the
+                                    gradients of the sources; alpha then holds
+                                    the accumulated opacities [T, S_a] */
+    int alpha_lo;                /* with accumulate: first opacity level      */
+    int sh2;                     /* append the 9 SH2 values of the pixel's view
+                                    direction (encode_view_dir_sh2,
+                                    src/cuda/encoding.cu:9-130) to the MLP
+                                    input after the features                  */
+    const float* inv_mvp;        /* with sh2: [B,4,4] row-major inverse MVPs  */
 } ds_step_desc;
 
 #define DS_LOSS_MSE 0
@@ -257,8 +278,10 @@ typedef struct ds_step_desc {
 #define DS_STAGE_AUX 8      /* opacity auxiliary gradient                    */
 #define DS_STAGE_FINAL 9    /* reductions, loss, vertex / logit gradients    */
 
-/* Length of the grad_mlp vector for a configuration. */
+/* Length of the grad_mlp vector for a configuration (levels concatenated,
+ * no SH2 inputs; with sh2 != 0, 9 SH2 inputs after the features). */
 int64_t ds_step_mlp_params(int levels, int channels, int hidden);
+int64_t ds_step_mlp_params_sh2(int levels, int channels, int hidden, int sh2);
 
 /* Workspace bytes for ds_train_step; 0 if the configuration is invalid or
  * has no compiled shading kernel. */
@@ -266,6 +289,19 @@ size_t ds_step_workspace_size(const ds_step_desc* d);
 
 /* Enqueue one fused forward + backward step (see above). */
 int ds_train_step(const ds_step_desc* d, void* stream);
+
+/* The step split at the colour, for a caller whose loss is its own (the
+ * operator behind a torch.autograd.Function): ds_step_forward enqueues the
+ * forward up to the colour (color [P,3], pix_loss, rast, alpha, pos; the
+ * descriptor's loss_kind is ignored); ds_step_backward, on the same
+ * descriptor and workspace after it, takes dLoss/dcolor [P,3] (device) and
+ * enqueues the MLP adjoint, the feature scatter, the edge and opacity
+ * gradients (the opacity term of the reference's opacity_aux_loss, weight 1,
+ * python/diffsoup/rasterize.py:139-252, included) and every parameter
+ * gradient; it does not write loss.  Replaces the autograd backward of the
+ * reference's composed ops (examples/02_synthetic.py:354-385). */
+int ds_step_forward(const ds_step_desc* d, void* stream);
+int ds_step_backward(const ds_step_desc* d, const float* grad_color, void* stream);
 
 /* Tile autotuner: for each candidate tile edge (8, 16 or 32; at most 8
  * candidates) whose workspace fits d->workspace_bytes, runs one warm-up step
@@ -362,6 +398,18 @@ typedef struct ds_adam_group {
     uint16_t* param_bf16_out; /* if non-null, also write the updated values
                          rounded to bf16 (nearest even): the copy a step
                          with feat_bf16 reads                             */
+    const float* lr_mult; /* if non-null, a device float read when the update
+                         runs; the learning rate is multiplied by it (a
+                         captured graph picks up whatever the caller last
+                         stored there)                                      */
+    int64_t decay_steps; /* if > 0, the learning rate is also multiplied by
+                         decay_final ^ (i / decay_steps), i = *step + 1
+                         clamped to [1, decay_steps]: the reference's
+                         exp_decay_mult(i, decay_steps, decay_final)
+                         (examples/utils.py:435-442) at iteration i, formed
+                         on the device from the step counter, so replays
+                         follow the schedule with no host involvement      */
+    double decay_final;
 } ds_adam_group;
 
 /* At most 16 groups. */

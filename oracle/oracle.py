@@ -381,20 +381,41 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
     dtype = dtype or torch.float64
     s = soup
     B = s.B
+    acc = getattr(s, "accumulate", False)
+    sh = getattr(s, "sh2", False)
     res = [tuple(r) for r in s.resolutions]
     off = np.cumsum([0] + [h * w for h, w in res])
+    lo, hi = s.levels
     if pos is None:
         pos = project(s.verts, s.mvp)
-    if alpha is None:
+    if acc:
+        # the reference's composition: sigmoid(accumulate_to_level(...)) of the
+        # multires sources, rounded once from double like the step
+        # (examples/02_synthetic.py:355-356)
+        a_src = s.alpha_logits.reshape(s.T, -1, 1)
+        a_acc = accumulate_fwd(s.alpha_lo, s.alpha_level, s.alpha_level, a_src).reshape(s.T, -1)
+        if alpha is None:
+            alpha = (1.0 / (1.0 + np.exp(-a_acc.astype(np.float64)))).astype(np.float32)
+        f_acc = accumulate_fwd(lo, hi, hi, s.feat)
+        feat_tab = (1.0 / (1.0 + np.exp(-f_acc.astype(np.float64)))).astype(np.float32)
+    elif alpha is None:
         alpha = (1.0 / (1.0 + np.exp(-s.alpha_logits.astype(np.float64)))).astype(np.float32)
     # views one at a time (any mix of resolutions), packed like the step's buffers
     rast = np.concatenate([rasterize((H, W), pos[b:b + 1], s.tri, s.alpha_level, alpha, True, seed,
                                      view_base=b).reshape(-1, 4) for b, (H, W) in enumerate(res)])
-    lo, hi = s.levels
     P = rast.shape[0]
     covered = rast[:, 3] > 0
     rast_cov = np.ascontiguousarray(rast[covered])   # background gathers are zero and carry no gradient
-    X = multires_color(rast_cov, lo, hi, s.feat, True).reshape(rast_cov.shape[0], -1)
+    if acc:
+        X = multires_color(rast_cov, hi, hi, feat_tab, False).reshape(rast_cov.shape[0], -1)
+    else:
+        X = multires_color(rast_cov, lo, hi, s.feat, True).reshape(rast_cov.shape[0], -1)
+    DF = X.shape[1]
+    if sh:
+        # encode_view_dir_sh2 per view (src/cuda/encoding.cu:89-130), concatenated after the features
+        enc = np.concatenate([sh2(rast[off[b]:off[b + 1]].reshape(1, H, W, 4), s.inv_mvp[b:b + 1]).reshape(-1, 9)
+                              for b, (H, W) in enumerate(res)])
+        X = np.concatenate([X, enc[covered]], axis=1)
     idx = torch.from_numpy(np.nonzero(covered)[0])
     Xt = torch.tensor(X, dtype=dtype, requires_grad=True)
     Ws = [(torch.tensor(w, dtype=dtype, requires_grad=True), torch.tensor(b, dtype=dtype, requires_grad=True))
@@ -417,7 +438,12 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
         loss = 0.8 * (tgt - color).abs().mean() + 0.2 * 0.5 * (1 - torch.stack(views).mean())
         gcol = torch.autograd.grad(loss, color, retain_graph=True)[0]
     loss.backward()
-    grad_feat = multires_color_bwd(rast_cov, lo, hi, s.feat.shape, Xt.grad.numpy().astype(np.float32), True)
+    dXf = np.ascontiguousarray(Xt.grad.numpy()[:, :DF].astype(np.float32))
+    if acc:
+        g_acc = multires_color_bwd(rast_cov, hi, hi, feat_tab.shape, dXf, False)
+        grad_feat = accumulate_bwd(lo, hi, hi, s.feat.shape, g_acc * (feat_tab * (1.0 - feat_tab)))
+    else:
+        grad_feat = multires_color_bwd(rast_cov, lo, hi, s.feat.shape, dXf, True)
     color32 = color.detach().numpy().astype(np.float32)
     if gcol is not None:
         grad_color32 = gcol.numpy().astype(np.float32)
@@ -435,6 +461,9 @@ def reference_step(soup, seed, dtype=None, pos=None, alpha=None, color_in=None, 
                                s.tri, s.alpha_level, alpha)
     gverts = np.einsum("bij,bvi->vj", s.mvp.astype(np.float64), gpos.astype(np.float64))[:, :3]
     galog = ga / (3 * P) * alpha * (1 - alpha)
+    if acc:
+        galog = accumulate_bwd(s.alpha_lo, s.alpha_level, s.alpha_level, (s.T, s.alpha_logits.shape[1], 1),
+                               galog.reshape(s.T, -1, 1)).reshape(s.T, -1)
     gmlp = np.concatenate([Ws[0][0].grad.numpy().ravel(), Ws[1][0].grad.numpy().ravel(),
                            Ws[2][0].grad.numpy().ravel(), Ws[0][1].grad.numpy(), Ws[1][1].grad.numpy(),
                            Ws[2][1].grad.numpy()])

@@ -380,3 +380,42 @@ def test_edge_grad_against_finite_differences(name):
     else:
         assert np.all(want[:, 2] == 0) and np.all(est[:, 2] == 0)    # depth moves no pixel here
     np.testing.assert_allclose(est, want, rtol=0, atol=F.TOLERANCE[name] * np.abs(want).max())
+
+
+@pytest.mark.parametrize("levels", [(0, 0), (1, 3)])
+def test_reference_composition_gradients_match_central_differences(levels):
+    """The oracle's reference-composition step (sigmoid of accumulated
+    multires features and opacities, single-level gather, SH2 inputs, the
+    16 -> 16 -> 16 -> 3 ColorMLP, MSE) against float64 central differences of
+    its own loss: feature-source and MLP gradients (the raster does not
+    depend on either, so the loss is smooth in them)."""
+    import dataclasses
+
+    from diffsoup_amd import scenes
+    s = scenes.reference_training_soup(4, T=400, views=1, res=(48, 56), levels=levels, alpha_levels=levels)
+    r = O.reference_step(s, 9)
+    cov = r["covered"]
+    assert cov.sum() > 100
+    rng = np.random.default_rng(1)
+    tris = np.unique(r["rast"][cov, 3].astype(int) - 1)
+    eps = 1e-3
+
+    def loss_with(**kw):
+        return O.reference_step(dataclasses.replace(s, **kw), 9, pos=r["pos"], alpha=r["alpha"])["loss"]
+
+    for _ in range(5):
+        t, k, c = int(rng.choice(tris)), int(rng.integers(0, s.feat.shape[1])), int(rng.integers(0, 7))
+        fp, fm = s.feat.copy(), s.feat.copy()
+        fp[t, k, c] += eps
+        fm[t, k, c] -= eps
+        fd = (loss_with(feat=fp) - loss_with(feat=fm)) / (2 * eps)
+        got = r["grad_feat"][t, k, c]
+        assert abs(got - fd) <= 5e-3 * abs(fd) + 2e-9, (t, k, c, got, fd)
+    for _ in range(4):   # W1 entries, including the SH2 columns 7..15
+        i, j = int(rng.integers(0, 16)), int(rng.integers(0, 16))
+        mp = [(w.copy(), b.copy()) for w, b in s.mlp]
+        mm = [(w.copy(), b.copy()) for w, b in s.mlp]
+        mp[0][0][i, j] += eps
+        mm[0][0][i, j] -= eps
+        fd = (loss_with(mlp=mp) - loss_with(mlp=mm)) / (2 * eps)
+        assert abs(r["grad_mlp"][16 * i + j] - fd) <= 5e-3 * abs(fd) + 2e-9, (i, j, r["grad_mlp"][16 * i + j], fd)
