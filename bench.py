@@ -327,7 +327,18 @@ WORKLOADS = {
     "stress300k": ("stress_soup", True,
                    "300K triangles (40% in stacked depth layers), 64 views of mixed resolution up to 1920x1080, "
                    "bf16 features and MLP as bf16_100k"),
+    # the reference's own training iteration at its synthetic driver's size
+    "reference15k_l0": ("reference_sphere_torus_soup", False,
+                        "the reference's own iteration (examples/02_synthetic.py:354-388): 15K triangles, 4 views "
+                        "800x800, 7-channel features and opacities at levels 0-0 accumulated + sigmoid, SH2 inputs, "
+                        "MLP 16-16-16-3 fp32, L1 + SSIM, edge + opacity grads, Adam/VectorAdam with exp_decay_mult"),
+    "reference15k_l5": ("reference_sphere_torus_soup", False,
+                        "as reference15k_l0 after the driver's lift to levels 2-5 (561 samples per triangle at "
+                        "level 5)"),
 }
+
+# scene-generator keyword arguments of the workloads that take any
+WORKLOAD_ARGS = {"reference15k_l0": dict(levels=(0, 0)), "reference15k_l5": dict(levels=(2, 5))}
 
 
 def other_workload(args, dev):
@@ -346,20 +357,25 @@ def run_workload(name, args, dev):
 
     gen, bf16, desc = WORKLOADS[name]
     t0 = time.perf_counter()
-    soup = getattr(scenes, gen)(0)
+    soup = getattr(scenes, gen)(0, **WORKLOAD_ARGS.get(name, {}))
     gen_s = time.perf_counter() - t0
     mvp = torch.from_numpy(soup.mvp).to(dev)
     target = torch.from_numpy(soup.target).to(dev)
+    ref_iter = soup.accumulate
+    inv = torch.from_numpy(soup.inv_mvp).to(dev) if soup.sh2 else None
     step = fused.FusedStep(soup.resolutions, soup.T, soup.verts.shape[0], soup.alpha_level, soup.levels,
-                           soup.feat.shape[2], soup.mlp_dims[1], bf16=bf16, bf16_features=bf16, device=dev)
+                           soup.feat.shape[2], soup.mlp_dims[1], bf16=bf16, bf16_features=bf16, device=dev,
+                           loss="l1_ssim" if ref_iter else "mse", accumulate=soup.accumulate,
+                           alpha_lo=soup.alpha_lo, sh2=soup.sh2)
     params = fused.SoupParams.from_soup(soup, dev)
     updated = fused.SoupParams.from_soup(soup, dev)
     if bf16:
         params.with_bf16_features()
         updated.with_bf16_features()
-    tune = step.autotune(params, mvp, target, tiles=(8, 16, 32), reps=2, cache=None)
-    opt = fused.soup_optimizer(step, params, out=updated)
-    cap = fused.CapturedStep(step, params, mvp, target, opt, seed=1000)
+    tune = step.autotune(params, mvp, target, tiles=(8, 16, 32), reps=2, cache=None, inv_mvp=inv)
+    # the reference's schedule over its 10,000 steps (examples/02_synthetic.py:341-349)
+    opt = fused.soup_optimizer(step, params, out=updated, decay=(10_000, 0.01) if ref_iter else None)
+    cap = fused.CapturedStep(step, params, mvp, target, opt, seed=1000, inv_mvp=inv)
     steps, warmup = min(args.steps, 20), min(args.warmup, 3)
     for _ in range(warmup):
         cap.replay()
@@ -379,11 +395,16 @@ def run_workload(name, args, dev):
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
         for i in range(3):
             step.time_stage(stage, *evs[i])
-            step.run(params, mvp, target, seed=3000 + i)
+            step.run(params, mvp, target, seed=3000 + i, inv_mvp=inv)
         torch.cuda.synchronize()
         kms = sum(a.elapsed_time(b) for a, b in evs) / 3
         step.time_stage(None)
-        stages[stage] = roofline(step, stage, kms, {})
+        if ref_iter and stage == "shade":
+            # colour pass, L1 + SSIM kernels and adjoint pass: no single roof
+            stages[stage] = {"avg_launch_ms": round(kms, 5), "bound": "colour pass + L1/SSIM + adjoint pass",
+                             "achieved": None, "unit": None, "frac": None}
+        else:
+            stages[stage] = roofline(step, stage, kms, {})
     out = {"metric": f"fwd+bwd iters/s ({name})", "value": round(1000.0 / ms, 2), "unit": "it/s",
            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
            "dtype": "bf16" if bf16 else "f32", "data": "synthetic",

@@ -82,81 +82,181 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
 // target lattice (accumulate_to_level, python/diffsoup/multires.py:279-339,
 // src/cuda/multires.cu:314-389), then its sigmoid (examples/02_synthetic.py:
 // 355-356): dst [T, S_T, C].  The backward reads dst (the sigmoid) and the
-// gradient of dst and scatters scale * g * s (1 - s) onto src's lattices.
+// gradient of dst and writes scale * (transpose of the accumulation)(g s (1 - s))
+// into grad_src.
 struct AccumJob {
     const float* src;
     float* dst;
     const float* grad_dst;   // backward
-    float* grad_src;         // backward (zeroed, accumulated)
+    float* grad_src;         // backward (every row written)
     float scale;             // backward
     int lo, hi, target, C, S_tot, S_T;
 };
 
-__device__ __forceinline__ void accum_point(const AccumJob& J, long long lid, int& t, float& b0, float& b1) {
-    t = static_cast<int>(lid / J.S_T);
-    const uint32_t k = static_cast<uint32_t>(lid - (long long)t * J.S_T);
-    uint32_t x, y;
-    lattice_xy(k, x, y);
-    const float Rf = static_cast<float>(1u << J.target);
-    b0 = static_cast<float>(x) / Rf;
-    b1 = static_cast<float>(y) / Rf;
+// Copy n floats of a row into LDS with eight loads in flight per thread
+// before the first is waited on (a row is a few thousand floats, so one load
+// per iteration would leave each thread waiting out the memory latency
+// once per element).
+__device__ __forceinline__ void stage_row(float* dst, const float* __restrict__ src, int n) {
+    constexpr int U = 8;
+    const int nt = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + (U - 1) * nt < n; i += U * nt) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * nt];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[i + u * nt] = v[u];
+    }
+    for (; i < n; i += nt) dst[i] = src[i];
 }
 
-// Forward of both accumulations (features, then opacities), grid-stride over
-// the target samples of both; each sample's channels are summed in the order
-// of ds_accumulate_fwd and the sigmoid is evaluated in double and rounded once.
-__global__ void k_accum_sigmoid(int T, AccumJob f, AccumJob a) {
-    const long long nf = (long long)T * f.S_T, n = nf + (long long)T * a.S_T;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n; id += stride) {
-        const bool is_f = id < nf;
-        const AccumJob& J = is_f ? f : a;
-        const long long lid = is_f ? id : id - nf;
-        int t;
-        float b0, b1;
-        accum_point(J, lid, t, b0, b1);
-        const float* src = J.src + (long long)t * J.S_tot * J.C;
-        for (int j = 0; j < J.C; ++j) {
-            float v = 0.f;
+// Forward of both accumulations, one workgroup per triangle (grid-stride):
+// the triangle's source row is staged in LDS, then each thread takes target
+// samples, forms the sample's three corners at every level once and sums all
+// C channels in the order of ds_accumulate_fwd.  Opacities go through a
+// sigmoid evaluated in double and rounded once (their last bit decides mask
+// ties, as in the step's prologue); features through the fp32 sigmoid (the
+// reference applies torch's fp32 sigmoid).  C is a template argument so the
+// index arithmetic folds; CG is the generic runtime-width fallback.
+template <int C>
+__device__ __forceinline__ void accum_row_fwd(const AccumJob& J, int t, float* row, bool exact_sigmoid) {
+    const int c = C > 0 ? C : J.C;
+    const int n_src = J.S_tot * c, n_dst = J.S_T * c;
+    stage_row(row, J.src + (long long)t * n_src, n_src);
+    __syncthreads();
+    const float Rf = static_cast<float>(1u << J.target);
+    float* dst = J.dst + (long long)t * n_dst;
+    constexpr int CA = C > 0 ? C : 16;
+    for (int k = threadIdx.x; k < J.S_T; k += blockDim.x) {
+        uint32_t x, y;
+        lattice_xy(static_cast<uint32_t>(k), x, y);
+        const float b0 = static_cast<float>(x) / Rf, b1 = static_cast<float>(y) / Rf;
+        for (int j0 = 0; j0 < c; j0 += CA) {
+            const int nc = min(CA, c - j0);
+            float v[CA];
+#pragma unroll
+            for (int jj = 0; jj < CA; ++jj) v[jj] = 0.f;
             uint32_t off = 0;
             for (int l = J.lo; l <= J.hi; ++l) {
-                const Corners c = lattice_corners(b0, b1, static_cast<uint32_t>(l));
-                const float* fl = src + (long long)off * J.C;
-                v += c.w[0] * fl[c.idx[0] * J.C + j];
-                v += c.w[1] * fl[c.idx[1] * J.C + j];
-                v += c.w[2] * fl[c.idx[2] * J.C + j];
+                // one cell per level, all channels summed in the order of the
+                // single-channel loop: corner 0, 1, 2 of each level in turn
+                const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(l));
+                const float* f0 = row + (off + cr.idx[0]) * c + j0;
+                const float* f1 = row + (off + cr.idx[1]) * c + j0;
+                const float* f2 = row + (off + cr.idx[2]) * c + j0;
+#pragma unroll
+                for (int jj = 0; jj < CA; ++jj)
+                    if (C > 0 || jj < nc) {
+                        v[jj] += cr.w[0] * f0[jj];
+                        v[jj] += cr.w[1] * f1[jj];
+                        v[jj] += cr.w[2] * f2[jj];
+                    }
                 off += feats_at_level(static_cast<uint32_t>(l));
             }
-            J.dst[lid * J.C + j] = static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(v))));
+#pragma unroll
+            for (int jj = 0; jj < CA; ++jj)
+                if (C > 0 || jj < nc)
+                    dst[k * c + j0 + jj] = exact_sigmoid
+                                               ? static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(v[jj]))))
+                                               : 1.0f / (1.0f + expf(-v[jj]));
         }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_accum_sigmoid(int T, AccumJob f, AccumJob a) {
+    extern __shared__ float lds_row[];
+    for (int t = blockIdx.x; t < T; t += gridDim.x) {
+        if (f.C == 7) accum_row_fwd<7>(f, t, lds_row, false);
+        else if (f.C == 8) accum_row_fwd<8>(f, t, lds_row, false);
+        else accum_row_fwd<0>(f, t, lds_row, false);
+        accum_row_fwd<1>(a, t, lds_row, true);
     }
 }
 
-// Backward of both: grad_src += scale * grad_dst * s (1 - s) scattered through
-// the transpose of the accumulation (ds_accumulate_bwd's atomics).
-__global__ void k_accum_sigmoid_bwd(int T, AccumJob f, AccumJob a) {
-    const long long nf = (long long)T * f.S_T, n = nf + (long long)T * a.S_T;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n; id += stride) {
-        const bool is_f = id < nf;
-        const AccumJob& J = is_f ? f : a;
-        const long long lid = is_f ? id : id - nf;
-        int t;
-        float b0, b1;
-        accum_point(J, lid, t, b0, b1);
-        float* gsrc = J.grad_src + (long long)t * J.S_tot * J.C;
-        uint32_t off = 0;
-        for (int l = J.lo; l <= J.hi; ++l) {
-            const Corners c = lattice_corners(b0, b1, static_cast<uint32_t>(l));
-            float* dst = gsrc + (long long)off * J.C;
-            for (int j = 0; j < J.C; ++j) {
-                const float sg = J.dst[lid * J.C + j];
-                const float g = J.scale * J.grad_dst[lid * J.C + j] * (sg * (1.0f - sg));
-                if (g == 0.f) continue;
-                for (int i = 0; i < 3; ++i) unsafeAtomicAdd(&dst[c.idx[i] * J.C + j], g * c.w[i]);
+// Backward, one workgroup per triangle, by restriction.  The weight of
+// source vertex V of level l at a target vertex p is V's hat function on the
+// level-l lattice (it equals the weight lattice_corners gives V wherever p
+// falls), and the lattices are nested: level l + 1 splits every level-l cell
+// at its edge midpoints, so phi^l_V = phi^(l+1)_2V + 1/2 sum of phi^(l+1)_N
+// over the six level-(l+1) neighbours N of 2V (the midpoints of V's edges).
+// Hence, with r_hi = g = scale grad_dst s (1 - s) at the target vertices,
+// grad_src at level l is r_l[V] = r_(l+1)[2V] + 1/2 sum_N r_(l+1)[N], one
+// level at a time from the top: seven LDS reads per coarse vertex and
+// channel instead of a walk over every target vertex in V's support, no
+// atomics, and every source row written once.  (The sums are regrouped with
+// respect to the scatter of ds_accumulate_bwd; the arithmetic is otherwise
+// the same, the halves exact.)
+template <int C>
+__device__ __forceinline__ void accum_row_bwd(const AccumJob& J, int t, float* lds) {
+    const int c = C > 0 ? C : J.C;
+    const int n_dst = J.S_T * c;
+    const float* gd = J.grad_dst + (long long)t * n_dst;
+    const float* sd = J.dst + (long long)t * n_dst;
+    float* fine = lds;                 // r of the finer level, [S_(l+1), c]
+    float* coarse = lds + n_dst;       // r of level l, [S_l, c]
+    {
+        constexpr int U = 4;   // loads in flight per thread: 2 U
+        const int nt = blockDim.x;
+        int i = threadIdx.x;
+        for (; i + (U - 1) * nt < n_dst; i += U * nt) {
+            float a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                a[u] = gd[i + u * nt];
+                b[u] = sd[i + u * nt];
             }
-            off += feats_at_level(static_cast<uint32_t>(l));
+#pragma unroll
+            for (int u = 0; u < U; ++u) fine[i + u * nt] = J.scale * a[u] * (b[u] * (1.0f - b[u]));
         }
+        for (; i < n_dst; i += nt) {
+            const float sg = sd[i];
+            fine[i] = J.scale * gd[i] * (sg * (1.0f - sg));
+        }
+    }
+    __syncthreads();
+    // offsets of the source levels in a row
+    int off_hi = 0;
+    for (int l = J.lo; l < J.hi; ++l) off_hi += static_cast<int>(feats_at_level(static_cast<uint32_t>(l)));
+    float* out = J.grad_src + (long long)t * J.S_tot * c;
+    // level hi is the identity (the target lattice is level hi)
+    for (int i = threadIdx.x; i < n_dst; i += blockDim.x) out[off_hi * c + i] = fine[i];
+    int off = off_hi;
+    for (int l = J.hi - 1; l >= J.lo; --l) {
+        const int S = static_cast<int>(feats_at_level(static_cast<uint32_t>(l)));
+        const int Rf = 1 << (l + 1);   // the finer level's edge
+        off -= S;
+        for (int u = threadIdx.x; u < S * c; u += blockDim.x) {
+            const int k = u / c, j = u - k * c;
+            uint32_t X, Y;
+            lattice_xy(static_cast<uint32_t>(k), X, Y);
+            const int x = 2 * static_cast<int>(X), y = 2 * static_cast<int>(Y);
+            auto at = [&](int px, int py) -> float {
+                if (px < 0 || py < 0 || px + py > Rf) return 0.f;
+                const int n = px + py;
+                return fine[(((n * (n + 1)) >> 1) + py) * c + j];
+            };
+            const float nb = ((at(x + 1, y) + at(x - 1, y)) + (at(x, y + 1) + at(x, y - 1))) +
+                             (at(x + 1, y - 1) + at(x - 1, y + 1));
+            coarse[u] = at(x, y) + 0.5f * nb;
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < S * c; u += blockDim.x) {
+            out[off * c + u] = coarse[u];
+            fine[u] = coarse[u];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_accum_sigmoid_bwd(int T, AccumJob f, AccumJob a) {
+    extern __shared__ float lds_row[];
+    for (int t = blockIdx.x; t < T; t += gridDim.x) {
+        if (f.C == 7) accum_row_bwd<7>(f, t, lds_row);
+        else if (f.C == 8) accum_row_bwd<8>(f, t, lds_row);
+        else accum_row_bwd<0>(f, t, lds_row);
+        accum_row_bwd<1>(a, t, lds_row);
     }
 }
 
@@ -384,6 +484,9 @@ int step_prepare(const ds_step_desc* d, void* stream, Step& st) {
     StepLayout& L = st.L;
     if (!layout(d, st.v, L))
         return step_fail(DS_ERR_UNSUPPORTED, "no compiled shading kernel for this (levels, channels, hidden, sh2)");
+    if (d->accumulate && 4 * std::max<long long>(L.S_src * d->channels, L.Sa_src) > 160 * 1024)
+        return step_fail(DS_ERR_UNSUPPORTED, "accumulate: a triangle's source row (levels x channels) must fit "
+                                             "160 KB of LDS (levels up to 5 with 7 channels do)");
     if (!d->workspace || d->workspace_bytes < L.total)
         return step_fail(DS_ERR_WORKSPACE, "workspace smaller than ds_step_workspace_size");
     if (reinterpret_cast<uintptr_t>(d->workspace) & 255u) return step_fail(DS_ERR_INVALID, "workspace must be 256-byte aligned");
@@ -434,6 +537,29 @@ hipError_t mark(const ds_step_desc* d, hipStream_t s, int stage, bool begin) {
     return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
 }
 
+// LDS of the accumulation kernels: one source row (forward) or the target
+// row plus the level below it (backward) of the larger of the two jobs.
+size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L) {
+    const size_t below = d->feat_hi > 0 ? feats_at_level(d->feat_hi - 1) : 0;
+    const size_t below_a = d->alpha_level > 0 ? feats_at_level(d->alpha_level - 1) : 0;
+    return sizeof(float) * std::max(std::max<size_t>((size_t)L.S_src * d->channels, (size_t)L.Sa_src),
+                                    std::max<size_t>((size_t)(L.S_shade + below) * d->channels,
+                                                     (size_t)L.Sa + below_a));
+}
+
+// Rows beyond the default 64 KB of LDS need each kernel's opt-in, once.
+hipError_t accum_lds_opt_in() {
+    static bool done = false;
+    if (done) return hipSuccess;
+    for (const void* k : {reinterpret_cast<const void*>(k_accum_sigmoid),
+                          reinterpret_cast<const void*>(k_accum_sigmoid_bwd)}) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+    }
+    done = true;
+    return hipSuccess;
+}
+
 AccumJob feat_job(const ds_step_desc* d, const Step& st) {
     return AccumJob{d->feat, st.feat_acc, st.gfeat_acc, d->grad_feat, 1.0f, d->feat_lo, d->feat_hi, d->feat_hi,
                     d->channels, static_cast<int>(st.L.S_src), static_cast<int>(st.L.S_shade)};
@@ -452,24 +578,21 @@ int enqueue_forward(const ds_step_desc* d, Step& st) {
     const bool acc = d->accumulate != 0;
     const long long n_alpha = (long long)d->T * L.Sa;
     const long long n_feat = (long long)d->T * L.S_shade * d->channels;   // the table the shading writes
-    const long long n_src = (long long)d->T * L.S_src * d->channels;
-    const long long n_asrc = (long long)d->T * L.Sa_src;
     const long long n_pos = 4LL * d->B * d->V;
     hipError_t e;
     const long long prep_n = std::max(std::max(std::max(n_alpha, (long long)d->T), (long long)L.n_tiles),
                                       std::max(std::max(n_pos / 4, n_feat / 4),
                                                std::max((long long)d->B * d->V, (long long)kStatCount)));
     DS_TRY(mark(d, s, DS_STAGE_PREP, true), "event");
-    k_step_prep<<<std::min<unsigned>(blocks_for(std::max(prep_n, acc ? n_src / 4 : 0LL), 256), 4096), 256, 0, s>>>(
+    k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
         acc ? nullptr : d->alpha_logits, d->alpha, n_alpha, st.galpha, st.gpos, n_pos, st.tri_count, d->T,
-        acc ? st.gfeat_acc : d->grad_feat, n_feat, acc ? d->grad_feat : nullptr, acc ? n_src : 0,
-        acc ? d->grad_alpha_logits : nullptr, acc ? n_asrc : 0, d->B, d->V, d->verts, d->mvp,
+        acc ? st.gfeat_acc : d->grad_feat, n_feat, nullptr, 0, nullptr, 0, d->B, d->V, d->verts, d->mvp,
         reinterpret_cast<float4*>(d->pos), st.rw.tile_count, L.n_tiles, st.rw.stats);
     DS_TRY(hipGetLastError(), "step prep");
-    if (acc) {
-        const long long n = (long long)d->T * (L.S_shade + L.Sa);
-        k_accum_sigmoid<<<std::min<unsigned>(blocks_for(n, 256), 8192), 256, 0, s>>>(d->T, feat_job(d, st),
-                                                                                     alpha_job(d, st));
+    if (acc && d->T > 0) {
+        DS_TRY(accum_lds_opt_in(), "accumulate LDS");
+        k_accum_sigmoid<<<std::min(d->T, 4096), 256, accum_lds_bytes(d, L), s>>>(d->T, feat_job(d, st),
+                                                                                alpha_job(d, st));
         DS_TRY(hipGetLastError(), "accumulate + sigmoid");
     }
     DS_TRY(mark(d, s, DS_STAGE_PREP, false), "event");
@@ -545,10 +668,10 @@ int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, b
                 ? const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(d->seed_device))
                 : nullptr);
         DS_TRY(hipGetLastError(), "epilogue");
-        if (acc) {
-            const long long n = (long long)d->T * (L.S_shade + L.Sa);
-            k_accum_sigmoid_bwd<<<std::min<unsigned>(blocks_for(n, 256), 8192), 256, 0, s>>>(d->T, feat_job(d, st),
-                                                                                             alpha_job(d, st));
+        if (acc && d->T > 0) {
+            DS_TRY(accum_lds_opt_in(), "accumulate LDS");
+            k_accum_sigmoid_bwd<<<std::min(d->T, 4096), 256, accum_lds_bytes(d, L), s>>>(d->T, feat_job(d, st),
+                                                                                        alpha_job(d, st));
             DS_TRY(hipGetLastError(), "accumulate + sigmoid backward");
         }
     }

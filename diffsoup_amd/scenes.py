@@ -313,6 +313,23 @@ def reference_training_soup(seed=0, T=3000, views=2, res=(192, 224), levels=(0, 
                 accumulate=True, alpha_lo=alo, sh2=True, inv_mvp=inv_mvp)
 
 
+def reference_sphere_torus_soup(seed=0, views=4, res=(800, 800), levels=(0, 0), src_std=0.5) -> Soup:
+    """The reference's own training iteration at the size of its synthetic
+    driver (examples/02_synthetic.py: 15K primitives, batch of 4 views at
+    800x800): the sphere + torus + floaters geometry of sphere_torus_soup,
+    with 7-channel feature and opacity sources over ``levels`` (accumulated
+    and passed through a sigmoid), SH2 inputs and ColorMLP(16, 16,
+    n_layers=2)."""
+    import dataclasses
+
+    g = sphere_torus_soup(seed, views=views, res=res)
+    r = reference_training_soup(seed + 1, T=g.T, views=views, res=res, levels=levels, alpha_levels=levels,
+                                src_std=src_std)
+    inv_mvp = np.linalg.inv(g.mvp.astype(np.float64)).astype(np.float32)
+    return dataclasses.replace(r, verts=g.verts, tri=g.tri, mvp=g.mvp, inv_mvp=inv_mvp, target=g.target,
+                               name=f"reference_sphere_torus_{g.T}_levels{levels[0]}-{levels[1]}")
+
+
 def project_vertices_np(verts, mvp):
     """clip[b, v] = mvp[b] @ [verts[v], 1] in float32 (examples/utils.py:412-426)."""
     vh = np.concatenate([verts, np.ones((verts.shape[0], 1), np.float32)], -1)
