@@ -72,6 +72,8 @@ def parse():
                     help="write the SQ counter summary of two --pmc passes of --profile-driver")
     ap.add_argument("--summarize-rocprof", nargs=4, metavar=("KERNEL_STATS_CSV", "FETCH_CSV", "WRITE_CSV", "OUT_MD"),
                     help="write the markdown summary of a kernel trace and two counter runs, then exit")
+    ap.add_argument("--summarize-workload-traffic", nargs=4, metavar=("WORKLOAD", "FETCH_CSV", "WRITE_CSV", "OUT_MD"),
+                    help="write the HBM-traffic summary of two --pmc runs of --profile-driver --workload WORKLOAD")
     ap.add_argument("--summarize-workload-trace", nargs=3, metavar=("WORKLOAD", "KERNEL_STATS_CSV", "OUT_MD"),
                     help="write the markdown summary of a kernel trace of `bench.py --workload WORKLOAD`")
     return ap.parse_args()
@@ -112,6 +114,9 @@ def main():
     if args.summarize_workload_trace:
         summarize_workload_trace(*args.summarize_workload_trace)
         return
+    if args.summarize_workload_traffic:
+        summarize_workload_traffic(*args.summarize_workload_traffic)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -122,7 +127,10 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.workload != "soup15k":
-        other_workload(args, dev)
+        if args.profile_driver:
+            profile_driver_workload(args.workload, dev)
+        else:
+            other_workload(args, dev)
         return
     soup = scenes.sphere_torus_soup(0)
     if args.profile_driver:
@@ -491,12 +499,18 @@ def _kernel_key(name):
     m = re.search(r"::(k_\w+)", name)
     if m:
         k = m.group(1)
-        t = re.search(r"Cfg<([^>]*)>", name) or re.search(r"k_\w+<([^>(]*)>", name)
-        # k_shade<Cfg<...>, bf16 rows, two-pass form>
-        flags = re.search(r"Cfg<[^>]*>,\s*(true|false)(?:,\s*(true|false))?>", name)
-        rows = ",bf16rows" if flags and flags.group(1) == "true" else ""
-        rows += ",two-pass" if flags and flags.group(2) == "true" else ""
-        return k + ("<" + t.group(1).replace(" ", "") + rows + ">" if t else "")
+        cfg = re.search(r"Cfg<([^>]*)>((?:,\s*(?:true|false))*)>", name)
+        if cfg:
+            # k_shade<Cfg<levels, channels, hidden, bf16 MLP, SH2>, bf16 rows, two-pass form, direct rows>
+            p = [x.strip() for x in cfg.group(1).split(",")]
+            flags = [x.strip() for x in cfg.group(2).split(",") if x.strip()]
+            name_ = ",".join(p[:4]) + (",sh2" if len(p) > 4 and p[4] == "true" else "")
+            for flag, label in zip(flags, ("bf16rows", "two-pass", "direct")):
+                if flag == "true":
+                    name_ += "," + label
+            return f"{k}<{name_}>"
+        t = re.search(r"k_\w+<([^>(]*)>", name)
+        return k + ("<" + t.group(1).replace(" ", "") + ">" if t else "")
     if "neg" in name:
         return "calibration torch.neg 256 MiB"
     return re.sub(r"\(.*", "", name)[:50]
@@ -630,6 +644,75 @@ def profile_driver(soup, dev):
     for _ in range(2):
         torch.neg(a, out=b)
     torch.cuda.synchronize()
+
+
+def profile_driver_workload(name, dev, tile=16, steps=2):
+    """Counter-collection driver for one larger workload: ``steps`` eager
+    fused steps at tile ``tile`` (the tile the autotuner picks for these
+    workloads), then the 256 MiB torch.neg calibration kernel."""
+    import torch
+
+    from diffsoup_amd import fused, scenes
+
+    gen, bf16, _ = WORKLOADS[name]
+    soup = getattr(scenes, gen)(0, **WORKLOAD_ARGS.get(name, {}))
+    mvp = torch.from_numpy(soup.mvp).to(dev)
+    target = torch.from_numpy(soup.target).to(dev)
+    inv = torch.from_numpy(soup.inv_mvp).to(dev) if soup.sh2 else None
+    st = fused.FusedStep(soup.resolutions, soup.T, soup.verts.shape[0], soup.alpha_level, soup.levels,
+                         soup.feat.shape[2], soup.mlp_dims[1], bf16=bf16, bf16_features=bf16, device=dev, tile=tile,
+                         loss="l1_ssim" if soup.accumulate else "mse", accumulate=soup.accumulate,
+                         alpha_lo=soup.alpha_lo, sh2=soup.sh2)
+    prm = fused.SoupParams.from_soup(soup, dev)
+    if bf16:
+        prm.with_bf16_features()
+    for i in range(steps):
+        st.run(prm, mvp, target, seed=100 + i, inv_mvp=inv)
+    torch.cuda.synchronize()
+    del st, prm
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+
+
+def summarize_workload_traffic(workload, fetch_csv, write_csv, out_md):
+    """Markdown table of HBM traffic per launch of one larger workload's step
+    kernels (two counter runs of ``--profile-driver --workload``), scaled by
+    the torch.neg calibration as in the headline summary."""
+    import collections
+    import csv
+
+    res = collections.defaultdict(dict)
+    for ctr, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        agg = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            agg[_kernel_key(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        for k, v in agg.items():
+            res[k][ctr] = sum(v) / len(v)
+    cal = res["calibration torch.neg 256 MiB"]
+    fs, ws = 262144.0 / cal["FETCH_SIZE"], 262144.0 / cal["WRITE_SIZE"]
+    cmd = "\n".join(
+        f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc {c} --output-format csv -d $REPO/gpurun_out/pmc_{workload}_{c} "
+        f"-o run -- python3 $REPO/bench.py --workload {workload} --profile-driver" for c in ("FETCH_SIZE", "WRITE_SIZE"))
+    out = [f"# HBM traffic per launch: `bench.py --workload {workload}`\n",
+           f"Workload: {WORKLOADS[workload][2]}.  Two eager fused steps at tile 16, one counter per run; `$REPO` is the "
+           "repository root.\n", "```\n" + cmd + "\n```\n",
+           f"Calibration (torch.neg over 256 MiB reads and writes 262144 KiB): FETCH_SIZE {cal['FETCH_SIZE']:.0f} KiB, "
+           f"WRITE_SIZE {cal['WRITE_SIZE']:.0f} KiB; scale factors fetch x{fs:.3f}, write x{ws:.3f}.\n",
+           "| kernel | fetch MiB | write MiB | traffic MiB |", "|---|---:|---:|---:|"]
+    order = sorted(res, key=lambda k: -(res[k].get("FETCH_SIZE", 0) * fs + res[k].get("WRITE_SIZE", 0) * ws))
+    for k in order[:20]:
+        if k.startswith("calibration") or not k.startswith("k_"):
+            continue
+        f = res[k].get("FETCH_SIZE", 0) * fs / 1024
+        w = res[k].get("WRITE_SIZE", 0) * ws / 1024
+        out.append(f"| `{k}` | {f:.1f} | {w:.1f} | {f + w:.1f} |")
+    out += ["", "Summary written by:", "```\npython3 bench.py --summarize-workload-traffic " + workload + " " + fetch_csv +
+            " " + write_csv + " " + out_md + "\n```"]
+    with open(out_md, "w") as fh:
+        fh.write("\n".join(out) + "\n")
 
 
 def plain_its(soup, prm, mvp, target, steps):

@@ -39,6 +39,9 @@ struct ShadeArgs {
     // covered pixel's view, row and column, and their inverse MVPs [B,4,4].
     Views views;
     const float* inv_mvp;
+    // where each wave's range of list starts: [nblocks * 4 + 1], written by
+    // launch_shade_bounds for the same nblocks
+    const int* bounds;
 };
 
 bool shade_supported(int nl, int c, int dh, bool sh);
@@ -48,5 +51,14 @@ hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const f
                               const float* w3, float* blobs, hipStream_t s);
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
                         hipStream_t s);
+// The waves' ranges of the triangle-grouped list (before any shading launch).
+hipError_t launch_shade_bounds(const int* tri_off, int T, int nblocks, int* bounds, hipStream_t s);
+// Clears the gradient rows no shading wave stores whole (before an adjoint
+// pass that does not take its rows from global memory).
+hipError_t launch_zero_rows(const int* tri_off, const PixRec* list, const int* bounds, int T, int nblocks,
+                            int row, float* grad, hipStream_t s);
+// Whether a configuration gathers and scatters its rows in global memory
+// (rows too large for the LDS plan): its gradient table must then be zeroed.
+bool shade_direct_rows(int nl, int c, int dh, bool sh, bool bf16, bool feat_bf16, int S_tot);
 
 }  // namespace ds

@@ -42,6 +42,13 @@ constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Four fp32 values to bf16, rounded to nearest even, by the hardware's
+// packed conversion (v_cvt_pk_bf16_f32, two values per instruction).
+__device__ __forceinline__ i16x4 to_bf16x4(f32x4 v) {
+    return __builtin_bit_cast(i16x4, __builtin_convertvector(v, bf16x4));
+}
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -158,7 +165,8 @@ __device__ __forceinline__ float4 ld4(const unsigned short* p) {
 template <class K>
 __device__ __forceinline__ void stage_put(typename K::ST* reg, int width, int u0, int j, f32x4 v) {
     if constexpr (K::BF16) {
-        for (int r = 0; r < 4; ++r) reg[(u0 + r) * K::SPX + j] = bf16_rne(v[r]);
+        const i16x4 b = to_bf16x4(v);
+        for (int r = 0; r < 4; ++r) reg[(u0 + r) * K::SPX + j] = static_cast<unsigned short>(b[r]);
     } else {
         *reinterpret_cast<float4*>(reg + j * (width + 4) + u0) = make_float4(v[0], v[1], v[2], v[3]);
     }
@@ -212,17 +220,11 @@ struct Mma<false> {
 
 template <>
 struct Mma<true> {
-    __device__ static __forceinline__ i16x4 pack(f32x4 v) {
-        i16x4 r;
-        r[0] = static_cast<short>(bf16_rne(v[0]));
-        r[1] = static_cast<short>(bf16_rne(v[1]));
-        r[2] = static_cast<short>(bf16_rne(v[2]));
-        r[3] = static_cast<short>(bf16_rne(v[3]));
-        return r;
-    }
-    __device__ static __forceinline__ f32x4 tile(const float* blob, int lane, f32x4 in, f32x4 acc) {
+    __device__ static __forceinline__ i16x4 pack(f32x4 v) { return to_bf16x4(v); }
+    // in: the input tile already packed to bf16
+    __device__ static __forceinline__ f32x4 tile_packed(const float* blob, int lane, i16x4 in, f32x4 acc) {
         const i16x4 a = reinterpret_cast<const i16x4*>(blob)[lane];
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, pack(in), acc, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, in, acc, 0, 0, 0);
     }
     // staged bf16 values, [unit][pixel] rows of spx: A(i, k) = At[i][k], B(k, j) = Bt[j][k]
     __device__ static __forceinline__ f32x4 outer(const unsigned short* At, const unsigned short* Bt, int spx,
@@ -251,13 +253,22 @@ __device__ __forceinline__ f32x4 stage_outer(const typename K::ST* A, int wa, in
         return Mma<false>::outer(A + ua, wa + 4, B + ub, wb + 4, lane, acc);
 }
 
-// y[m] = bias + sum_t W(m, t) * in[t] for NOUT output tiles and NIN input tiles.
+// y[m] = bias + sum_t W(m, t) * in[t] for NOUT output tiles and NIN input
+// tiles (bf16: each input tile is converted once, not once per output tile).
 template <class K, int NOUT, int NIN>
 __device__ __forceinline__ void dense(const float* blobs, int base, int lane, const f32x4 (&in)[NIN],
                                       f32x4 (&out)[NOUT]) {
-    for (int m = 0; m < NOUT; ++m)
-        for (int t = 0; t < NIN; ++t)
-            out[m] = Mma<K::BF16>::tile(blob_at<K>(blobs, base, m, NIN, t), lane, in[t], out[m]);
+    if constexpr (K::BF16) {
+        i16x4 pin[NIN];
+        for (int t = 0; t < NIN; ++t) pin[t] = Mma<true>::pack(in[t]);
+        for (int m = 0; m < NOUT; ++m)
+            for (int t = 0; t < NIN; ++t)
+                out[m] = Mma<true>::tile_packed(blob_at<K>(blobs, base, m, NIN, t), lane, pin[t], out[m]);
+    } else {
+        for (int m = 0; m < NOUT; ++m)
+            for (int t = 0; t < NIN; ++t)
+                out[m] = Mma<false>::tile(blob_at<K>(blobs, base, m, NIN, t), lane, in[t], out[m]);
+    }
 }
 
 // The 4 channels c .. c+3 of one sample straight from global memory (DIRECT
@@ -270,6 +281,22 @@ __device__ __forceinline__ float4 ld4_global(const float* p, int c) {
         return make_float4(c < C ? p[0] : 0.f, c + 1 < C ? p[1] : 0.f, c + 2 < C ? p[2] : 0.f,
                            c + 3 < C ? p[3] : 0.f);
     }
+}
+
+// dst[i] = src[i] for i < n over the lanes of one wave, 8 loads per lane in
+// flight before the stores.
+template <class T>
+__device__ __forceinline__ void copy_batched(T* dst, const T* __restrict__ src, int n, int lane) {
+    constexpr int U = 8;
+    int i = lane;
+    for (; i + (U - 1) * kWave < n; i += U * kWave) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * kWave];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[i + u * kWave] = v[u];
+    }
+    for (; i < n; i += kWave) dst[i] = src[i];
 }
 
 // e[k] for a lane-varying k in [0, 9) without dynamic register indexing.
@@ -330,53 +357,15 @@ k_shade(ShadeArgs a, int rowp) {
         for (int k = 0; k < K::NL; ++k) { loff[k] = o; o += feats_at_level(a.lo + k); }
     }
 
-    // Each wave owns a contiguous range of the triangle-grouped pixel list
-    // and walks the triangle segments inside it.  A triangle whose pixels all
-    // fall in one wave's range gets its gradient row stored; a triangle split
-    // between waves is added with atomics into the zeroed table.  A
+    // Each wave owns a contiguous range [bounds[wid], bounds[wid + 1]) of the
+    // triangle-grouped pixel list (k_shade_bounds) and walks the triangle
+    // segments inside it.  A triangle whose pixels all fall in one wave's
+    // range gets its gradient row stored; a triangle split between waves is
+    // added with atomics into its row, which k_zero_rows cleared.  A
     // segment's triangle and end come from its first record (wave-uniform
     // loads), so the walk never reads the offset table.
-    //
-    // The ranges split an estimated cost evenly, not the pixel count: a pixel
-    // costs 1 and each triangle's segment kSegCost more (its row staging,
-    // coarse-level reduction and row store, and on average half a tile of
-    // idle lanes), with the triangle index standing in for the number of
-    // non-empty segments before it.  Cost(t) = tri_off[t] + kSegCost t is
-    // monotone, so a 64-way search over triangles (one vector load per
-    // level) finds the segment holding each range boundary.
-    constexpr long long kSegCost = 16;
-    const int n_waves = gridDim.x * kWavesPerBlock;
     const int wid = blockIdx.x * kWavesPerBlock + wave;
-    const int total = a.tri_off[a.T];
-    const long long ctot = total + kSegCost * a.T;
-    int r0, r1;
-    {
-        const long long q0 = ctot * wid / n_waves, q1 = ctot * (wid + 1) / n_waves;
-        // largest t in [0, T] with Cost(t) <= q, for both boundaries at once
-        int lo0 = 0, hi0 = a.T + 1, lo1 = 0, hi1 = a.T + 1;
-        while (hi0 - lo0 > 1 || hi1 - lo1 > 1) {
-            const int st0 = (hi0 - lo0 + kWave - 1) / kWave, st1 = (hi1 - lo1 + kWave - 1) / kWave;
-            const int p0 = lo0 + lane * st0, p1 = lo1 + lane * st1;
-            const bool ok0 = p0 < hi0 && a.tri_off[p0] + kSegCost * p0 <= q0;
-            const bool ok1 = p1 < hi1 && a.tri_off[p1] + kSegCost * p1 <= q1;
-            // lane 0 probes lo, which always qualifies
-            const int k0 = 63 - __builtin_clzll(__ballot(ok0));
-            const int k1 = 63 - __builtin_clzll(__ballot(ok1));
-            lo0 += k0 * st0;
-            hi0 = min(hi0, lo0 + st0);
-            lo1 += k1 * st1;
-            hi1 = min(hi1, lo1 + st1);
-        }
-        // the boundary's pixel inside that triangle's segment
-        auto boundary = [&](long long q, int tq) -> int {
-            if (tq >= a.T) return total;
-            const int tb = a.tri_off[tq], tn = a.tri_off[tq + 1];
-            const long long over = q - (tb + kSegCost * tq) - kSegCost;
-            return over <= 0 ? tb : static_cast<int>(min<long long>(tn, tb + over));
-        };
-        r0 = wid == 0 ? 0 : boundary(q0, lo0);
-        r1 = wid == n_waves - 1 ? total : boundary(q1, lo1);
-    }
+    const int r0 = a.bounds[wid], r1 = a.bounds[wid + 1];
     int t = 0, te = 0;
     bool first_whole = true;   // the first segment starts at r0
     if (r0 < r1) {
@@ -397,6 +386,10 @@ k_shade(ShadeArgs a, int rowp) {
         pos = end;
         float4* g4 = reinterpret_cast<float4*>(grow);
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // The row's loads go out in batches of 8 per lane before the first
+        // LDS store waits on them: a row is read once per step, so each load
+        // is an HBM miss, and a load-store-load chain would pay that latency
+        // for every 64 x 8 or 64 x 16 bytes of the row.
         if constexpr (DIRECT) {
             // nothing staged
         } else if constexpr (C != CP) {
@@ -408,9 +401,8 @@ k_shade(ShadeArgs a, int rowp) {
             }
         } else if constexpr (FB) {
             // bf16 features stay bf16 in the row: 4 values per 8-byte copy
-            const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row);
-            uint2* f2 = reinterpret_cast<uint2*>(frow);
-            for (int i = lane; i < row / 4; i += kWave) f2[i] = src[i];
+            copy_batched(reinterpret_cast<uint2*>(frow),
+                         reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row), row / 4, lane);
         } else if (a.feat_bf16) {
             float4* f4 = reinterpret_cast<float4*>(frow);
             // bf16 features: 4 values per 8-byte load, widened exactly to fp32
@@ -421,9 +413,8 @@ k_shade(ShadeArgs a, int rowp) {
                                     __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
             }
         } else {
-            float4* f4 = reinterpret_cast<float4*>(frow);
-            const float4* src = reinterpret_cast<const float4*>(a.feat + (long long)t * row);
-            for (int i = lane; i < row / 4; i += kWave) f4[i] = src[i];
+            copy_batched(reinterpret_cast<float4*>(frow), reinterpret_cast<const float4*>(a.feat + (long long)t * row),
+                         row / 4, lane);
         }
         wave_lds_sync();
 
@@ -792,6 +783,80 @@ k_shade(ShadeArgs a, int rowp) {
     }
 }
 
+// The shading waves' ranges of the triangle-grouped pixel list.  They split
+// an estimated cost evenly, not the pixel count: a pixel costs 1 and each
+// triangle's segment kSegCost more (its row staging, coarse-level reduction
+// and row store, and on average half a tile of idle lanes), with the
+// triangle index standing in for the number of non-empty segments before
+// it.  Cost(t) = tri_off[t] + kSegCost t is monotone, so a 64-way search
+// over triangles (one vector load per level) finds the segment holding each
+// boundary.  One wave per boundary b in [0, n_waves]: bounds[b] is where
+// wave b's range starts (bounds[0] = 0, bounds[n_waves] = the list length).
+constexpr long long kSegCost = 16;
+
+__global__ void __launch_bounds__(256) k_shade_bounds(const int* __restrict__ tri_off, int T, int n_waves,
+                                                      int* __restrict__ bounds) {
+    const int b = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    const int lane = threadIdx.x % kWave;
+    if (b > n_waves) return;
+    const int total = tri_off[T];
+    const long long q = (total + kSegCost * T) * b / n_waves;
+    // largest t in [0, T] with Cost(t) <= q
+    int lo = 0, hi = T + 1;
+    while (hi - lo > 1) {
+        const int st = (hi - lo + kWave - 1) / kWave;
+        const int pr = lo + lane * st;
+        const bool ok = pr < hi && tri_off[pr] + kSegCost * pr <= q;
+        // lane 0 probes lo, which always qualifies
+        lo += (63 - __builtin_clzll(__ballot(ok))) * st;
+        hi = min(hi, lo + st);
+    }
+    int r;
+    if (b == 0) {
+        r = 0;
+    } else if (b == n_waves || lo >= T) {
+        r = total;
+    } else {
+        // the boundary's pixel inside that triangle's segment
+        const int tb = tri_off[lo], tn = tri_off[lo + 1];
+        const long long over = q - (tb + kSegCost * lo) - kSegCost;
+        r = over <= 0 ? tb : static_cast<int>(min<long long>(tn, tb + over));
+    }
+    if (lane == 0) bounds[b] = r;
+}
+
+// Clears the gradient rows the shading does not store whole: triangles with
+// no covered pixel (their gradient is zero) and triangles split between two
+// shading waves (both add into the row with atomics).  Every other row is
+// stored whole by the wave that owns it, so the table needs no clearing
+// beforehand.  One wave per row job, grid-stride: jobs [0, T) test triangle
+// t for emptiness, jobs [T, T + n_waves) test the triangle at wave boundary
+// j - T for a split.
+__global__ void __launch_bounds__(256) k_zero_rows(const int* __restrict__ tri_off, const PixRec* __restrict__ list,
+                                                   const int* __restrict__ bounds, int T, int n_waves, int row,
+                                                   float* __restrict__ grad) {
+    const int lane = threadIdx.x % kWave;
+    const long long n_jobs = (long long)T + n_waves;
+    const long long stride = (long long)gridDim.x * (blockDim.x / kWave);
+    for (long long job = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; job < n_jobs; job += stride) {
+        int t = -1;
+        if (job < T) {
+            if (tri_off[job] == tri_off[job + 1]) t = static_cast<int>(job);
+        } else {
+            const int r = bounds[job - T];
+            if (r > 0 && r < tri_off[T] && list[r - 1].tri == list[r].tri) t = list[r].tri;
+        }
+        if (t < 0) continue;
+        float* dst = grad + (long long)t * row;
+        if ((row & 3) == 0) {
+            float4* d4 = reinterpret_cast<float4*>(dst);
+            for (int i = lane; i < row / 4; i += kWave) d4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            for (int i = lane; i < row; i += kWave) dst[i] = 0.f;
+        }
+    }
+}
+
 // Per-lane weight blobs for one configuration, from the fp32 master weights
 // W1 [DH, DIN], W2 [DH, DH], W3 [3, DH].  Entry (layer, m, t, lane, e) is the
 // A operand for output row 16 m + lane % 16 and input row 16 t + 4 (lane / 16) + e.
@@ -933,6 +998,47 @@ hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const f
     DS_SHADE_CONFIGS(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_shade_bounds(const int* tri_off, int T, int nblocks, int* bounds, hipStream_t s) {
+    const int n_waves = nblocks * kWavesPerBlock;
+    const int threads = (n_waves + 1) * kWave;
+    k_shade_bounds<<<(threads + 255) / 256, 256, 0, s>>>(tri_off, T, n_waves, bounds);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(const int* tri_off, const PixRec* list, const int* bounds, int T, int nblocks,
+                            int row, float* grad, hipStream_t s) {
+    const int n_waves = nblocks * kWavesPerBlock;
+    const long long jobs = (long long)T + n_waves;
+    const long long blocks = std::min<long long>((jobs + 3) / 4, 4096);
+    if (blocks > 0) k_zero_rows<<<static_cast<unsigned>(blocks), 256, 0, s>>>(tri_off, list, bounds, T, n_waves, row, grad);
+    return hipGetLastError();
+}
+
+namespace {
+template <class K, bool FB>
+bool direct_rows(int S_tot) {
+    using P = Plan<K, FB>;
+    const int rowp = (S_tot * K::CP + 3) & ~3;
+    const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
+    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
+    return std::max(lds, red) > 160 * 1024;
+}
+template <int NL, int C, int DH, bool SH>
+bool config_direct(bool bf16, bool feat_bf16, int S_tot) {
+    if (!bf16) return direct_rows<Cfg<NL, C, DH, false, SH>, false>(S_tot);
+    if (feat_bf16 && C % 4 == 0) return direct_rows<Cfg<NL, C, DH, true, SH>, true>(S_tot);
+    return direct_rows<Cfg<NL, C, DH, true, SH>, false>(S_tot);
+}
+}  // namespace
+
+bool shade_direct_rows(int nl, int c, int dh, bool sh, bool bf16, bool feat_bf16, int S_tot) {
+#define X(NL, C, DH, SH) \
+    if (nl == NL && c == C && dh == DH && sh == SH) return config_direct<NL, C, DH, SH>(bf16, feat_bf16, S_tot);
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return false;
 }
 
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,

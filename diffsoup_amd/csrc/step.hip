@@ -372,11 +372,12 @@ inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct StepLayout {
     size_t raster, gpos, galpha, tile_list, tile_ncov, tri_count, tri_off, list, scan, bgloss, blobs, partials,
-        tile_px_off, px_list, scan2, photo, feat_acc, gfeat_acc,
+        tile_px_off, px_list, scan2, photo, feat_acc, gfeat_acc, bounds,
         total;
     int n_tiles, nblocks, nparam;
     int nl;          // levels the shading kernel gathers (1 with accumulate)
     bool sh;         // SH2 inputs
+    bool direct;     // the shading reads and scatters its rows in global memory
     long long P;
     long long S_src;   // samples per triangle of d->feat
     long long S_shade; // samples per triangle of the table the shading reads
@@ -421,6 +422,9 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.bgloss = o; o += up256(sizeof(float) * (size_t)L.n_tiles);
     L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(L.nl, d->channels, d->hidden, L.sh));
     L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
+    L.bounds = o; o += up256(sizeof(int) * ((size_t)L.nblocks * 4 + 1));
+    L.direct = shade_direct_rows(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0,
+                                 !acc && d->feat_bf16 != nullptr, static_cast<int>(L.S_shade));
     L.feat_acc = L.gfeat_acc = o;
     if (acc) {
         const size_t nacc = sizeof(float) * (size_t)d->T * L.S_shade * d->channels;
@@ -445,7 +449,7 @@ struct Step {
     RasterWorkspace rw;
     float *gpos, *galpha, *bgloss, *blobs, *partials, *feat_acc, *gfeat_acc, *photo;
     int2* tile_list;
-    int *tile_ncov, *tri_count, *tri_off, *tile_px_off;
+    int *tile_ncov, *tri_count, *tri_off, *tile_px_off, *bounds;
     PixRec* list;
     int4* px_list;
     void *scan_tmp, *scan_tmp2;
@@ -524,6 +528,7 @@ int step_prepare(const ds_step_desc* d, void* stream, Step& st) {
     st.feat_acc = d->accumulate ? reinterpret_cast<float*>(ws + L.feat_acc) : nullptr;
     st.gfeat_acc = d->accumulate ? reinterpret_cast<float*>(ws + L.gfeat_acc) : nullptr;
     st.photo = reinterpret_cast<float*>(ws + L.photo);
+    st.bounds = reinterpret_cast<int*>(ws + L.bounds);
     st.inv_n = 1.0f / (3.0f * static_cast<float>(L.P));
     return DS_OK;
 }
@@ -581,12 +586,13 @@ int enqueue_forward(const ds_step_desc* d, Step& st) {
     const long long n_pos = 4LL * d->B * d->V;
     hipError_t e;
     const long long prep_n = std::max(std::max(std::max(n_alpha, (long long)d->T), (long long)L.n_tiles),
-                                      std::max(std::max(n_pos / 4, n_feat / 4),
+                                      std::max(std::max(n_pos / 4, L.direct ? n_feat / 4 : 0LL),
                                                std::max((long long)d->B * d->V, (long long)kStatCount)));
     DS_TRY(mark(d, s, DS_STAGE_PREP, true), "event");
     k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
         acc ? nullptr : d->alpha_logits, d->alpha, n_alpha, st.galpha, st.gpos, n_pos, st.tri_count, d->T,
-        acc ? st.gfeat_acc : d->grad_feat, n_feat, nullptr, 0, nullptr, 0, d->B, d->V, d->verts, d->mvp,
+        acc ? st.gfeat_acc : d->grad_feat, L.direct ? n_feat : 0, nullptr, 0, nullptr, 0, d->B, d->V, d->verts,
+        d->mvp,
         reinterpret_cast<float4*>(d->pos), st.rw.tile_count, L.n_tiles, st.rw.stats);
     DS_TRY(hipGetLastError(), "step prep");
     if (acc && d->T > 0) {
@@ -620,6 +626,7 @@ int enqueue_forward(const ds_step_desc* d, Step& st) {
     DS_TRY(launch_fill_triangle_lists(st.v, st.tile_list, st.tile_ncov, d->rast, d->target, st.tri_off, st.list,
                                       st.tile_px_off, st.px_list, s),
            "triangle lists");
+    DS_TRY(launch_shade_bounds(st.tri_off, d->T, L.nblocks, st.bounds, s), "shading ranges");
     return DS_OK;
 }
 
@@ -632,8 +639,12 @@ int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ex
                  acc ? nullptr : reinterpret_cast<const unsigned short*>(d->feat_bf16),
                  acc ? st.gfeat_acc : d->grad_feat, static_cast<int>(L.S_shade), acc ? d->feat_hi : d->feat_lo,
                  st.blobs, d->b1, d->b2, d->b3, d->color, d->grad_color, d->pix_loss, 2.0f * st.inv_n,
-                 st.partials, ext_grad, fwd_only, st.v, d->inv_mvp};
+                 st.partials, ext_grad, fwd_only, st.v, d->inv_mvp, st.bounds};
     hipError_t e;
+    if (!fwd_only && !L.direct)
+        DS_TRY(launch_zero_rows(st.tri_off, st.list, st.bounds, d->T, L.nblocks,
+                                static_cast<int>(L.S_shade) * d->channels, acc ? st.gfeat_acc : d->grad_feat, st.s),
+               "zero rows");
     DS_TRY(launch_shade(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, sa, L.nblocks, st.s), "shade");
     return DS_OK;
 }
