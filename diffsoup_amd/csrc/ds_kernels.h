@@ -46,6 +46,7 @@ enum Stat : int {
     kStatPairs = 1,     // (triangle, tile) pairs produced by binning
     kStatOverflow = 2,  // 1 when pairs exceeded the bin capacity (fallback path ran)
     kStatBadIndex = 3,  // 1 when a triangle referenced a vertex outside [0, V)
+    kStatAuxWalkQuads = 4,  // 8x8 quads whose opacity-gradient records overflowed (their bins were walked)
     kStatCount = 8,
 };
 
@@ -114,6 +115,15 @@ struct BackgroundShade {
     int* tri_count;        // [T] covered pixels per triangle over all views (zeroed before)
     float grad_scale;      // dL/dcolor = grad_scale * (color - target)
     float background;      // constant background colour
+    // Fragment records for the opacity gradient (null: none): every covered
+    // fragment not known to lie behind its pixel's visible surface when the
+    // walk reached it, as (triangle | pixel-in-quad << 24, b0, b1, depth bits),
+    // aux_per_quad slots per 8x8 quad (tile-major, quads in wave order);
+    // aux_n[quad] = the quad's count (above aux_per_quad: overflowed, only
+    // the first aux_per_quad were stored).
+    int4* aux_rec;
+    int* aux_n;
+    int aux_per_quad;
 };
 hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource& a, int mask_mode,
                                        MaskSeed seed, float* rast, const RasterWorkspace& w,
@@ -121,6 +131,12 @@ hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource&
 hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const float* rast,
                               const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
                               hipStream_t s);
+// The opacity gradient from the records a training-step resolve emitted
+// (BackgroundShade::aux_rec); the quads of a tile whose records overflowed
+// walk its bins instead (the same result).
+hipError_t launch_opacity_aux_records(const Views& v, int T, const AlphaSource& a, const float* rast,
+                                      const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
+                                      const int4* rec, const int* aux_n, int per_quad, hipStream_t s);
 hipError_t launch_pixel_l1(long long P, int C, const float* color, const float* target,
                            float* pix_loss, hipStream_t s);
 

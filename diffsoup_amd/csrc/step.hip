@@ -4,11 +4,14 @@
 //                   accumulators and counters, clip-space projection
 //   k_blobs         per-lane MLP weight operands (shade.hip)
 //   raster bin      setup / scan / emit (raster.hip)
-//   resolve+shade   depth test, background shading, covered-pixel list
+//   resolve+shade   depth test, background shading, covered-pixel list,
+//                   fragment records for the opacity gradient
 //   k_shade         gather -> MLP -> blend -> loss gradient, MLP adjoint,
 //                   feature scatter
 //   edge grad       silhouette / intersection gradients into clip positions
-//   opacity aux     score-function opacity gradient (same bins, no re-binning)
+//   opacity aux     score-function opacity gradient over the fragment
+//                   records (a second walk of the bins for a tile whose
+//                   records overflowed)
 //   k_epilogue      MLP weight-gradient reduction, the loss, clip -> world
 //                   vertex gradients and the sigmoid backward
 //
@@ -372,7 +375,7 @@ inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct StepLayout {
     size_t raster, gpos, galpha, tile_list, tile_ncov, tri_count, tri_off, list, scan, bgloss, blobs, partials,
-        tile_px_off, px_list, scan2, photo, feat_acc, gfeat_acc, bounds,
+        tile_px_off, px_list, scan2, photo, feat_acc, gfeat_acc, bounds, aux_rec, aux_n,
         total;
     int n_tiles, nblocks, nparam;
     int nl;          // levels the shading kernel gathers (1 with accumulate)
@@ -383,6 +386,7 @@ struct StepLayout {
     long long S_shade; // samples per triangle of the table the shading reads
     long long Sa_src;  // samples per triangle of d->alpha_logits
     long long Sa;      // opacity samples at alpha_level
+    int aux_per_quad;  // fragment records per 8x8 quad for the opacity gradient
 };
 
 int shade_blocks(long long P) {
@@ -423,6 +427,10 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     L.blobs = o; o += up256(sizeof(float) * shade_blob_floats(L.nl, d->channels, d->hidden, L.sh));
     L.partials = o; o += up256(sizeof(float) * (size_t)L.nblocks * L.nparam);
     L.bounds = o; o += up256(sizeof(int) * ((size_t)L.nblocks * 4 + 1));
+    L.aux_per_quad = d->aux_records > 0 ? static_cast<int>(std::min<int64_t>(d->aux_records, 1 << 16)) : 128;
+    const size_t n_quads = (size_t)L.n_tiles * nt2 / 64;
+    L.aux_rec = o; o += up256(sizeof(int4) * n_quads * L.aux_per_quad);
+    L.aux_n = o; o += up256(sizeof(int) * n_quads);
     L.direct = shade_direct_rows(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0,
                                  !acc && d->feat_bf16 != nullptr, static_cast<int>(L.S_shade));
     L.feat_acc = L.gfeat_acc = o;
@@ -451,7 +459,8 @@ struct Step {
     int2* tile_list;
     int *tile_ncov, *tri_count, *tri_off, *tile_px_off, *bounds;
     PixRec* list;
-    int4* px_list;
+    int4 *px_list, *aux_rec;
+    int* aux_n;
     void *scan_tmp, *scan_tmp2;
     float inv_n;   // 1 / (3 P): the MSE mean and the opacity term's 1 / numel(color)
     hipStream_t s;
@@ -521,6 +530,8 @@ int step_prepare(const ds_step_desc* d, void* stream, Step& st) {
     st.scan_tmp = ws + L.scan;
     st.tile_px_off = reinterpret_cast<int*>(ws + L.tile_px_off);
     st.px_list = reinterpret_cast<int4*>(ws + L.px_list);
+    st.aux_rec = reinterpret_cast<int4*>(ws + L.aux_rec);
+    st.aux_n = reinterpret_cast<int*>(ws + L.aux_n);
     st.scan_tmp2 = ws + L.scan2;
     st.bgloss = reinterpret_cast<float*>(ws + L.bgloss);
     st.blobs = reinterpret_cast<float*>(ws + L.blobs);
@@ -613,7 +624,7 @@ int enqueue_forward(const ds_step_desc* d, Step& st) {
     DS_TRY(mark(d, s, DS_STAGE_BIN, false), "event");
     AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(L.Sa)};
     BackgroundShade bg{d->target, d->color, d->grad_color, d->pix_loss, st.bgloss, st.tile_list, st.tile_ncov,
-                       st.tri_count, 2.0f * st.inv_n, d->background};
+                       st.tri_count, 2.0f * st.inv_n, d->background, st.aux_rec, st.aux_n, L.aux_per_quad};
     DS_TRY(mark(d, s, DS_STAGE_RESOLVE, true), "event");
     const MaskSeed seed{d->seed, reinterpret_cast<const unsigned long long*>(d->seed_device)};
     DS_TRY(launch_raster_resolve_shade(st.v, d->T, a, d->stochastic ? kMaskHash : kMaskDeterministic, seed,
@@ -663,7 +674,9 @@ int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, b
     DS_TRY(mark(d, s, DS_STAGE_EDGE, false), "event");
     DS_TRY(mark(d, s, DS_STAGE_AUX, true), "event");
     AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(L.Sa)};
-    DS_TRY(launch_opacity_aux(st.v, d->T, a, d->rast, d->pix_loss, st.galpha, st.rw, s), "opacity aux");
+    DS_TRY(launch_opacity_aux_records(st.v, d->T, a, d->rast, d->pix_loss, st.galpha, st.rw, st.aux_rec,
+                                      st.aux_n, L.aux_per_quad, s),
+           "opacity aux");
     DS_TRY(mark(d, s, DS_STAGE_AUX, false), "event");
     DS_TRY(mark(d, s, DS_STAGE_FINAL, true), "event");
     {

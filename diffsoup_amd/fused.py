@@ -48,7 +48,7 @@ class StepDesc(ctypes.Structure):
         ("seed_device", ctypes.c_void_p), ("feat_bf16", ctypes.c_void_p),
         ("loss_kind", ctypes.c_int), ("advance_seed", ctypes.c_int),
         ("accumulate", ctypes.c_int), ("alpha_lo", ctypes.c_int), ("sh2", ctypes.c_int),
-        ("inv_mvp", ctypes.c_void_p),
+        ("inv_mvp", ctypes.c_void_p), ("aux_records", ctypes.c_int64),
     ]
 
 
@@ -117,11 +117,16 @@ class FusedStep:
 
     ``sh2=True`` appends the 9 SH2 values of each pixel's view direction to
     the MLP input (``run`` then takes ``inv_mvp``); the reference's network
-    is ``accumulate=True, channels=7, hidden=16, sh2=True``."""
+    is ``accumulate=True, channels=7, hidden=16, sh2=True``.
+
+    ``aux_records`` is the number of fragment records per 8x8 pixel quad the
+    workspace keeps for the opacity gradient (0: 128, two per pixel); a quad
+    whose fragments overflow them computes that gradient by a second walk of
+    its tile's bins (the same result, slower)."""
 
     def __init__(self, resolutions, T, V, alpha_level, levels, channels, hidden, *, bf16=False,
                  bf16_features=False, stochastic=True, tile=16, background=1.0, device="cuda", capacity=None,
-                 loss="mse", accumulate=False, alpha_lo=None, sh2=False):
+                 loss="mse", accumulate=False, alpha_lo=None, sh2=False, aux_records=0):
         self.dev = torch.device(device)
         self.res = [(int(h), int(w)) for h, w in resolutions]
         self.B = len(self.res)
@@ -161,6 +166,7 @@ class FusedStep:
         self._Ha = _lib.int_array([h for h, _ in self.res])
         self._Wa = _lib.int_array([w for _, w in self.res])
         self.capacity = int(capacity or max(4 * self.B * self.T, 1 << 16))
+        self.aux_records = int(aux_records)
         self.desc = StepDesc()
         self._fill_static()
         nbytes = int(L.ds_step_workspace_size(ctypes.byref(self.desc)))
@@ -187,6 +193,7 @@ class FusedStep:
                      "grad_alpha_logits", "grad_feat", "grad_mlp", "loss"):
             setattr(d, name, getattr(self, name).data_ptr())
         d.capacity = self.capacity
+        d.aux_records = self.aux_records
 
     def mlp_grads(self):
         """Views of grad_mlp as (dW1, db1, dW2, db2, dW3, db3)."""
@@ -364,7 +371,10 @@ class FusedStep:
         resolve: rast written for every pixel (16 B), background pixels read the
         target and write colour, colour gradient and L1 (40 B), covered pixels
         append their index (4 B), every (view, triangle) record is read once
-        (64 B) and every bin entry once (4 B).
+        (64 B) and every bin entry once (4 B), and every covered fragment is
+        listed for the opacity gradient (16 B; the fragments found behind
+        their pixel's surface during the walk, a few per cent, are not, so
+        this counts slightly high).
         shade: per covered pixel 60 B (rast, index, target, colour, colour
         gradient, L1) plus one read of the feature table (fp32 or bf16) and
         one write of its fp32 gradient.
@@ -372,20 +382,26 @@ class FusedStep:
         and colour gradient (40 B), each read once (a neighbour's values are
         another pixel's own reads), plus the clip-position gradient (16 B
         per view-vertex).
+        aux: per listed fragment (counted as every covered fragment) its
+        record (16 B) and its pixel's rast and L1 (20 B), plus one read of the
+        opacities and one write of their gradient (8 B per lattice sample).
         """
         P = self.P
         cov = self.covered_pixels()
-        pairs = int(self.stats()[1].item())
+        stats = self.stats()
+        pairs = int(stats[1].item())
+        frags = int(stats[0].item())
         n_feat = float(self.T * self.S_tot * self.C)
         feat_read = (2.0 if self.bf16_features else 4.0) * n_feat
         if stage == "resolve":
-            return 16.0 * P + 40.0 * (P - cov) + 4.0 * cov + 64.0 * self.B * self.T + 4.0 * pairs
+            return (16.0 * P + 40.0 * (P - cov) + 4.0 * cov + 64.0 * self.B * self.T + 4.0 * pairs
+                    + 16.0 * frags)
         if stage == "shade":
             return 60.0 * cov + feat_read + 4.0 * n_feat
         if stage == "edge":
             return (16.0 + 40.0) * cov + 16.0 * self.B * self.V
         if stage == "aux":
-            return 16.0 * P + 4.0 * P + 64.0 * self.B * self.T + 4.0 * pairs + 8.0 * self.T * self.Sa
+            return 36.0 * frags + 8.0 * self.T * self.Sa
         raise ValueError(stage)
 
     def iteration_bytes(self, adam: bool = True) -> float:

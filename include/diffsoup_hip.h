@@ -42,6 +42,9 @@ extern "C" {
 #define DS_STAT_PAIRS 1    /* (triangle, tile) pairs produced by binning       */
 #define DS_STAT_OVERFLOW 2 /* 1 if pairs exceeded the bin capacity             */
 #define DS_STAT_BAD_INDEX 3 /* 1 if a triangle indexed a vertex outside [0,V)  */
+#define DS_STAT_AUX_WALK_QUADS 4 /* training step: 8x8 quads whose opacity-
+                                    gradient records overflowed (ds_step_desc.
+                                    aux_records) and were walked again   */
 
 /* Library version string.  Replaces _core.__version__ (src/main.cpp:26). */
 const char* ds_version(void);
@@ -262,6 +265,14 @@ typedef struct ds_step_desc {
                                     src/cuda/encoding.cu:9-130) to the MLP
                                     input after the features                  */
     const float* inv_mvp;        /* with sh2: [B,4,4] row-major inverse MVPs  */
+    int64_t aux_records;         /* fragment records per 8x8 pixel quad the
+                                    workspace holds for the opacity gradient
+                                    (0: 128, two per pixel): the resolve lists
+                                    every fragment that may lie in front of
+                                    its pixel's surface, and a quad whose
+                                    fragments overflow its slots walks its
+                                    tile's bins again instead (the same
+                                    gradient, more time)                     */
 } ds_step_desc;
 
 #define DS_LOSS_MSE 0

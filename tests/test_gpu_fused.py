@@ -28,14 +28,15 @@ def _close(got, ref, rel, what):
     assert err <= rel * scale, f"{what}: max err {err:.3e} > {rel:.1e} x {scale:.3e}"
 
 
-def _run(dev, s, bf16, seed=5, bf16_features=False, loss="mse"):
+def _run(dev, s, bf16, seed=5, bf16_features=False, loss="mse", aux_records=0):
     from diffsoup_amd import fused
     prm = fused.SoupParams.from_soup(s, dev)
     if bf16_features:
         prm.with_bf16_features()
     nl = s.levels[1] - s.levels[0] + 1
     st = fused.FusedStep(s.resolutions, s.T, s.verts.shape[0], s.alpha_level, s.levels, s.feat.shape[2],
-                         s.mlp_dims[1], bf16=bf16, bf16_features=bf16_features, device=dev, loss=loss)
+                         s.mlp_dims[1], bf16=bf16, bf16_features=bf16_features, device=dev, loss=loss,
+                         aux_records=aux_records)
     loss = st.run(prm, torch.from_numpy(s.mvp).to(dev), torch.from_numpy(s.target).to(dev), seed)
     torch.cuda.synchronize()
     out = {k: getattr(st, k).cpu().numpy() for k in ("pos", "alpha", "rast", "color", "grad_color",
@@ -115,6 +116,29 @@ def test_fused_step_matches_reference(dev, small, medium, which, bf16, bf16_feat
     _close(g["grad_feat"], ref["grad_feat"], rel, "feature grads")
     _close(g["grad_verts"], ref["grad_verts"], 1e-4, "vertex grads")
     _close(g["grad_alpha_logits"], ref["grad_alpha_logits"], 1e-4, "opacity logit grads")
+
+
+def test_opacity_records_overflow_takes_the_bin_walk(dev, medium):
+    """The opacity gradient from the resolve's per-tile fragment records and,
+    for the tiles whose records overflow deliberately few slots, from a
+    second walk of their bins: the same gradient (up to the order of float
+    atomics), within the oracle's bar, every other output identical."""
+    g = _run(dev, medium, False)
+    walked = int(g["stats"][4])
+    print(f"tiles walked with the default slots: {walked}", flush=True)
+    assert walked == 0   # two slots per pixel hold this scene's records
+    g2 = _run(dev, medium, False, aux_records=4)
+    walked2 = int(g2["stats"][4])
+    print(f"tiles walked with 4 slots: {walked2}", flush=True)
+    assert walked2 > 0
+    for k in ("rast", "color"):
+        np.testing.assert_array_equal(g[k], g2[k], err_msg=k)
+    for k in ("grad_feat", "grad_mlp", "grad_verts"):
+        _close(g2[k], g[k], 1e-6, k)
+    _close(g2["grad_alpha_logits"], g["grad_alpha_logits"], 1e-5, "opacity logit grads (walk vs records)")
+    ref = O.reference_step(medium, 5, pos=g["pos"], alpha=g["alpha"], color_in=g["color"],
+                           grad_color_in=g["grad_color"])
+    _close(g2["grad_alpha_logits"], ref["grad_alpha_logits"], 1e-4, "opacity logit grads (walk)")
 
 
 def test_fused_fd_continuous_params(dev, small):
