@@ -98,6 +98,12 @@ struct Cfg {
     // per SIMD, 256 registers each) up to hidden width 32; the 80->64->64->3
     // configuration needs one wave's full 512 registers
     static constexpr int MIN_BLOCKS = DH <= 32 ? 2 : 1;
+    // claim rounds of the feature-gradient scatter before the remaining
+    // conflicts fall back to LDS atomics: a second round settles most of the
+    // conflicts at the coarser fine levels (eight 800x800 views, levels 0-3:
+    // shading 169 -> 160 us) but costs the 80->64->64->3 configuration
+    // registers it does not have (2.1 -> 3.0 ms)
+    static constexpr int CLAIM_ROUNDS = DH <= 32 ? 2 : 1;
     // per-wave LDS staging for the weight-gradient transposes: one (delta,
     // activation) pair at a time - (D1, X), (D2, H1), (D3, H2).  The f32
     // products stage floats as [pixel][unit] rows (width + 4 apart); the bf16
@@ -568,12 +574,12 @@ k_shade(ShadeArgs a, int rowp) {
             // reduced across the 16 pixel lanes once at its end.  Finer
             // levels add into the LDS row.  LDS float atomics are slow there
             // (the LDS stays busy ~2 cycles per lane), and at those levels the
-            // pixels of a tile rarely share a lattice sample, so each lane
+            // pixels of a pass share fewer lattice samples, so each lane
             // first claims its 4-channel block (it writes its lane id into
             // the block's claim word and reads it back); the one lane that
             // reads back its own id owns the block and adds with a plain
-            // 16-byte read-modify-write, and the rare losers add with
-            // atomics afterwards.
+            // 16-byte read-modify-write; the losers claim again (CLAIM_ROUNDS)
+            // and what is left adds with atomics.
             for (int tt = 0; tt < NI; ++tt) {
                 const int f0 = 16 * tt + 4 * g;
                 const bool in = valid && f0 < DF;
@@ -615,28 +621,36 @@ k_shade(ShadeArgs a, int rowp) {
                 if (__ballot(fine) == 0ull) continue;   // wave-uniform
                 int slot[3];
                 for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<CP>(loff[lv] + cr.idx[k], c) : 0;
-                if (fine)
+                // each round: every pending lane claims its blocks, the owners
+                // add and are done, the losers try again in the next round
+                bool pend[3] = {fine, fine, fine};
+#pragma unroll
+                for (int round = 0; round < K::CLAIM_ROUNDS; ++round) {
+                    if (round > 0 && __ballot(pend[0] || pend[1] || pend[2]) == 0ull) break;   // wave-uniform
                     for (int k = 0; k < 3; ++k)
-                        __hip_atomic_store(claim + (slot[k] >> 2), lane, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
-                wave_lds_sync();
-                bool own[3];
+                        if (pend[k])
+                            __hip_atomic_store(claim + (slot[k] >> 2), lane, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    wave_lds_sync();
+                    bool own[3];
+                    for (int k = 0; k < 3; ++k)
+                        own[k] = pend[k] && __hip_atomic_load(claim + (slot[k] >> 2), __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WAVEFRONT) == lane;
+                    for (int k = 0; k < 3; ++k)
+                        if (own[k]) {
+                            float4* q = reinterpret_cast<float4*>(grow + slot[k]);
+                            float4 v = *q;
+                            v.x += cr.w[k] * dx[tt][0];
+                            v.y += cr.w[k] * dx[tt][1];
+                            v.z += cr.w[k] * dx[tt][2];
+                            v.w += cr.w[k] * dx[tt][3];
+                            *q = v;
+                            pend[k] = false;
+                        }
+                    wave_lds_sync();
+                }
                 for (int k = 0; k < 3; ++k)
-                    own[k] = fine && __hip_atomic_load(claim + (slot[k] >> 2), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WAVEFRONT) == lane;
-                for (int k = 0; k < 3; ++k)
-                    if (own[k]) {
-                        float4* q = reinterpret_cast<float4*>(grow + slot[k]);
-                        float4 v = *q;
-                        v.x += cr.w[k] * dx[tt][0];
-                        v.y += cr.w[k] * dx[tt][1];
-                        v.z += cr.w[k] * dx[tt][2];
-                        v.w += cr.w[k] * dx[tt][3];
-                        *q = v;
-                    }
-                wave_lds_sync();
-                for (int k = 0; k < 3; ++k)
-                    if (fine && !own[k])
+                    if (pend[k])
                         for (int r = 0; r < 4; ++r) atomicAdd(grow + slot[k] + r, cr.w[k] * dx[tt][r]);
             }
 
