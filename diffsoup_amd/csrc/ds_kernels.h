@@ -17,7 +17,9 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-constexpr int kMaxViews = 64;
+// Views per launch: the view table below travels as a kernel argument
+// (about 5.3 KB at 256 views, the largest argument block of any kernel here).
+constexpr int kMaxViews = 256;
 
 // Per-launch description of the views packed into one batch.  Passed to the
 // kernels by value, so a captured graph keeps its own copy.  Views may have

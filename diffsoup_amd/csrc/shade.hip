@@ -306,10 +306,18 @@ __device__ __forceinline__ void copy_batched(T* dst, const T* __restrict__ src, 
 }
 
 // e[k] for a lane-varying k in [0, 9) without dynamic register indexing.
+// Each value passes through an empty asm first: a chain of selects between
+// loads of one array would otherwise be folded back into one load at a
+// lane-varying address, and the array kept in scratch memory.
 __device__ __forceinline__ float pick9(const float (&e)[9], int k) {
     float v = e[0];
+    asm("" : "+v"(v));
 #pragma unroll
-    for (int i = 1; i < 9; ++i) v = k == i ? e[i] : v;
+    for (int i = 1; i < 9; ++i) {
+        float ei = e[i];
+        asm("" : "+v"(ei));
+        v = k == i ? ei : v;
+    }
     return v;
 }
 

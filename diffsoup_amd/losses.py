@@ -60,7 +60,7 @@ class _Views:
         self.windows = (H - kh + 1) * (W - kw + 1)
 
 
-MAX_VIEWS = 64   # images per launch of the SSIM kernels; larger batches run in chunks
+MAX_VIEWS = 256   # images per launch of the SSIM kernels; larger batches run in chunks
 
 
 def _fwd(views, C, X, Y, win, c1, c2):

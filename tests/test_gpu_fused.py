@@ -191,18 +191,23 @@ def test_fused_fd_continuous_params(dev, small):
     assert checked == 14
 
 
-@pytest.mark.parametrize("which", ["small", "uneven", "medium"])
+@pytest.mark.parametrize("which", ["small", "uneven", "medium", "many_views"])
 def test_fused_step_l1_ssim_loss(dev, small, medium, which):
     """The reference's image loss 0.8 L1 + 0.2 (1 - SSIM) / 2 inside the fused
     step (colour pass, SSIM kernels, adjoint pass) against the oracle step
     with the loss and its colour gradient in float64 (SSIM per view, averaged
-    over views).  Tolerances: colour 1e-5 and loss 1e-5 relative (fp32), the
+    over views; "many_views" packs 200 views of mixed resolution into one
+    step, above the 64 of one SSIM launch of the per-op API).  Tolerances:
+    colour 1e-5 and loss 1e-5 relative (fp32), the
     colour gradient 1e-3 of its largest magnitude (fp32 SSIM moments), the
     parameter gradients 2e-3 (they inherit it); edge and opacity gradients
     1e-4 on the step's own colour and colour gradient."""
-    s = {"small": small, "medium": medium,
-         "uneven": scenes.batched_uneven_soup(5, T=4000, views=5, res_scale=0.2, edge_scale=5.0)}[which]
-    print(f"fused step with L1 + SSIM: {s.name} views {s.resolutions}", flush=True)
+    if which == "many_views":
+        s = scenes.batched_uneven_soup(6, T=3000, views=200, res_scale=0.06, edge_scale=12.0)
+    else:
+        s = {"small": small, "medium": medium,
+             "uneven": scenes.batched_uneven_soup(5, T=4000, views=5, res_scale=0.2, edge_scale=5.0)}[which]
+    print(f"fused step with L1 + SSIM: {s.name}, {len(s.resolutions)} views, first {s.resolutions[:4]}", flush=True)
     g = _run(dev, s, False, seed=7, loss="l1_ssim")
     rast = np.concatenate([O.rasterize((h, w), g["pos"][b:b + 1], s.tri, s.alpha_level, g["alpha"], True, 7,
                                        view_base=b).reshape(-1, 4) for b, (h, w) in enumerate(s.resolutions)])
