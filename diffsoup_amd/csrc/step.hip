@@ -96,22 +96,31 @@ struct AccumJob {
     int lo, hi, target, C, S_tot, S_T;
 };
 
-// Copy n floats of a row into LDS with eight loads in flight per thread
-// before the first is waited on (a row is a few thousand floats, so one load
-// per iteration would leave each thread waiting out the memory latency
-// once per element).
-__device__ __forceinline__ void stage_row(float* dst, const float* __restrict__ src, int n) {
+// Two source rows into LDS as one: n0 floats of s0 to d0, then n1 of s1 to
+// d1, with eight loads in flight per thread across both before the first is
+// waited on (a row is a few thousand floats, so one load per iteration would
+// leave each thread waiting out the memory latency once per element).
+__device__ __forceinline__ void stage_rows2(float* d0, const float* __restrict__ s0, int n0, float* d1,
+                                            const float* __restrict__ s1, int n1) {
     constexpr int U = 8;
-    const int nt = blockDim.x;
+    const int nt = blockDim.x, n = n0 + n1;
     int i = threadIdx.x;
     for (; i + (U - 1) * nt < n; i += U * nt) {
         float v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = src[i + u * nt];
+        for (int u = 0; u < U; ++u) {
+            const int e = i + u * nt;
+            v[u] = e < n0 ? s0[e] : s1[e - n0];
+        }
 #pragma unroll
-        for (int u = 0; u < U; ++u) dst[i + u * nt] = v[u];
+        for (int u = 0; u < U; ++u) {
+            const int e = i + u * nt;
+            if (e < n0) d0[e] = v[u]; else d1[e - n0] = v[u];
+        }
     }
-    for (; i < n; i += nt) dst[i] = src[i];
+    for (; i < n; i += nt) {
+        if (i < n0) d0[i] = s0[i]; else d1[i - n0] = s1[i - n0];
+    }
 }
 
 // Forward of both accumulations, one workgroup per triangle (grid-stride):
@@ -123,11 +132,9 @@ __device__ __forceinline__ void stage_row(float* dst, const float* __restrict__ 
 // reference applies torch's fp32 sigmoid).  C is a template argument so the
 // index arithmetic folds; CG is the generic runtime-width fallback.
 template <int C>
-__device__ __forceinline__ void accum_row_fwd(const AccumJob& J, int t, float* row, bool exact_sigmoid) {
+__device__ __forceinline__ void accum_row_fwd(const AccumJob& J, int t, const float* row, bool exact_sigmoid) {
     const int c = C > 0 ? C : J.C;
-    const int n_src = J.S_tot * c, n_dst = J.S_T * c;
-    stage_row(row, J.src + (long long)t * n_src, n_src);
-    __syncthreads();
+    const int n_dst = J.S_T * c;
     const float Rf = static_cast<float>(1u << J.target);
     float* dst = J.dst + (long long)t * n_dst;
     constexpr int CA = C > 0 ? C : 16;
@@ -165,16 +172,70 @@ __device__ __forceinline__ void accum_row_fwd(const AccumJob& J, int t, float* r
                                                : 1.0f / (1.0f + expf(-v[jj]));
         }
     }
-    __syncthreads();
 }
 
+// Forward of the feature accumulation by prolongation, in place in the
+// staged source row: with r_lo = src_lo, r_(l+1) = src_(l+1) + P r_l, where P
+// interpolates level l linearly onto level l + 1 (a level-(l+1) vertex is a
+// level-l vertex, or the midpoint of a level-l edge: the average of its two
+// ends, the cells split along the anti-diagonal as in lattice_corners).
+// r_hi is the accumulation at the target vertices (the same hat-function sums
+// as the corner form, grouped level by level), left contiguous in LDS, so
+// the sigmoid's output row is stored in one coalesced pass.  A thread takes
+// a fine vertex and all its channels; per level two LDS reads per channel at
+// most, against three corners per level at every target vertex in the
+// corner form.  Features only: the opacities keep the corner form above.
+template <int C>
+__device__ __forceinline__ void accum_feat_fwd(const AccumJob& J, int t, float* row) {
+    const int c = C > 0 ? C : J.C;
+    const int n_dst = J.S_T * c;
+    constexpr int CA = C > 0 ? C : 16;
+    int off = 0;   // first sample of level l in the row
+    for (int l = J.lo; l < J.hi; ++l) {
+        const int off_f = off + static_cast<int>(feats_at_level(static_cast<uint32_t>(l)));
+        const int S_f = static_cast<int>(feats_at_level(static_cast<uint32_t>(l + 1)));
+        for (int k = threadIdx.x; k < S_f; k += blockDim.x) {
+            uint32_t X, Y;
+            lattice_xy(static_cast<uint32_t>(k), X, Y);
+            // the level-l vertices a (and b) whose average is P r_l here
+            uint32_t ax, ay, bx, by;
+            if (!(X & 1u) && !(Y & 1u)) { ax = bx = X >> 1; ay = by = Y >> 1; }
+            else if (!(Y & 1u)) { ax = X >> 1; bx = ax + 1u; ay = by = Y >> 1; }       // X odd
+            else if (!(X & 1u)) { ax = bx = X >> 1; ay = Y >> 1; by = ay + 1u; }       // Y odd
+            else { ax = (X + 1u) >> 1; ay = Y >> 1; bx = X >> 1; by = (Y + 1u) >> 1; }  // both odd
+            const uint32_t ka = (ax + ay) * (ax + ay + 1u) / 2u + ay, kb = (bx + by) * (bx + by + 1u) / 2u + by;
+            const float* pa = row + (off + static_cast<int>(ka)) * c;
+            const float* pb = row + (off + static_cast<int>(kb)) * c;
+            float* q = row + (off_f + k) * c;
+            for (int j0 = 0; j0 < c; j0 += CA) {
+#pragma unroll
+                for (int jj = 0; jj < CA; ++jj)
+                    if (C > 0 || j0 + jj < c) q[j0 + jj] += 0.5f * (pa[j0 + jj] + pb[j0 + jj]);
+            }
+        }
+        __syncthreads();
+        off = off_f;
+    }
+    float* dst = J.dst + (long long)t * n_dst;
+    const float* r = row + off * c;
+    for (int i = threadIdx.x; i < n_dst; i += blockDim.x) dst[i] = 1.0f / (1.0f + expf(-r[i]));
+}
+
+// Both source rows of a triangle are staged together (features, then the
+// opacities after them), so a triangle waits out one round of loads.
 __global__ void __launch_bounds__(256) k_accum_sigmoid(int T, AccumJob f, AccumJob a) {
     extern __shared__ float lds_row[];
+    const int nf = f.S_tot * f.C, na = a.S_tot;
+    float* frow = lds_row;
+    float* arow = lds_row + nf;
     for (int t = blockIdx.x; t < T; t += gridDim.x) {
-        if (f.C == 7) accum_row_fwd<7>(f, t, lds_row, false);
-        else if (f.C == 8) accum_row_fwd<8>(f, t, lds_row, false);
-        else accum_row_fwd<0>(f, t, lds_row, false);
-        accum_row_fwd<1>(a, t, lds_row, true);
+        stage_rows2(frow, f.src + (long long)t * nf, nf, arow, a.src + (long long)t * na, na);
+        __syncthreads();
+        if (f.C == 7) accum_feat_fwd<7>(f, t, frow);
+        else if (f.C == 8) accum_feat_fwd<8>(f, t, frow);
+        else accum_feat_fwd<0>(f, t, frow);
+        accum_row_fwd<1>(a, t, arow, true);
+        __syncthreads();   // the next triangle's staging overwrites both rows
     }
 }
 
@@ -251,6 +312,9 @@ __device__ __forceinline__ void accum_row_bwd(const AccumJob& J, int t, float* l
         }
         __syncthreads();
     }
+    // with a single level only the identity copy read the row: order those
+    // reads before the next job's (or triangle's) writes of it
+    if (J.lo == J.hi) __syncthreads();
 }
 
 __global__ void __launch_bounds__(256) k_accum_sigmoid_bwd(int T, AccumJob f, AccumJob a) {
@@ -467,6 +531,7 @@ struct Step {
 };
 
 int step_prepare(const ds_step_desc* d, void* stream, Step& st);
+size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L);
 int enqueue_forward(const ds_step_desc* d, Step& st);
 int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ext_grad);
 int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, bool write_loss);
@@ -497,9 +562,10 @@ int step_prepare(const ds_step_desc* d, void* stream, Step& st) {
     StepLayout& L = st.L;
     if (!layout(d, st.v, L))
         return step_fail(DS_ERR_UNSUPPORTED, "no compiled shading kernel for this (levels, channels, hidden, sh2)");
-    if (d->accumulate && 4 * std::max<long long>(L.S_src * d->channels, L.Sa_src) > 160 * 1024)
-        return step_fail(DS_ERR_UNSUPPORTED, "accumulate: a triangle's source row (levels x channels) must fit "
-                                             "160 KB of LDS (levels up to 5 with 7 channels do)");
+    if (d->accumulate && accum_lds_bytes(d, L) > 160 * 1024)
+        return step_fail(DS_ERR_UNSUPPORTED, "accumulate: a triangle's two source rows (levels x channels, and "
+                                             "the opacity levels) must fit 160 KB of LDS (levels up to 6 with 7 "
+                                             "channels do)");
     if (!d->workspace || d->workspace_bytes < L.total)
         return step_fail(DS_ERR_WORKSPACE, "workspace smaller than ds_step_workspace_size");
     if (reinterpret_cast<uintptr_t>(d->workspace) & 255u) return step_fail(DS_ERR_INVALID, "workspace must be 256-byte aligned");
@@ -553,14 +619,14 @@ hipError_t mark(const ds_step_desc* d, hipStream_t s, int stage, bool begin) {
     return ev ? hipEventRecord(static_cast<hipEvent_t>(ev), s) : hipSuccess;
 }
 
-// LDS of the accumulation kernels: one source row (forward) or the target
-// row plus the level below it (backward) of the larger of the two jobs.
+// LDS of the accumulation kernels: both source rows (forward), or the
+// target row plus the level below it of the larger job (backward).
 size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L) {
     const size_t below = d->feat_hi > 0 ? feats_at_level(d->feat_hi - 1) : 0;
     const size_t below_a = d->alpha_level > 0 ? feats_at_level(d->alpha_level - 1) : 0;
-    return sizeof(float) * std::max(std::max<size_t>((size_t)L.S_src * d->channels, (size_t)L.Sa_src),
-                                    std::max<size_t>((size_t)(L.S_shade + below) * d->channels,
-                                                     (size_t)L.Sa + below_a));
+    return sizeof(float) * std::max<size_t>((size_t)L.S_src * d->channels + (size_t)L.Sa_src,
+                                            std::max<size_t>((size_t)(L.S_shade + below) * d->channels,
+                                                             (size_t)L.Sa + below_a));
 }
 
 // Rows beyond the default 64 KB of LDS need each kernel's opt-in, once.
