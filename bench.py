@@ -536,7 +536,7 @@ def summarize_rocprof(stats_csv, fetch_csv, write_csv, out_md):
     fs, ws = 262144.0 / cal["FETCH_SIZE"], 262144.0 / cal["WRITE_SIZE"]
     cmd_stats = ("cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
                  "-d $REPO/gpurun_out/prof_bench -o bench -- python3 $REPO/bench.py --steps 20 --warmup 3 "
-                 "--no-cpu-baseline --no-plain")
+                 "--no-cpu-baseline --no-plain --no-workloads")
     cmd_pmc = "\n".join(
         f"cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc {c} --output-format csv -d $REPO/gpurun_out/pmc_{c} "
         f"-o run -- python3 $REPO/bench.py --profile-driver" for c in ("FETCH_SIZE", "WRITE_SIZE"))
