@@ -47,6 +47,9 @@ struct ShadeArgs {
 bool shade_supported(int nl, int c, int dh, bool sh);
 size_t shade_blob_floats(int nl, int c, int dh, bool sh);
 int shade_nparam(int nl, int c, int dh, bool sh);
+// Resident shading workgroups per compute unit of a compiled configuration
+// (the kernel's register budget: two up to hidden width 32, else one).
+int shade_blocks_per_cu(int nl, int c, int dh, bool sh);
 hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const float* w1, const float* w2,
                               const float* w3, float* blobs, hipStream_t s);
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,

@@ -7,9 +7,9 @@
 // contiguous ranges, one per wave.  For each triangle segment in its range a
 // wave stages the triangle's feature row (all levels, all channels) in LDS,
 // shades the segment's pixels (any view) in tiles of 16, accumulates the input
-// gradient into an LDS gradient row (the two coarsest lattice levels in
-// per-lane registers reduced at the end of the segment, finer levels with LDS
-// atomics), and writes that row once - with plain coalesced stores when the
+// gradient into an LDS gradient row (lattice levels 0-2 as fp32 MFMA products
+// accumulated in registers over the segment, finer levels by claimed LDS
+// read-modify-writes), and writes that row once - with plain coalesced stores when the
 // triangle lies wholly in the range, with global atomics only for the few
 // triangles split between two ranges.  Every layer is an MFMA product with
 // the layer's output units on the M axis and the 16 pixels on the N axis.  In
@@ -103,7 +103,15 @@ struct Cfg {
     // conflicts at the coarser fine levels (eight 800x800 views, levels 0-3:
     // shading 169 -> 160 us) but costs the 80->64->64->3 configuration
     // registers it does not have (2.1 -> 3.0 ms)
-    static constexpr int CLAIM_ROUNDS = DH <= 32 ? 2 : 1;
+    static constexpr int CLAIM_ROUNDS = 2;
+    // one wave per SIMD (512 registers): lattice levels 0-2 scatter by fp32
+    // MFMA, with their accumulators and the bias gradients' block in
+    // registers (80->64->64->3 on 100K triangles: shading 2.13 -> 1.77 ms).
+    // At two waves per SIMD the per-lane coarse sums, claims and per-lane
+    // bias sums are faster (eight 800x800 views, levels 0-3, 32-wide MLP:
+    // 159 us against 177-192 us with the MFMA forms): fp32 MFMA issue is
+    // what the smaller networks are short of.
+    static constexpr bool MF = MIN_BLOCKS == 1;
     // per-wave LDS staging for the weight-gradient transposes: one (delta,
     // activation) pair at a time - (D1, X), (D2, H1), (D3, H2).  The f32
     // products stage floats as [pixel][unit] rows (width + 4 apart); the bf16
@@ -121,6 +129,19 @@ struct Cfg {
 };
 
 constexpr int up16(int b) { return (b + 15) & ~15; }
+
+// Lattice levels up to this one (3, 6 and 15 samples: one 16-row MFMA block
+// each) scatter their feature gradient by MFMA; finer levels by claims.
+constexpr int kMfmaTopLevel = 2;
+// What that scatter stages, one level at a time, in the per-wave stage
+// region: the level's input tile of dX transposed ([16 pixels][16] floats,
+// rows 20 apart) and its 16 pixels' corner records (three indices, three
+// weights: 32 B each).
+template <class K>
+struct MfPlan {
+    static constexpr int NMF = K::NL < kMfmaTopLevel + 1 ? K::NL : kMfmaTopLevel + 1;
+    static constexpr int BYTES = 16 * 20 * 4 + 16 * 32;
+};
 
 // Feature-row length (floats) of levels 0 .. nl-1 with c channels.
 constexpr int row_floats_from_zero(int nl, int c) {
@@ -140,7 +161,9 @@ struct Plan {
     using FT = typename std::conditional<FB, unsigned short, float>::type;
     __host__ __device__ static constexpr int frow_bytes(int rowp) { return up16(rowp * static_cast<int>(sizeof(FT))); }
     __host__ __device__ static constexpr int stage_bytes() {
-        return up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)));
+        return !K::MF || up16(K::STAGE * static_cast<int>(sizeof(typename K::ST))) > MfPlan<K>::BYTES
+                   ? up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)))
+                   : MfPlan<K>::BYTES;
     }
     // feature row, gradient row, transpose stage, and (CLAIM) the claim table
     // of the gradient row's 4-channel blocks (one int each: rowp bytes)
@@ -242,6 +265,25 @@ struct Mma<true> {
     }
 };
 
+// acc[i][n] += sum over the 16 staged pixels of unit ua + i of transpose
+// region A (width wa), for column n == col only: the B operand is that
+// column's indicator.  Bias gradients contract a delta with ones.
+template <class K>
+__device__ __forceinline__ f32x4 stage_sum(const typename K::ST* A, int wa, int ua, int col, int lane, f32x4 acc) {
+    const int i = lane % 16, g = lane / 16;
+    if constexpr (K::BF16) {
+        const i16x4 a = *reinterpret_cast<const i16x4*>(A + (ua + i) * K::SPX + 4 * g);
+        const short one = i == col ? static_cast<short>(0x3f80) : static_cast<short>(0);   // bf16 1.0
+        const i16x4 b = {one, one, one, one};
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    } else {
+        const float one = i == col ? 1.f : 0.f;
+        for (int s = 0; s < 4; ++s)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[(4 * s + g) * (wa + 4) + ua + i], one, acc, 0, 0, 0);
+        return acc;
+    }
+}
+
 // Offset (in floats) of a layer tile pair inside the blob table.
 template <class K>
 __device__ __forceinline__ const float* blob_at(const float* blobs, int layer_base, int m, int nin, int t) {
@@ -321,6 +363,30 @@ __device__ __forceinline__ float pick9(const float (&e)[9], int k) {
     return v;
 }
 
+// Adds the MFMA scatter blocks of the levels 0 .. n-1 of the row (n = the
+// number of levels at or below kMfmaTopLevel) into the LDS gradient row.
+// Lane (g, n) holds G[4 g + r][n] of block q: sample 4 g + r of level q,
+// feature column n of the level's input tile; each (sample, channel) of these
+// levels is held by exactly one lane and added by nothing else, so a plain
+// read-modify-write.
+template <class K, int NMF>
+__device__ __forceinline__ void add_mf_blocks(const f32x4 (&acc)[NMF], int n, int lo, const uint32_t* loff,
+                                              float* grow, int g, int j) {
+    constexpr int C = K::C, CP = K::CP;
+#pragma unroll
+    for (int q = 0; q < NMF; ++q) {
+        if (q >= n) break;
+        const int tq = q * C / 16, c0 = q * C - 16 * tq;
+        const int S = static_cast<int>(feats_at_level(lo + q));
+        const int ch = j - c0;
+        if (ch >= 0 && ch < C)
+            for (int r = 0; r < 4; ++r) {
+                const int sm = 4 * g + r;
+                if (sm < S) grow[grad_slot<CP>(loff[q] + sm, ch & ~3) + (ch & 3)] += acc[q][r];
+            }
+    }
+}
+
 // SPLIT selects the two-pass forms (a.fwd_only, a.ext_grad); the fused
 // single-pass instance carries neither test.  DIRECT rows are not staged: a
 // triangle's row that does not fit the LDS plan (single-level rows at high
@@ -331,6 +397,8 @@ template <class K, bool FB, bool SPLIT, bool DIRECT>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock, K::MIN_BLOCKS)
 k_shade(ShadeArgs a, int rowp) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, CP = K::CP, DF = K::DF, DIN = K::DIN, DH = K::DH;
+    constexpr bool MF = K::MF;
+    constexpr int NMF = MF ? MfPlan<K>::NMF : 1;   // levels that may scatter by MFMA
     static_assert(!FB || C == CP, "bf16 rows need whole 4-channel blocks");
     static_assert(!DIRECT || K::NL == 1, "direct rows are compiled for single-level features");
     using P = Plan<K, FB>;
@@ -355,13 +423,17 @@ k_shade(ShadeArgs a, int rowp) {
     }
     __syncthreads();
 
-    f32x4 gW1[NH][NI], gW2[NH][NH], gW3[NH], gb1[NH], gb2[NH], gb3;
+    // weight-gradient accumulators; the bias gradients either per lane (gb*,
+    // reduced over the pixel lanes at the end) or (MF) in one MFMA block
+    // gbias: column m < NH holds b1's units 16 m + row, column NH + m b2's,
+    // column 2 NH (rows 0-2) b3's
+    f32x4 gW1[NH][NI], gW2[NH][NH], gW3[NH], gb1[NH], gb2[NH], gb3, gbias;
     for (int m = 0; m < NH; ++m) {
         for (int n = 0; n < NI; ++n) gW1[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int n = 0; n < NH; ++n) gW2[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
         gW3[m] = gb1[m] = gb2[m] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    gb3 = f32x4{0.f, 0.f, 0.f, 0.f};
+    gb3 = gbias = f32x4{0.f, 0.f, 0.f, 0.f};
     float loss = 0.f;
 
     // level offsets inside a triangle's feature row
@@ -387,6 +459,12 @@ k_shade(ShadeArgs a, int rowp) {
         te = a.list[r0].seg_end;
         first_whole = r0 == 0 || a.list[r0 - 1].tri != t;
     }
+    // PF: prefetch registers for the next segment's bf16 feature row (rows up
+    // to the plan's level-0 row; longer rows are copied at segment start)
+    constexpr bool PF = FB && K::MF;
+    constexpr int PF_N = PF ? (P::ROW0 / 4 + kWave - 1) / kWave : 1;
+    uint2 pf[PF_N];
+    bool have_pf = false;
     for (int pos = r0; pos < r1;) {
         const int beg = pos, end = min(te, r1);
         const int npx = end - beg;
@@ -415,8 +493,16 @@ k_shade(ShadeArgs a, int rowp) {
             }
         } else if constexpr (FB) {
             // bf16 features stay bf16 in the row: 4 values per 8-byte copy
-            copy_batched(reinterpret_cast<uint2*>(frow),
-                         reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row), row / 4, lane);
+            // (or, PF, the row the previous segment's end already loaded)
+            if (PF && have_pf) {
+#pragma unroll
+                for (int u = 0; u < PF_N; ++u)
+                    if (lane + u * kWave < row / 4) reinterpret_cast<uint2*>(frow)[lane + u * kWave] = pf[u];
+                have_pf = false;
+            } else {
+                copy_batched(reinterpret_cast<uint2*>(frow),
+                             reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t * row), row / 4, lane);
+            }
         } else if (a.feat_bf16) {
             float4* f4 = reinterpret_cast<float4*>(frow);
             // bf16 features: 4 values per 8-byte load, widened exactly to fp32
@@ -432,8 +518,13 @@ k_shade(ShadeArgs a, int rowp) {
         }
         wave_lds_sync();
 
-        float racc[6][4];
-        for (int vtx = 0; vtx < 6; ++vtx)
+        // MF: the MFMA-scattered levels' gradient blocks (samples x feature
+        // columns of the level's input tile), accumulated over the segment's
+        // tiles; otherwise the two coarsest levels' per-lane sums (racc)
+        f32x4 sacc[NMF];
+        for (int q = 0; q < NMF; ++q) sacc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+        float racc[MF ? 1 : 6][4];
+        for (int vtx = 0; vtx < (MF ? 1 : 6); ++vtx)
             for (int r = 0; r < 4; ++r) racc[vtx][r] = 0.f;
 
         // lanes of group 0 also take the pixel's target colour from the record
@@ -576,18 +667,17 @@ k_shade(ShadeArgs a, int rowp) {
 
             // ---- scatter the input gradient onto the triangle's gradient row.
             // Lane (g, j) holds dX of pixel j for 4 channels of one level and
-            // that level's lattice corners.  The two coarsest lattices (3 and
-            // 6 vertices) are shared by nearly all pixels of a tile, so they
-            // accumulate per lane in registers over the whole segment and are
-            // reduced across the 16 pixel lanes once at its end.  Finer
-            // levels add into the LDS row.  LDS float atomics are slow there
-            // (the LDS stays busy ~2 cycles per lane), and at those levels the
-            // pixels of a pass share fewer lattice samples, so each lane
-            // first claims its 4-channel block (it writes its lane id into
-            // the block's claim word and reads it back); the one lane that
-            // reads back its own id owns the block and adds with a plain
-            // 16-byte read-modify-write; the losers claim again (CLAIM_ROUNDS)
-            // and what is left adds with atomics.
+            // that level's lattice corners.  The lattices of levels 0-2 (3,
+            // 6 and 15 samples) are shared by most pixels of a tile: there
+            // the scatter is a matrix product, below.  Finer levels add into
+            // the LDS row.  LDS float atomics are slow there (the LDS stays
+            // busy ~2 cycles per lane), so each lane first claims its
+            // 4-channel block (it writes its lane id into the block's claim
+            // word and reads it back); the one lane that reads back its own
+            // id owns the block and adds with a plain 16-byte
+            // read-modify-write; the losers claim again (CLAIM_ROUNDS) and
+            // what is left adds with atomics.
+#pragma unroll
             for (int tt = 0; tt < NI; ++tt) {
                 const int f0 = 16 * tt + 4 * g;
                 const bool in = valid && f0 < DF;
@@ -607,16 +697,21 @@ k_shade(ShadeArgs a, int rowp) {
                     }
                     continue;
                 }
-                const bool coarse = tt == 0 && a.lo + lv <= 1;
-                if (in && coarse) {
-                    for (int vtx = 0; vtx < 6; ++vtx) {
-                        const float wv = (cr.idx[0] == (uint32_t)vtx ? cr.w[0] : 0.f) +
-                                         (cr.idx[1] == (uint32_t)vtx ? cr.w[1] : 0.f) +
-                                         (cr.idx[2] == (uint32_t)vtx ? cr.w[2] : 0.f);
-                        for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
+                bool fine;
+                if constexpr (MF) {
+                    fine = in && a.lo + lv > kMfmaTopLevel;
+                } else {
+                    const bool coarse = tt == 0 && a.lo + lv <= 1;
+                    if (in && coarse) {
+                        for (int vtx = 0; vtx < 6; ++vtx) {
+                            const float wv = (cr.idx[0] == (uint32_t)vtx ? cr.w[0] : 0.f) +
+                                             (cr.idx[1] == (uint32_t)vtx ? cr.w[1] : 0.f) +
+                                             (cr.idx[2] == (uint32_t)vtx ? cr.w[2] : 0.f);
+                            for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
+                        }
                     }
+                    fine = in && !coarse;
                 }
-                const bool fine = in && !coarse;
                 if constexpr (!P::CLAIM) {
                     // no room for the claim table: LDS atomics
                     if (fine)
@@ -662,9 +757,49 @@ k_shade(ShadeArgs a, int rowp) {
                         for (int r = 0; r < 4; ++r) atomicAdd(grow + slot[k] + r, cr.w[k] * dx[tt][r]);
             }
 
-            // ---- bias gradients stay per lane until the end
-            for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
-            gb3 += d3[0];
+            // ---- levels 0-2: G[s][f] += sum_p W[s][p] dX[p][f] on the fp32
+            // MFMA, samples s on M, the level's input tile's 16 feature
+            // columns f on N, the tile's pixels p on K (four steps of four).
+            // W[s][p] is pixel p's barycentric weight at sample s (the sum
+            // over its corners that are s).  dX passes through the stage
+            // region transposed ([pixel][f], rows of 20 floats) and the
+            // corners through a per-pixel table, so that lane (g, i) of step
+            // st can form W[i][4 st + g] and read dX[4 st + g][i].  Columns
+            // of other levels sharing the tile accumulate unused values.
+            const int nmf = kMfmaTopLevel + 1 - a.lo;   // levels scattered by MFMA (wave-uniform)
+            if constexpr (!DIRECT && MF) {
+                float* trn = reinterpret_cast<float*>(stage);
+                int4* cwi = reinterpret_cast<int4*>(trn + 16 * 20);
+                float4* cwf = reinterpret_cast<float4*>(cwi + 16);
+#pragma unroll
+                for (int q = 0; q < NMF; ++q) {
+                    if (q >= nmf) break;
+                    const int tq = q * C / 16, c0 = q * C - 16 * tq;
+                    if (q == 0 || tq != (q - 1) * C / 16)
+                        *reinterpret_cast<f32x4*>(trn + j * 20 + 4 * g) = dx[tq];
+                    if (4 * g == (c0 & ~3)) {
+                        const Corners& cr = crs[tq];
+                        cwi[j] = make_int4(static_cast<int>(cr.idx[0]), static_cast<int>(cr.idx[1]),
+                                           static_cast<int>(cr.idx[2]), 0);
+                        cwf[j] = make_float4(cr.w[0], cr.w[1], cr.w[2], 0.f);
+                    }
+                    wave_lds_sync();
+#pragma unroll
+                    for (int st = 0; st < 4; ++st) {
+                        const int p = 4 * st + g;
+                        const int4 ci = cwi[p];
+                        const float4 cf = cwf[p];
+                        const float w = (ci.x == j ? cf.x : 0.f) + (ci.y == j ? cf.y : 0.f) + (ci.z == j ? cf.z : 0.f);
+                        sacc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, trn[p * 20 + j], sacc[q], 0, 0, 0);
+                    }
+                    wave_lds_sync();
+                }
+            }
+            if constexpr (!MF) {
+                // bias gradients per lane until the end
+                for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
+                gb3 += d3[0];
+            }
 
             // ---- weight gradients: transpose (delta, activation) pairs through
             // LDS one at a time and contract over the 16 pixels
@@ -675,18 +810,22 @@ k_shade(ShadeArgs a, int rowp) {
                 for (int m = 0; m < NH; ++m) stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh1[m]);
                 for (int tt = 0; tt < NI; ++tt) stage_put<K>(sA, DINP, 16 * tt + 4 * g, j, x[tt]);
                 wave_lds_sync();
-                for (int m = 0; m < NH; ++m)
+                for (int m = 0; m < NH; ++m) {
                     for (int nn = 0; nn < NI; ++nn)
                         gW1[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DINP, 16 * nn, lane, gW1[m][nn]);
+                    if constexpr (MF) gbias = stage_sum<K>(sD, DH, 16 * m, m, lane, gbias);
+                }
                 wave_lds_sync();
                 for (int m = 0; m < NH; ++m) {
                     stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh2[m]);
                     stage_put<K>(sA, DH, 16 * m + 4 * g, j, h1[m]);
                 }
                 wave_lds_sync();
-                for (int m = 0; m < NH; ++m)
+                for (int m = 0; m < NH; ++m) {
                     for (int nn = 0; nn < NH; ++nn)
                         gW2[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DH, 16 * nn, lane, gW2[m][nn]);
+                    if constexpr (MF) gbias = stage_sum<K>(sD, DH, 16 * m, NH + m, lane, gbias);
+                }
                 wave_lds_sync();
                 ST* sD3 = stage;
                 ST* sH2 = stage + K::region(16);
@@ -695,6 +834,7 @@ k_shade(ShadeArgs a, int rowp) {
                 wave_lds_sync();
                 for (int m = 0; m < NH; ++m)
                     gW3[m] = stage_outer<K>(sD3, 16, 0, sH2, DH, 16 * m, lane, gW3[m]);
+                if constexpr (MF) gbias = stage_sum<K>(sD3, 16, 0, 2 * NH, lane, gbias);
                 wave_lds_sync();
             }
             rec = next;
@@ -714,9 +854,14 @@ k_shade(ShadeArgs a, int rowp) {
             continue;
         }
 
-        // ---- the coarse-level sums: reduce over the 16 pixel lanes of each
-        // lane group, then one lane per group adds them to the LDS row
-        {
+        // ---- the coarse levels' blocks into the LDS row
+        if constexpr (MF) {
+            add_mf_blocks<K, NMF>(sacc, kMfmaTopLevel + 1 - a.lo, a.lo, loff, grow, g, j);
+            for (int q = 0; q < NMF; ++q) sacc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+            wave_lds_sync();
+        } else {
+            // the two coarsest levels' sums: reduce over the 16 pixel lanes of
+            // each lane group, then one lane per group adds them to the LDS row
             const int f0 = 4 * g;
             const int lv = f0 / C, c = f0 % C;
             const bool coarse = f0 < DF && a.lo + lv <= 1;
@@ -733,6 +878,20 @@ k_shade(ShadeArgs a, int rowp) {
                     }
             }
             wave_lds_sync();
+        }
+
+        // ---- PF: the next segment's bf16 row goes out before this segment's
+        // gradient row: loads issued ahead of the stores complete without
+        // waiting for them (the memory counter is in order), so the row's
+        // latency overlaps the stores and the row clearing
+        if constexpr (PF) {
+            if (te < r1 && row / 4 <= PF_N * kWave) {
+                const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t_next * row);
+#pragma unroll
+                for (int u = 0; u < PF_N; ++u)
+                    pf[u] = lane + u * kWave < row / 4 ? src[lane + u * kWave] : make_uint2(0u, 0u);
+                have_pf = true;
+            }
         }
 
         // ---- the triangle's feature gradient row
@@ -754,13 +913,15 @@ k_shade(ShadeArgs a, int rowp) {
         te = te_next;
     }
 
-    // ---- reduce the bias gradients over the 16 pixels of each lane group
-    for (int m = 0; m < NH; ++m)
-        for (int r = 0; r < 4; ++r) {
-            gb1[m][r] = row16_sum(gb1[m][r]);
-            gb2[m][r] = row16_sum(gb2[m][r]);
-        }
-    for (int r = 0; r < 4; ++r) gb3[r] = row16_sum(gb3[r]);
+    if constexpr (!MF) {
+        // reduce the per-lane bias gradients over the 16 pixels of each lane group
+        for (int m = 0; m < NH; ++m)
+            for (int r = 0; r < 4; ++r) {
+                gb1[m][r] = row16_sum(gb1[m][r]);
+                gb2[m][r] = row16_sum(gb2[m][r]);
+            }
+        for (int r = 0; r < 4; ++r) gb3[r] = row16_sum(gb3[r]);
+    }
     for (int d = 32; d > 0; d >>= 1) loss += __shfl_xor(loss, d, kWave);
 
     // ---- workgroup reduction, one partial per block: every wave stores its
@@ -784,7 +945,7 @@ k_shade(ShadeArgs a, int rowp) {
                 if (f < DIN) rW1[u * DIN + f] = gW1[m][nn][r];
             }
             for (int nn = 0; nn < NH; ++nn) rW2[u * DH + 16 * nn + j] = gW2[m][nn][r];
-            if (j == 0) {
+            if (!MF && j == 0) {
                 rb1[u] = gb1[m][r];
                 rb2[u] = gb2[m][r];
             }
@@ -793,8 +954,16 @@ k_shade(ShadeArgs a, int rowp) {
         if (g == 0)
             for (int r = 0; r < 3; ++r) rW3[r * DH + 16 * m + j] = gW3[m][r];
     }
-    if (g == 0 && j == 0)
+    if constexpr (MF) {
+        // bias block: lane (g, n) holds rows 4 g + r of column n
+        for (int r = 0; r < 4; ++r) {
+            if (j < NH) rb1[16 * j + 4 * g + r] = gbias[r];
+            else if (j < 2 * NH) rb2[16 * (j - NH) + 4 * g + r] = gbias[r];
+            else if (j == 2 * NH && g == 0 && r < 3) rb3[r] = gbias[r];
+        }
+    } else if (g == 0 && j == 0) {
         for (int r = 0; r < 3; ++r) rb3[r] = gb3[r];
+    }
     if (lane == 0) *rl = loss;
     __syncthreads();
     float* out = a.partials + (long long)blockIdx.x * K::NPARAM;
@@ -1004,6 +1173,14 @@ size_t shade_blob_floats(int nl, int c, int dh, bool sh) {
     DS_SHADE_CONFIGS(X)
 #undef X
     return 0;
+}
+
+int shade_blocks_per_cu(int nl, int c, int dh, bool sh) {
+#define X(NL, C, DH, SH) \
+    if (nl == NL && c == C && dh == DH && sh == SH) return Cfg<NL, C, DH, false, SH>::MIN_BLOCKS;
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return 1;
 }
 
 int shade_nparam(int nl, int c, int dh, bool sh) {

@@ -453,9 +453,22 @@ struct StepLayout {
     int aux_per_quad;  // fragment records per 8x8 quad for the opacity gradient
 };
 
-int shade_blocks(long long P) {
+// Compute units of the current device (256 on MI355X when the query fails).
+int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+        return 256;
+    return n;
+}
+
+// Shading workgroups: one round of resident workgroups (per_cu on every
+// compute unit), each wave taking one cost-balanced range; fewer for small
+// batches (64 covered pixels per workgroup at least).
+int shade_blocks(long long P, int per_cu) {
     long long nb = (P + 63) / 64;
-    if (nb > 512) nb = 512;
+    const long long cap = static_cast<long long>(device_cus()) * per_cu;
+    if (nb > cap) nb = cap;
     if (nb < 1) nb = 1;
     return static_cast<int>(nb);
 }
@@ -467,7 +480,7 @@ bool layout(const ds_step_desc* d, const Views& v, StepLayout& L) {
     if (!shade_supported(L.nl, d->channels, d->hidden, L.sh)) return false;
     L.n_tiles = v.tile_base[v.n];
     L.P = v.pix_base[v.n];
-    L.nblocks = shade_blocks(L.P);
+    L.nblocks = shade_blocks(L.P, shade_blocks_per_cu(L.nl, d->channels, d->hidden, L.sh));
     L.nparam = shade_nparam(L.nl, d->channels, d->hidden, L.sh);
     L.Sa = feats_at_level(d->alpha_level);
     L.Sa_src = acc ? feats_in_levels(d->alpha_lo, d->alpha_level) : L.Sa;
