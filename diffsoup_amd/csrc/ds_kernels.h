@@ -160,13 +160,100 @@ hipError_t launch_edge_grad(const Views& v, int C, const float* color, const flo
                             const float* rast, int V, const float* pos, float* grad_pos,
                             int T, const int* tri, hipStream_t s);
 
+// The shading waves' ranges of the triangle-grouped pixel list, and the
+// gradient rows the shading does not store whole.  The ranges split an
+// estimated cost evenly, not the pixel count: a pixel costs 1 and each
+// triangle's segment kSegCost more (its row staging, coarse-level reduction
+// and row store, and on average half a tile of idle lanes), with the triangle
+// index standing in for the number of non-empty segments before it.
+// Cost(t) = tri_off[t] + kSegCost t is monotone, so a 64-way search over
+// triangles (one vector load per level) finds the segment holding each
+// boundary.  bounds[b], b in [0, n_waves], is where shading wave b's range
+// starts (bounds[0] = 0, bounds[n_waves] = the list length).  A triangle
+// whose pixels all fall in one wave's range gets its gradient row stored
+// whole by that wave; a triangle split between ranges is added into with
+// atomics, and a triangle with no covered pixel is not visited: with grad
+// non-null those rows are cleared here (rows of row floats).
+struct ShadeRanges {
+    const int* tri_off;   // [T + 1]
+    int T;
+    int n_waves;
+    int* bounds;          // out [n_waves + 1]
+    float* grad;          // [T, row] rows to clear, or null
+    int row;
+};
+constexpr long long kSegCost = 16;
+
+// Wave-level jobs, wave-uniform: jobs [0, n_waves] are the boundaries (a
+// boundary inside a triangle's segment also clears that row), then one job
+// per 64 triangles clears the rows of those with no covered pixel.
+__device__ __forceinline__ long long shade_range_jobs(const ShadeRanges& r) {
+    return (long long)r.n_waves + 1 + (r.grad ? (r.T + 63) / 64 : 0);
+}
+
+__device__ __forceinline__ void clear_row(float* dst, int row, int lane) {
+    if ((row & 3) == 0 && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0) {
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        for (int i = lane; i < row / 4; i += 64) d4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        for (int i = lane; i < row; i += 64) dst[i] = 0.f;
+    }
+}
+
+__device__ __forceinline__ void shade_range_job(const ShadeRanges& r, long long job, int lane) {
+    const int T = r.T;
+    if (job > r.n_waves) {
+        const int t0 = static_cast<int>(job - r.n_waves - 1) * 64;
+        const int t = t0 + lane;
+        unsigned long long empty = __ballot(t < T && r.tri_off[t] == r.tri_off[t + 1]);
+        while (empty) {
+            const int k = __builtin_ctzll(empty);
+            empty &= empty - 1;
+            clear_row(r.grad + (long long)(t0 + k) * r.row, r.row, lane);
+        }
+        return;
+    }
+    const int b = static_cast<int>(job);
+    const int total = r.tri_off[T];
+    const long long q = (total + kSegCost * T) * b / r.n_waves;
+    // largest t in [0, T] with Cost(t) <= q
+    int lo = 0, hi = T + 1;
+    while (hi - lo > 1) {
+        const int st = (hi - lo + 63) / 64;
+        const int pr = lo + lane * st;
+        const bool ok = pr < hi && r.tri_off[pr] + kSegCost * pr <= q;
+        // lane 0 probes lo, which always qualifies
+        lo += (63 - __builtin_clzll(__ballot(ok))) * st;
+        hi = min(hi, lo + st);
+    }
+    int at;
+    bool split = false;
+    if (b == 0) {
+        at = 0;
+    } else if (b == r.n_waves || lo >= T) {
+        at = total;
+    } else {
+        // the boundary's pixel inside that triangle's segment
+        const int tb = r.tri_off[lo], tn = r.tri_off[lo + 1];
+        const long long over = q - (tb + kSegCost * lo) - kSegCost;
+        at = over <= 0 ? tb : static_cast<int>(min<long long>(tn, tb + over));
+        split = at > tb && at < tn;
+    }
+    if (lane == 0) r.bounds[b] = at;
+    if (split && r.grad) clear_row(r.grad + (long long)lo * r.row, r.row, lane);
+}
+
 // Covered pixels grouped by triangle: list[tri_off[t] + rank] = the pixel's
 // PixRec (target colours from target [P,3]); with px_list, also the covered
 // pixels in tile order at px_list[tile_px_off[tile] + k] = (pixel, view,
-// visible triangle, 0).
+// visible triangle, 0).  Further workgroups of the same launch run the
+// shading ranges' jobs (ShadeRanges).
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
                                       const float* rast, const float* target, const int* tri_off, PixRec* list,
-                                      const int* tile_px_off, int4* px_list, hipStream_t s);
+                                      const int* tile_px_off, int4* px_list, const ShadeRanges& ranges,
+                                      hipStream_t s);
+// The shading ranges' jobs on their own (a backward without its forward's lists).
+hipError_t launch_shade_ranges(const ShadeRanges& ranges, hipStream_t s);
 
 // Over a covered-pixel list of (pixel, view, visible triangle, 0) entries,
 // *n_list of them (read on the device, at most max_n); C must be 3.

@@ -10,6 +10,9 @@
 
 namespace ds {
 
+// Waves per shading workgroup (each takes one range of the pixel list).
+constexpr int kShadeWavesPerBlock = 4;
+
 struct ShadeArgs {
     const float4* rast;     // [P] (b0, b1, depth, id + 1)
     const int* tri_off;     // [T + 1] start of each triangle's covered pixels in list
@@ -39,8 +42,8 @@ struct ShadeArgs {
     // covered pixel's view, row and column, and their inverse MVPs [B,4,4].
     Views views;
     const float* inv_mvp;
-    // where each wave's range of list starts: [nblocks * 4 + 1], written by
-    // launch_shade_bounds for the same nblocks
+    // where each wave's range of list starts: [nblocks * kShadeWavesPerBlock
+    // + 1], the ShadeRanges bounds for the same nblocks
     const int* bounds;
 };
 
@@ -55,11 +58,7 @@ hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const f
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
                         hipStream_t s);
 // The waves' ranges of the triangle-grouped list (before any shading launch).
-hipError_t launch_shade_bounds(const int* tri_off, int T, int nblocks, int* bounds, hipStream_t s);
-// Clears the gradient rows no shading wave stores whole (before an adjoint
-// pass that does not take its rows from global memory).
-hipError_t launch_zero_rows(const int* tri_off, const PixRec* list, const int* bounds, int T, int nblocks,
-                            int row, float* grad, hipStream_t s);
+
 // Whether a configuration gathers and scatters its rows in global memory
 // (rows too large for the LDS plan): its gradient table must then be zeroed.
 bool shade_direct_rows(int nl, int c, int dh, bool sh, bool bf16, bool feat_bf16, int S_tot);

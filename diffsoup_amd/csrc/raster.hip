@@ -521,13 +521,24 @@ __global__ void __launch_bounds__(1024) k_sum_covered(const int* __restrict__ ti
 // list[tri_off[t] + rank] = pixel for every covered pixel of every tile, and
 // (when px_list is given) the same pixels in tile order:
 // px_list[tile_px_off[tile] + k] = pixel.
+// Workgroups from n_tiles on run the shading ranges' jobs instead, one wave
+// per job (grid-stride over the job list).
 template <int NT>
 __global__ void __launch_bounds__(NT) k_fill_lists(Views v, const int2* __restrict__ tile_list,
                                                    const int* __restrict__ tile_ncov,
                                                    const float4* __restrict__ rast,
                                                    const float* __restrict__ target,
                                                    const int* __restrict__ tri_off, PixRec* __restrict__ list,
-                                                   const int* __restrict__ tile_px_off, int4* __restrict__ px_list) {
+                                                   const int* __restrict__ tile_px_off, int4* __restrict__ px_list,
+                                                   int n_tiles, ShadeRanges ranges) {
+    if (static_cast<int>(blockIdx.x) >= n_tiles) {
+        constexpr int WPB = NT / kWave;
+        const long long waves = (long long)(gridDim.x - n_tiles) * WPB;
+        const long long jobs = shade_range_jobs(ranges);
+        for (long long job = (long long)(blockIdx.x - n_tiles) * WPB + threadIdx.x / kWave; job < jobs; job += waves)
+            shade_range_job(ranges, job, threadIdx.x % kWave);
+        return;
+    }
     const int k = threadIdx.x;
     if (k >= tile_ncov[blockIdx.x]) return;
     const int2 e = tile_list[(long long)blockIdx.x * NT + k];
@@ -855,27 +866,46 @@ hipError_t launch_opacity_aux_records(const Views& v, int T, const AlphaSource& 
     return aux_launch(v, T, a, rast, pix_loss, grad_alpha, w, rec, aux_n, per_quad, s);
 }
 
+// Workgroups for the shading ranges' jobs: one wave per job, at most 4096 waves.
+static unsigned range_blocks(const ShadeRanges& r, int waves_per_block) {
+    const long long jobs = (long long)r.n_waves + 1 + (r.grad ? (r.T + 63) / 64 : 0);
+    const long long waves = std::min<long long>(jobs, 4096);
+    return static_cast<unsigned>((waves + waves_per_block - 1) / waves_per_block);
+}
+
 hipError_t launch_fill_triangle_lists(const Views& v, const int2* tile_list, const int* tile_ncov,
                                       const float* rast, const float* target, const int* tri_off, PixRec* list,
-                                      const int* tile_px_off, int4* px_list, hipStream_t s) {
+                                      const int* tile_px_off, int4* px_list, const ShadeRanges& ranges,
+                                      hipStream_t s) {
     const int n_tiles = v.tile_base[v.n];
-    if (n_tiles == 0) return hipSuccess;
     const float4* r4 = reinterpret_cast<const float4*>(rast);
     switch (v.tile) {
         case 8:
-            k_fill_lists<64><<<n_tiles, 64, 0, s>>>(v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off,
-                                                    px_list);
+            k_fill_lists<64><<<n_tiles + range_blocks(ranges, 1), 64, 0, s>>>(
+                v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off, px_list, n_tiles, ranges);
             break;
         case 16:
-            k_fill_lists<256><<<n_tiles, 256, 0, s>>>(v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off,
-                                                      px_list);
+            k_fill_lists<256><<<n_tiles + range_blocks(ranges, 4), 256, 0, s>>>(
+                v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off, px_list, n_tiles, ranges);
             break;
         case 32:
-            k_fill_lists<1024><<<n_tiles, 1024, 0, s>>>(v, tile_list, tile_ncov, r4, target, tri_off, list,
-                                                        tile_px_off, px_list);
+            k_fill_lists<1024><<<n_tiles + range_blocks(ranges, 16), 1024, 0, s>>>(
+                v, tile_list, tile_ncov, r4, target, tri_off, list, tile_px_off, px_list, n_tiles, ranges);
             break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_shade_ranges(ShadeRanges ranges) {
+    const long long waves = (long long)gridDim.x * 4;
+    const long long jobs = shade_range_jobs(ranges);
+    for (long long job = blockIdx.x * 4LL + threadIdx.x / kWave; job < jobs; job += waves)
+        shade_range_job(ranges, job, threadIdx.x % kWave);
+}
+
+hipError_t launch_shade_ranges(const ShadeRanges& ranges, hipStream_t s) {
+    k_shade_ranges<<<range_blocks(ranges, 4), 256, 0, s>>>(ranges);
     return hipGetLastError();
 }
 

@@ -39,7 +39,7 @@ namespace ds {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
+constexpr int kWavesPerBlock = kShadeWavesPerBlock;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -974,80 +974,6 @@ k_shade(ShadeArgs a, int rowp) {
     }
 }
 
-// The shading waves' ranges of the triangle-grouped pixel list.  They split
-// an estimated cost evenly, not the pixel count: a pixel costs 1 and each
-// triangle's segment kSegCost more (its row staging, coarse-level reduction
-// and row store, and on average half a tile of idle lanes), with the
-// triangle index standing in for the number of non-empty segments before
-// it.  Cost(t) = tri_off[t] + kSegCost t is monotone, so a 64-way search
-// over triangles (one vector load per level) finds the segment holding each
-// boundary.  One wave per boundary b in [0, n_waves]: bounds[b] is where
-// wave b's range starts (bounds[0] = 0, bounds[n_waves] = the list length).
-constexpr long long kSegCost = 16;
-
-__global__ void __launch_bounds__(256) k_shade_bounds(const int* __restrict__ tri_off, int T, int n_waves,
-                                                      int* __restrict__ bounds) {
-    const int b = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-    const int lane = threadIdx.x % kWave;
-    if (b > n_waves) return;
-    const int total = tri_off[T];
-    const long long q = (total + kSegCost * T) * b / n_waves;
-    // largest t in [0, T] with Cost(t) <= q
-    int lo = 0, hi = T + 1;
-    while (hi - lo > 1) {
-        const int st = (hi - lo + kWave - 1) / kWave;
-        const int pr = lo + lane * st;
-        const bool ok = pr < hi && tri_off[pr] + kSegCost * pr <= q;
-        // lane 0 probes lo, which always qualifies
-        lo += (63 - __builtin_clzll(__ballot(ok))) * st;
-        hi = min(hi, lo + st);
-    }
-    int r;
-    if (b == 0) {
-        r = 0;
-    } else if (b == n_waves || lo >= T) {
-        r = total;
-    } else {
-        // the boundary's pixel inside that triangle's segment
-        const int tb = tri_off[lo], tn = tri_off[lo + 1];
-        const long long over = q - (tb + kSegCost * lo) - kSegCost;
-        r = over <= 0 ? tb : static_cast<int>(min<long long>(tn, tb + over));
-    }
-    if (lane == 0) bounds[b] = r;
-}
-
-// Clears the gradient rows the shading does not store whole: triangles with
-// no covered pixel (their gradient is zero) and triangles split between two
-// shading waves (both add into the row with atomics).  Every other row is
-// stored whole by the wave that owns it, so the table needs no clearing
-// beforehand.  One wave per row job, grid-stride: jobs [0, T) test triangle
-// t for emptiness, jobs [T, T + n_waves) test the triangle at wave boundary
-// j - T for a split.
-__global__ void __launch_bounds__(256) k_zero_rows(const int* __restrict__ tri_off, const PixRec* __restrict__ list,
-                                                   const int* __restrict__ bounds, int T, int n_waves, int row,
-                                                   float* __restrict__ grad) {
-    const int lane = threadIdx.x % kWave;
-    const long long n_jobs = (long long)T + n_waves;
-    const long long stride = (long long)gridDim.x * (blockDim.x / kWave);
-    for (long long job = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; job < n_jobs; job += stride) {
-        int t = -1;
-        if (job < T) {
-            if (tri_off[job] == tri_off[job + 1]) t = static_cast<int>(job);
-        } else {
-            const int r = bounds[job - T];
-            if (r > 0 && r < tri_off[T] && list[r - 1].tri == list[r].tri) t = list[r].tri;
-        }
-        if (t < 0) continue;
-        float* dst = grad + (long long)t * row;
-        if ((row & 3) == 0) {
-            float4* d4 = reinterpret_cast<float4*>(dst);
-            for (int i = lane; i < row / 4; i += kWave) d4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-            for (int i = lane; i < row; i += kWave) dst[i] = 0.f;
-        }
-    }
-}
-
 // Per-lane weight blobs for one configuration, from the fp32 master weights
 // W1 [DH, DIN], W2 [DH, DH], W3 [3, DH].  Entry (layer, m, t, lane, e) is the
 // A operand for output row 16 m + lane % 16 and input row 16 t + 4 (lane / 16) + e.
@@ -1197,22 +1123,6 @@ hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const f
     DS_SHADE_CONFIGS(X)
 #undef X
     return hipErrorInvalidValue;
-}
-
-hipError_t launch_shade_bounds(const int* tri_off, int T, int nblocks, int* bounds, hipStream_t s) {
-    const int n_waves = nblocks * kWavesPerBlock;
-    const int threads = (n_waves + 1) * kWave;
-    k_shade_bounds<<<(threads + 255) / 256, 256, 0, s>>>(tri_off, T, n_waves, bounds);
-    return hipGetLastError();
-}
-
-hipError_t launch_zero_rows(const int* tri_off, const PixRec* list, const int* bounds, int T, int nblocks,
-                            int row, float* grad, hipStream_t s) {
-    const int n_waves = nblocks * kWavesPerBlock;
-    const long long jobs = (long long)T + n_waves;
-    const long long blocks = std::min<long long>((jobs + 3) / 4, 4096);
-    if (blocks > 0) k_zero_rows<<<static_cast<unsigned>(blocks), 256, 0, s>>>(tri_off, list, bounds, T, n_waves, row, grad);
-    return hipGetLastError();
 }
 
 namespace {

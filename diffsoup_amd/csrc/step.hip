@@ -545,7 +545,7 @@ struct Step {
 
 int step_prepare(const ds_step_desc* d, void* stream, Step& st);
 size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L);
-int enqueue_forward(const ds_step_desc* d, Step& st);
+int enqueue_forward(const ds_step_desc* d, Step& st, bool clear_rows);
 int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ext_grad);
 int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, bool write_loss);
 
@@ -665,9 +665,19 @@ AccumJob alpha_job(const ds_step_desc* d, const Step& st) {
                     d->alpha_level, d->alpha_level, 1, static_cast<int>(st.L.Sa_src), static_cast<int>(st.L.Sa)};
 }
 
+// The shading ranges' job description: wave boundaries, and (clear_rows) the
+// gradient rows the shading does not store whole.
+ShadeRanges shade_ranges(const ds_step_desc* d, const Step& st, bool clear_rows) {
+    const StepLayout& L = st.L;
+    const bool acc = d->accumulate != 0;
+    return ShadeRanges{st.tri_off, d->T, L.nblocks * kShadeWavesPerBlock, st.bounds,
+                       clear_rows && !L.direct ? (acc ? st.gfeat_acc : d->grad_feat) : nullptr,
+                       static_cast<int>(L.S_shade) * d->channels};
+}
+
 // Prologue, binning, resolve (with background shading) and the covered-pixel
-// lists: everything before the shading kernel.
-int enqueue_forward(const ds_step_desc* d, Step& st) {
+// lists with the shading ranges: everything before the shading kernel.
+int enqueue_forward(const ds_step_desc* d, Step& st, bool clear_rows) {
     const StepLayout& L = st.L;
     const hipStream_t s = st.s;
     const bool acc = d->accumulate != 0;
@@ -714,9 +724,8 @@ int enqueue_forward(const ds_step_desc* d, Step& st) {
                                       st.tile_px_off, st.scan_tmp2, s),
            "triangle / tile scans");
     DS_TRY(launch_fill_triangle_lists(st.v, st.tile_list, st.tile_ncov, d->rast, d->target, st.tri_off, st.list,
-                                      st.tile_px_off, st.px_list, s),
-           "triangle lists");
-    DS_TRY(launch_shade_bounds(st.tri_off, d->T, L.nblocks, st.bounds, s), "shading ranges");
+                                      st.tile_px_off, st.px_list, shade_ranges(d, st, clear_rows), s),
+           "triangle lists and shading ranges");
     return DS_OK;
 }
 
@@ -731,10 +740,6 @@ int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ex
                  st.blobs, d->b1, d->b2, d->b3, d->color, d->grad_color, d->pix_loss, 2.0f * st.inv_n,
                  st.partials, ext_grad, fwd_only, st.v, d->inv_mvp, st.bounds};
     hipError_t e;
-    if (!fwd_only && !L.direct)
-        DS_TRY(launch_zero_rows(st.tri_off, st.list, st.bounds, d->T, L.nblocks,
-                                static_cast<int>(L.S_shade) * d->channels, acc ? st.gfeat_acc : d->grad_feat, st.s),
-               "zero rows");
     DS_TRY(launch_shade(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, sa, L.nblocks, st.s), "shade");
     return DS_OK;
 }
@@ -810,7 +815,7 @@ int ds_train_step(const ds_step_desc* d, void* stream) {
     Step st;
     int rc = step_prepare(d, stream, st);
     if (rc != DS_OK) return rc;
-    if ((rc = enqueue_forward(d, st)) != DS_OK) return rc;
+    if ((rc = enqueue_forward(d, st, true)) != DS_OK) return rc;
     hipError_t e;
     if ((e = mark(d, st.s, DS_STAGE_SHADE, true)) != hipSuccess) return step_fail(DS_ERR_HIP, "event");
     if (d->loss_kind == DS_LOSS_L1_SSIM) {
@@ -833,7 +838,7 @@ int ds_step_forward(const ds_step_desc* d, void* stream) {
     Step st;
     int rc = step_prepare(d, stream, st);
     if (rc != DS_OK) return rc;
-    if ((rc = enqueue_forward(d, st)) != DS_OK) return rc;
+    if ((rc = enqueue_forward(d, st, false)) != DS_OK) return rc;
     return enqueue_shade(d, st, 1, nullptr);
 }
 
@@ -844,6 +849,9 @@ int ds_step_backward(const ds_step_desc* d, const float* grad_color, void* strea
     Step st;
     int rc = step_prepare(d, stream, st);
     if (rc != DS_OK) return rc;
+    // the forward's ranges again, now clearing the rows the adjoint adds into
+    if (launch_shade_ranges(shade_ranges(d, st, true), st.s) != hipSuccess)
+        return step_fail(DS_ERR_HIP, "shading ranges");
     if ((rc = enqueue_shade(d, st, 0, grad_color)) != DS_OK) return rc;
     return enqueue_backward(d, st, grad_color, false);
 }
