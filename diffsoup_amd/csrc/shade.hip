@@ -50,12 +50,6 @@ __device__ __forceinline__ i16x4 to_bf16x4(f32x4 v) {
     return __builtin_bit_cast(i16x4, __builtin_convertvector(v, bf16x4));
 }
 
-__device__ __forceinline__ unsigned short bf16_rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<unsigned short>(u >> 16);
-}
-
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + expf(-z)); }
 
 // Sum over the 16 lanes of this lane's DPP row (lanes 16 r .. 16 r + 15),
@@ -125,6 +119,8 @@ struct Cfg {
                                                                               : 2 * region(DH);
     static constexpr int STAGE = STAGE_ > region(16) + region(DH) ? STAGE_ : region(16) + region(DH);
     static_assert(C % 4 == 0 || NL == 1, "channels per level must be a multiple of 4 with several levels");
+    static_assert(NTILES == blob_tiles(NI, NH) && TILE_ELEMS == blob_tile_floats(BF16),
+                  "the step prologue's blob layout (BlobJob) must be this one");
     static_assert(DH % 16 == 0, "hidden width must be a multiple of 16");
 };
 
@@ -974,48 +970,6 @@ k_shade(ShadeArgs a, int rowp) {
     }
 }
 
-// Per-lane weight blobs for one configuration, from the fp32 master weights
-// W1 [DH, DIN], W2 [DH, DH], W3 [3, DH].  Entry (layer, m, t, lane, e) is the
-// A operand for output row 16 m + lane % 16 and input row 16 t + 4 (lane / 16) + e.
-template <class K>
-__global__ void k_blobs(const float* __restrict__ w1, const float* __restrict__ w2,
-                        const float* __restrict__ w3, float* __restrict__ blobs) {
-    constexpr int NI = K::NI, NH = K::NH, DIN = K::DIN, DH = K::DH;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= K::NTILES * kWave) return;
-    const int tile = id / kWave, lane = id % kWave;
-    int base, nin, layer;
-    if (tile < K::L2) { layer = 0; base = K::L1; nin = NI; }
-    else if (tile < K::L3) { layer = 1; base = K::L2; nin = NH; }
-    else if (tile < K::L3T) { layer = 2; base = K::L3; nin = NH; }
-    else if (tile < K::L2T) { layer = 3; base = K::L3T; nin = 1; }
-    else if (tile < K::L1T) { layer = 4; base = K::L2T; nin = NH; }
-    else { layer = 5; base = K::L1T; nin = NH; }
-    const int m = (tile - base) / nin, t = (tile - base) % nin;
-    const int o = 16 * m + lane % 16;
-    float v[4];
-    for (int e = 0; e < 4; ++e) {
-        const int i = 16 * t + 4 * (lane / 16) + e;
-        float w = 0.f;
-        switch (layer) {
-            case 0: w = (i < DIN) ? w1[o * DIN + i] : 0.f; break;
-            case 1: w = w2[o * DH + i]; break;
-            case 2: w = (o < 3) ? w3[o * DH + i] : 0.f; break;
-            case 3: w = (i < 3) ? w3[i * DH + o] : 0.f; break;
-            case 4: w = w2[i * DH + o]; break;
-            case 5: w = (o < DIN) ? w1[i * DIN + o] : 0.f; break;
-        }
-        v[e] = w;
-    }
-    float* dst = blobs + tile * K::TILE_ELEMS;
-    if constexpr (K::BF16) {
-        unsigned short* d16 = reinterpret_cast<unsigned short*>(dst);
-        for (int e = 0; e < 4; ++e) d16[lane * 4 + e] = bf16_rne(v[e]);
-    } else {
-        for (int e = 0; e < 4; ++e) dst[e * kWave + lane] = v[e];
-    }
-}
-
 template <class K, bool FB, bool SPLIT, bool DIRECT>
 hipError_t shade_launch(const ShadeArgs& a, int nblocks, int rowp, size_t bytes, hipStream_t s) {
     static size_t opted_in = 65536;   // per instance: more LDS needs the kernel's opt-in
@@ -1050,13 +1004,6 @@ hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
         if (std::max(lds0, red) <= 160 * 1024) return shade_forms<K, FB, true>(a, nblocks, 0, std::max(lds0, red), s);
     }
     return hipErrorInvalidValue;
-}
-
-template <class K>
-hipError_t blobs_dispatch(const float* w1, const float* w2, const float* w3, float* blobs, hipStream_t s) {
-    const int n = K::NTILES * kWave;
-    k_blobs<K><<<(n + 255) / 256, 256, 0, s>>>(w1, w2, w3, blobs);
-    return hipGetLastError();
 }
 
 // One configuration: fp32 MLP, bf16 MLP on fp32 rows, or bf16 MLP on bf16
@@ -1112,17 +1059,6 @@ int shade_blocks_per_cu(int nl, int c, int dh, bool sh) {
 int shade_nparam(int nl, int c, int dh, bool sh) {
     const int din = nl * c + (sh ? 9 : 0);
     return dh * din + dh * dh + 3 * dh + dh + dh + 3 + 1;
-}
-
-hipError_t launch_shade_blobs(int nl, int c, int dh, bool sh, bool bf16, const float* w1, const float* w2,
-                              const float* w3, float* blobs, hipStream_t s) {
-#define X(NL, C, DH, SH)                                                                    \
-    if (nl == NL && c == C && dh == DH && sh == SH)                                         \
-        return bf16 ? blobs_dispatch<Cfg<NL, C, DH, true, SH>>(w1, w2, w3, blobs, s)        \
-                    : blobs_dispatch<Cfg<NL, C, DH, false, SH>>(w1, w2, w3, blobs, s);
-    DS_SHADE_CONFIGS(X)
-#undef X
-    return hipErrorInvalidValue;
 }
 
 namespace {

@@ -45,14 +45,15 @@ __device__ __forceinline__ void zero_floats(float* p, long long n, long long i0,
 // instead), zeroed gradient accumulators and per-triangle counts, clip =
 // M_b [x, y, z, 1] for every (view, vertex) (row-major M,
 // examples/utils.py:412-426, summed in the order ((m0 x + m1 y) + m2 z) + m3),
-// and the raster bins' cleared tile counters and stats block.
+// the raster bins' cleared tile counters and stats block, and the shading
+// kernel's weight operands.
 __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict__ alpha, long long n_alpha,
                             float* __restrict__ galpha, float* __restrict__ gpos, long long n_pos,
                             int* __restrict__ tri_count, int T, float* __restrict__ gfeat, long long n_feat,
                             float* __restrict__ z1, long long n_z1, float* __restrict__ z2, long long n_z2,
                             int B, int V, const float* __restrict__ verts, const float* __restrict__ mvp,
                             float4* __restrict__ pos, int* __restrict__ tile_count, int n_tiles,
-                            long long* __restrict__ stats) {
+                            long long* __restrict__ stats, BlobJob blob) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     for (long long i = i0; i < n_alpha; i += stride) {
@@ -78,6 +79,7 @@ __global__ void k_step_prep(const float* __restrict__ logits, float* __restrict_
     }
     for (long long i = i0; i < n_tiles; i += stride) tile_count[i] = 0;
     if (i0 < kStatCount) stats[i0] = 0;
+    for (long long i = i0; i < blob.n(); i += stride) build_blob(blob, static_cast<int>(i));
 }
 
 // One accumulation of the reference's composition: the level-summed lattice
@@ -685,15 +687,18 @@ int enqueue_forward(const ds_step_desc* d, Step& st, bool clear_rows) {
     const long long n_feat = (long long)d->T * L.S_shade * d->channels;   // the table the shading writes
     const long long n_pos = 4LL * d->B * d->V;
     hipError_t e;
+    const int din = L.nl * d->channels + (L.sh ? 9 : 0);
+    const BlobJob blob{d->w1, d->w2, d->w3, st.blobs, (din + 15) / 16, d->hidden / 16, din, d->hidden, d->bf16 != 0};
     const long long prep_n = std::max(std::max(std::max(n_alpha, (long long)d->T), (long long)L.n_tiles),
                                       std::max(std::max(n_pos / 4, L.direct ? n_feat / 4 : 0LL),
-                                               std::max((long long)d->B * d->V, (long long)kStatCount)));
+                                               std::max(std::max((long long)d->B * d->V, (long long)kStatCount),
+                                                        blob.n())));
     DS_TRY(mark(d, s, DS_STAGE_PREP, true), "event");
     k_step_prep<<<std::min<unsigned>(blocks_for(prep_n, 256), 4096), 256, 0, s>>>(
         acc ? nullptr : d->alpha_logits, d->alpha, n_alpha, st.galpha, st.gpos, n_pos, st.tri_count, d->T,
         acc ? st.gfeat_acc : d->grad_feat, L.direct ? n_feat : 0, nullptr, 0, nullptr, 0, d->B, d->V, d->verts,
         d->mvp,
-        reinterpret_cast<float4*>(d->pos), st.rw.tile_count, L.n_tiles, st.rw.stats);
+        reinterpret_cast<float4*>(d->pos), st.rw.tile_count, L.n_tiles, st.rw.stats, blob);
     DS_TRY(hipGetLastError(), "step prep");
     if (acc && d->T > 0) {
         DS_TRY(accum_lds_opt_in(), "accumulate LDS");
@@ -702,9 +707,7 @@ int enqueue_forward(const ds_step_desc* d, Step& st, bool clear_rows) {
         DS_TRY(hipGetLastError(), "accumulate + sigmoid");
     }
     DS_TRY(mark(d, s, DS_STAGE_PREP, false), "event");
-    DS_TRY(mark(d, s, DS_STAGE_BLOBS, true), "event");
-    DS_TRY(launch_shade_blobs(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, d->w1, d->w2, d->w3, st.blobs, s),
-           "blobs");
+    DS_TRY(mark(d, s, DS_STAGE_BLOBS, true), "event");   // the blobs are built in the prologue
     DS_TRY(mark(d, s, DS_STAGE_BLOBS, false), "event");
     DS_TRY(mark(d, s, DS_STAGE_PROJECT, true), "event");   // projection runs in the prologue
     DS_TRY(mark(d, s, DS_STAGE_PROJECT, false), "event");
