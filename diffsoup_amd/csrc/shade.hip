@@ -161,22 +161,25 @@ struct Plan {
                    ? up16(K::STAGE * static_cast<int>(sizeof(typename K::ST)))
                    : MfPlan<K>::BYTES;
     }
-    // feature row, gradient row, transpose stage, and (CLAIM) the claim table
-    // of the gradient row's 4-channel blocks (one int each: rowp bytes)
+    // NS slots (two where tiles run on into the next triangle: the kernels
+    // without MF) of feature row and gradient row, the transpose stage, and
+    // (CLAIM) NS claim tables of the gradient rows' 4-channel blocks (one int
+    // each: rowp bytes)
+    static constexpr int NS = K::MF ? 1 : 2;
     __host__ __device__ static constexpr int base_bytes(int rowp) {
-        return frow_bytes(rowp) + 4 * rowp + stage_bytes();
+        return NS * (frow_bytes(rowp) + 4 * rowp) + stage_bytes();
     }
     static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::CP) + 3) & ~3;
     static constexpr int LDS_MAX = 160 * 1024, BLOB = 4 * K::BLOB_ELEMS;
     // preference, for rows starting at level 0: operands and claim table in
     // LDS; claim table only; operands only; neither
-    static constexpr bool FIT_BOTH = BLOB + kWavesPerBlock * (base_bytes(ROW0) + ROW0) <= LDS_MAX;
-    static constexpr bool FIT_CLAIM = kWavesPerBlock * (base_bytes(ROW0) + ROW0) <= LDS_MAX;
+    static constexpr bool FIT_BOTH = BLOB + kWavesPerBlock * (base_bytes(ROW0) + NS * ROW0) <= LDS_MAX;
+    static constexpr bool FIT_CLAIM = kWavesPerBlock * (base_bytes(ROW0) + NS * ROW0) <= LDS_MAX;
     static constexpr bool FIT_BLOB = BLOB + kWavesPerBlock * base_bytes(ROW0) <= LDS_MAX;
     static constexpr bool CLAIM = FIT_BOTH || FIT_CLAIM;
     static constexpr bool LB = FIT_BOTH || (!FIT_CLAIM && FIT_BLOB);
     static constexpr int BLOB_BYTES = LB ? BLOB : 0;
-    __host__ __device__ static constexpr int wave_bytes(int rowp) { return base_bytes(rowp) + (CLAIM ? rowp : 0); }
+    __host__ __device__ static constexpr int wave_bytes(int rowp) { return base_bytes(rowp) + (CLAIM ? NS * rowp : 0); }
 };
 
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -405,10 +408,15 @@ k_shade(ShadeArgs a, int rowp) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
     unsigned char* wbase = reinterpret_cast<unsigned char*>(smem) + P::BLOB_BYTES + wave * P::wave_bytes(rowp);
-    FT* frow = reinterpret_cast<FT*>(wbase);
-    float* grow = reinterpret_cast<float*>(wbase + P::frow_bytes(rowp));
-    ST* stage = reinterpret_cast<ST*>(wbase + P::frow_bytes(rowp) + 4 * rowp);
-    int* claim = reinterpret_cast<int*>(wbase + P::base_bytes(rowp));
+    // per wave: NS slots of feature row and gradient row, the transpose
+    // stage, NS claim tables
+    FT* frow0 = reinterpret_cast<FT*>(wbase);
+    FT* frow1 = reinterpret_cast<FT*>(wbase + (P::NS - 1) * P::frow_bytes(rowp));
+    float* grow0 = reinterpret_cast<float*>(wbase + P::NS * P::frow_bytes(rowp));
+    float* grow1 = grow0 + (P::NS - 1) * rowp;
+    ST* stage = reinterpret_cast<ST*>(wbase + P::NS * (P::frow_bytes(rowp) + 4 * rowp));
+    int* claim0 = reinterpret_cast<int*>(wbase + P::base_bytes(rowp));
+    int* claim1 = claim0 + (P::NS - 1) * (rowp / 4);
     const int row = a.S_tot * C;
 
     // stage the weight blobs
@@ -440,44 +448,37 @@ k_shade(ShadeArgs a, int rowp) {
     }
 
     // Each wave owns a contiguous range [bounds[wid], bounds[wid + 1]) of the
-    // triangle-grouped pixel list (k_shade_bounds) and walks the triangle
-    // segments inside it.  A triangle whose pixels all fall in one wave's
+    // triangle-grouped pixel list (the ShadeRanges jobs) and walks it in
+    // tiles of 16 pixels.  A triangle whose pixels all fall in one wave's
     // range gets its gradient row stored; a triangle split between waves is
-    // added with atomics into its row, which k_zero_rows cleared.  A
-    // segment's triangle and end come from its first record (wave-uniform
-    // loads), so the walk never reads the offset table.
+    // added with atomics into its row, which the list kernel cleared.  A
+    // triangle's end and the next triangle come from records (wave-uniform
+    // loads), so the walk never reads the offset table.  XT (two waves per
+    // SIMD, rows staged in LDS): a tile may run on into the next triangle,
+    // whose row is then staged in the wave's second slot, so that the MLP
+    // lanes stay full (on eight 800x800 views 73.5% of the lanes held a
+    // pixel when tiles stopped at each triangle's end); the wave's current
+    // triangle is A, in slot sA, the next is N.
+    constexpr bool XT = !K::MF && !DIRECT;
     const int wid = blockIdx.x * kWavesPerBlock + wave;
     const int r0 = a.bounds[wid], r1 = a.bounds[wid + 1];
-    int t = 0, te = 0;
-    bool first_whole = true;   // the first segment starts at r0
-    if (r0 < r1) {
-        t = a.list[r0].tri;
-        te = a.list[r0].seg_end;
-        first_whole = r0 == 0 || a.list[r0 - 1].tri != t;
-    }
-    // PF: prefetch registers for the next segment's bf16 feature row (rows up
-    // to the plan's level-0 row; longer rows are copied at segment start)
+    const float4* recs4 = reinterpret_cast<const float4*>(a.list);
+    // PF: prefetch registers for the next triangle's bf16 feature row (rows up
+    // to the plan's level-0 row; longer rows are copied when staged)
     constexpr bool PF = FB && K::MF;
     constexpr int PF_N = PF ? (P::ROW0 / 4 + kWave - 1) / kWave : 1;
     uint2 pf[PF_N];
     bool have_pf = false;
-    for (int pos = r0; pos < r1;) {
-        const int beg = pos, end = min(te, r1);
-        const int npx = end - beg;
-        const bool owned = (beg > r0 || first_whole) && te <= r1;
-        // the next segment's triangle and end
-        int t_next = 0, te_next = 0;
-        if (te < r1) {
-            t_next = a.list[te].tri;
-            te_next = a.list[te].seg_end;
-        }
-        pos = end;
-        float4* g4 = reinterpret_cast<float4*>(grow);
+
+    // Stage triangle t's feature row into slot sl and clear that slot's
+    // gradient row.  The row's loads go out in batches of 8 per lane before
+    // the first LDS store waits on them: a row is read once per step, so each
+    // load is an HBM miss, and a load-store-load chain would pay that latency
+    // for every 64 x 8 or 64 x 16 bytes of the row.
+    auto begin_row = [&](int sl, int t) {
+        FT* frow = sl ? frow1 : frow0;
+        float4* g4 = reinterpret_cast<float4*>(sl ? grow1 : grow0);
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        // The row's loads go out in batches of 8 per lane before the first
-        // LDS store waits on them: a row is read once per step, so each load
-        // is an HBM miss, and a load-store-load chain would pay that latency
-        // for every 64 x 8 or 64 x 16 bytes of the row.
         if constexpr (DIRECT) {
             // nothing staged
         } else if constexpr (C != CP) {
@@ -489,7 +490,7 @@ k_shade(ShadeArgs a, int rowp) {
             }
         } else if constexpr (FB) {
             // bf16 features stay bf16 in the row: 4 values per 8-byte copy
-            // (or, PF, the row the previous segment's end already loaded)
+            // (or, PF, the row the previous triangle's end already loaded)
             if (PF && have_pf) {
 #pragma unroll
                 for (int u = 0; u < PF_N; ++u)
@@ -513,400 +514,452 @@ k_shade(ShadeArgs a, int rowp) {
                          row / 4, lane);
         }
         wave_lds_sync();
+    };
 
-        // MF: the MFMA-scattered levels' gradient blocks (samples x feature
-        // columns of the level's input tile), accumulated over the segment's
-        // tiles; otherwise the two coarsest levels' per-lane sums (racc)
-        f32x4 sacc[NMF];
-        for (int q = 0; q < NMF; ++q) sacc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-        float racc[MF ? 1 : 6][4];
-        for (int vtx = 0; vtx < (MF ? 1 : 6); ++vtx)
-            for (int r = 0; r < 4; ++r) racc[vtx][r] = 0.f;
+    // MF: the MFMA-scattered levels' gradient blocks (samples x feature
+    // columns of the level's input tile), accumulated over a triangle's
+    // tiles; otherwise the two coarsest levels' per-lane sums (racc) of the
+    // current triangle's pixels
+    f32x4 sacc[NMF];
+    for (int q = 0; q < NMF; ++q) sacc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float racc[MF ? 1 : 6][4];
+    for (int vtx = 0; vtx < (MF ? 1 : 6); ++vtx)
+        for (int r = 0; r < 4; ++r) racc[vtx][r] = 0.f;
 
-        // lanes of group 0 also take the pixel's target colour from the record
-        const float4* recs4 = reinterpret_cast<const float4*>(a.list);
-        float4 rec = j < npx ? recs4[2 * (beg + j)] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 rtg = g == 0 && j < npx ? recs4[2 * (beg + j) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int c0 = 0; c0 < npx; c0 += 16) {
-            const int idx = c0 + j;
-            const bool valid = idx < npx;
+    int tA = 0, teA = 0, sA = 0, tN = 0, teN = 0;
+    bool ownA = false, haveN = false;   // haveN: XT, N staged in slot 1 - sA
+    if (r0 < r1) {
+        tA = a.list[r0].tri;
+        teA = a.list[r0].seg_end;
+        ownA = (r0 == 0 || a.list[r0 - 1].tri != tA) && teA <= r1;
+        if (teA < r1) {
+            tN = a.list[teA].tri;
+            teN = a.list[teA].seg_end;
+        }
+        begin_row(sA, tA);
+    }
+    // lanes of group 0 also take the pixel's target colour from the record
+    float4 rec = r0 + j < r1 ? recs4[2 * (r0 + j)] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 rtg = g == 0 && r0 + j < r1 ? recs4[2 * (r0 + j) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int pos = r0; pos < r1;) {
+        // this tile: up to 16 pixels, within A, or (XT) running on into N
+        int end = min(pos + 16, r1);
+        if (end > teA) {
+            if constexpr (XT) {
+                if (!haveN) {
+                    begin_row(1 - sA, tN);
+                    haveN = true;
+                }
+                end = min(end, teN);
+            } else {
+                end = teA;
+            }
+        }
+        {
+            const int idx = pos + j;
+            const bool valid = idx < end;
+            const bool inA = idx < teA;
+            const int sl = inA ? sA : 1 - sA;
+            const int tl = inA ? tA : tN;
+            const FT* frow_l = sl ? frow1 : frow0;
+            float* grow_l = sl ? grow1 : grow0;
+            int* claim_l = sl ? claim1 : claim0;
+            (void)tl;
+            (void)frow_l;
+            (void)grow_l;
+            (void)claim_l;
             const int p = __float_as_int(rec.z);
             const float b0 = rec.x, b1 = rec.y;
             const float tg0 = rtg.x, tg1 = rtg.y, tg2 = rtg.z;
             // the next tile's records are in flight while this one shades
-            const bool nv = idx + 16 < npx;
-            const float4 next = nv ? recs4[2 * (beg + idx + 16)] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 ntg = g == 0 && nv ? recs4[2 * (beg + idx + 16) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-
-            // ---- the view direction's SH2 values of pixel j (the inputs after the features)
-            float shv[9];
-            if constexpr (K::SH) {
-                const Views& vw = a.views;
-                int b = 0, base = 0, Wb = vw.W[0], Hb = vw.H[0];
-                for (int k = 1; k < vw.n; ++k)
-                    if (p >= vw.pix_base[k]) { b = k; base = vw.pix_base[k]; Wb = vw.W[k]; Hb = vw.H[k]; }
-                const int local = p - base, py = local / Wb, px = local - py * Wb;
-                sh2_basis(a.inv_mvp + 16 * b, pixel_ndc(px, Wb), pixel_ndc(py, Hb), shv);
-            }
-
-            // ---- gather from the LDS row: lane (g, j) holds inputs 16 tt + 4 g + r of pixel j
-            f32x4 x[NI];
-            Corners crs[NI];
-            for (int tt = 0; tt < NI; ++tt) {
-                x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
-                crs[tt] = Corners{{0u, 0u, 0u}, {0.f, 0.f, 0.f}};
-                const int f0 = 16 * tt + 4 * g;
-                if (valid && f0 < DF) {
-                    const int lv = f0 / C, c = f0 % C;
-                    const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
-                    crs[tt] = cr;
-                    float4 v0, v1, v2;
-                    if (FB && f0 == 0) {
-                        // channels 0-3 of the first level feed the residual
-                        // blend, i.e. the output colour directly: with bf16
-                        // rows they are read from the fp32 master (three
-                        // 16-byte loads per pixel, cache-resident within a
-                        // triangle), so bf16 storage does not reach the colour
-                        const float* fg = a.feat + (long long)t * row;
-                        v0 = *reinterpret_cast<const float4*>(fg + cr.idx[0] * C);
-                        v1 = *reinterpret_cast<const float4*>(fg + cr.idx[1] * C);
-                        v2 = *reinterpret_cast<const float4*>(fg + cr.idx[2] * C);
-                    } else if constexpr (DIRECT) {
-                        const float* fg = a.feat + (long long)t * row + loff[lv] * C + c;
-                        v0 = ld4_global<C>(fg + cr.idx[0] * C, c);
-                        v1 = ld4_global<C>(fg + cr.idx[1] * C, c);
-                        v2 = ld4_global<C>(fg + cr.idx[2] * C, c);
-                    } else {
-                        const FT* fl = frow + loff[lv] * CP + c;
-                        v0 = ld4(fl + cr.idx[0] * CP);
-                        v1 = ld4(fl + cr.idx[1] * CP);
-                        v2 = ld4(fl + cr.idx[2] * CP);
-                    }
-                    float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
-                    e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
-                    e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;
-                    e2 += cr.w[0] * v0.z; e2 += cr.w[1] * v1.z; e2 += cr.w[2] * v2.z;
-                    e3 += cr.w[0] * v0.w; e3 += cr.w[1] * v1.w; e3 += cr.w[2] * v2.w;
-                    x[tt] = f32x4{e0, e1, e2, e3};
-                }
+            const bool nv = end + j < r1;
+            const float4 next = nv ? recs4[2 * (end + j)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 ntg = g == 0 && nv ? recs4[2 * (end + j) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            do {
+                // ---- the view direction's SH2 values of pixel j (the inputs after the features)
+                float shv[9];
                 if constexpr (K::SH) {
-                    for (int r = 0; r < 4; ++r) {
-                        const int k = f0 + r - DF;
-                        if (valid && k >= 0 && k < 9) x[tt][r] = pick9(shv, k);
+                    const Views& vw = a.views;
+                    int b = 0, base = 0, Wb = vw.W[0], Hb = vw.H[0];
+                    for (int k = 1; k < vw.n; ++k)
+                        if (p >= vw.pix_base[k]) { b = k; base = vw.pix_base[k]; Wb = vw.W[k]; Hb = vw.H[k]; }
+                    const int local = p - base, py = local / Wb, px = local - py * Wb;
+                    sh2_basis(a.inv_mvp + 16 * b, pixel_ndc(px, Wb), pixel_ndc(py, Hb), shv);
+                }
+
+                // ---- gather from the LDS row: lane (g, j) holds inputs 16 tt + 4 g + r of pixel j
+                f32x4 x[NI];
+                Corners crs[NI];
+                for (int tt = 0; tt < NI; ++tt) {
+                    x[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    crs[tt] = Corners{{0u, 0u, 0u}, {0.f, 0.f, 0.f}};
+                    const int f0 = 16 * tt + 4 * g;
+                    if (valid && f0 < DF) {
+                        const int lv = f0 / C, c = f0 % C;
+                        const Corners cr = lattice_corners(b0, b1, static_cast<uint32_t>(a.lo + lv));
+                        crs[tt] = cr;
+                        float4 v0, v1, v2;
+                        if (FB && f0 == 0) {
+                            // channels 0-3 of the first level feed the residual
+                            // blend, i.e. the output colour directly: with bf16
+                            // rows they are read from the fp32 master (three
+                            // 16-byte loads per pixel, cache-resident within a
+                            // triangle), so bf16 storage does not reach the colour
+                            const float* fg = a.feat + (long long)tl * row;
+                            v0 = *reinterpret_cast<const float4*>(fg + cr.idx[0] * C);
+                            v1 = *reinterpret_cast<const float4*>(fg + cr.idx[1] * C);
+                            v2 = *reinterpret_cast<const float4*>(fg + cr.idx[2] * C);
+                        } else if constexpr (DIRECT) {
+                            const float* fg = a.feat + (long long)tA * row + loff[lv] * C + c;
+                            v0 = ld4_global<C>(fg + cr.idx[0] * C, c);
+                            v1 = ld4_global<C>(fg + cr.idx[1] * C, c);
+                            v2 = ld4_global<C>(fg + cr.idx[2] * C, c);
+                        } else {
+                            const FT* fl = frow_l + loff[lv] * CP + c;
+                            v0 = ld4(fl + cr.idx[0] * CP);
+                            v1 = ld4(fl + cr.idx[1] * CP);
+                            v2 = ld4(fl + cr.idx[2] * CP);
+                        }
+                        float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+                        e0 += cr.w[0] * v0.x; e0 += cr.w[1] * v1.x; e0 += cr.w[2] * v2.x;
+                        e1 += cr.w[0] * v0.y; e1 += cr.w[1] * v1.y; e1 += cr.w[2] * v2.y;
+                        e2 += cr.w[0] * v0.z; e2 += cr.w[1] * v1.z; e2 += cr.w[2] * v2.z;
+                        e3 += cr.w[0] * v0.w; e3 += cr.w[1] * v1.w; e3 += cr.w[2] * v2.w;
+                        x[tt] = f32x4{e0, e1, e2, e3};
+                    }
+                    if constexpr (K::SH) {
+                        for (int r = 0; r < 4; ++r) {
+                            const int k = f0 + r - DF;
+                            if (valid && k >= 0 && k < 9) x[tt][r] = pick9(shv, k);
+                        }
                     }
                 }
-            }
 
-            // ---- forward
-            f32x4 h1[NH], h2[NH], y[1];
-            for (int m = 0; m < NH; ++m)
-                for (int r = 0; r < 4; ++r) { h1[m][r] = a.b1[16 * m + 4 * g + r]; h2[m][r] = a.b2[16 * m + 4 * g + r]; }
-            for (int r = 0; r < 4; ++r) y[0][r] = (g == 0 && r < 3) ? a.b3[r] : 0.f;
-            dense<K, NH, NI>(blobs, K::L1, lane, x, h1);
-            for (int m = 0; m < NH; ++m)
-                for (int r = 0; r < 4; ++r) h1[m][r] = fmaxf(h1[m][r], 0.f);
-            dense<K, NH, NH>(blobs, K::L2, lane, h1, h2);
-            for (int m = 0; m < NH; ++m)
-                for (int r = 0; r < 4; ++r) h2[m][r] = fmaxf(h2[m][r], 0.f);
-            dense<K, 1, NH>(blobs, K::L3, lane, h2, y);
+                // ---- forward
+                f32x4 h1[NH], h2[NH], y[1];
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) { h1[m][r] = a.b1[16 * m + 4 * g + r]; h2[m][r] = a.b2[16 * m + 4 * g + r]; }
+                for (int r = 0; r < 4; ++r) y[0][r] = (g == 0 && r < 3) ? a.b3[r] : 0.f;
+                dense<K, NH, NI>(blobs, K::L1, lane, x, h1);
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) h1[m][r] = fmaxf(h1[m][r], 0.f);
+                dense<K, NH, NH>(blobs, K::L2, lane, h1, h2);
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) h2[m][r] = fmaxf(h2[m][r], 0.f);
+                dense<K, 1, NH>(blobs, K::L3, lane, h2, y);
 
-            // ---- blend, loss and output gradient (lanes of group 0 own pixel j)
-            f32x4 d3[1] = {f32x4{0.f, 0.f, 0.f, 0.f}};
-            float dxr[4] = {0.f, 0.f, 0.f, 0.f};
-            if (g == 0 && valid) {
-                const float res = x[0][3];
-                const float tg[3] = {tg0, tg1, tg2};
-                float* col = a.color + (long long)p * 3;
-                float* gc = a.grad_color + (long long)p * 3;
-                float l1 = 0.f, dres = 0.f;
-                for (int c = 0; c < 3; ++c) {
-                    const float sg = sigmoidf(y[0][c]);
-                    const float rgb = x[0][c];
-                    const float out = (1.0f - res) * rgb + res * sg;
-                    float go;
-                    if (SPLIT && a.ext_grad) {
-                        go = a.ext_grad[(long long)p * 3 + c];
+                // ---- blend, loss and output gradient (lanes of group 0 own pixel j)
+                f32x4 d3[1] = {f32x4{0.f, 0.f, 0.f, 0.f}};
+                float dxr[4] = {0.f, 0.f, 0.f, 0.f};
+                if (g == 0 && valid) {
+                    const float res = x[0][3];
+                    const float tg[3] = {tg0, tg1, tg2};
+                    float* col = a.color + (long long)p * 3;
+                    float* gc = a.grad_color + (long long)p * 3;
+                    float l1 = 0.f, dres = 0.f;
+                    for (int c = 0; c < 3; ++c) {
+                        const float sg = sigmoidf(y[0][c]);
+                        const float rgb = x[0][c];
+                        const float out = (1.0f - res) * rgb + res * sg;
+                        float go;
+                        if (SPLIT && a.ext_grad) {
+                            go = a.ext_grad[(long long)p * 3 + c];
+                        } else {
+                            const float d = out - tg[c];
+                            loss += d * d;
+                            l1 += fabsf(d);
+                            go = a.grad_scale * d;
+                            col[c] = out;
+                            gc[c] = go;
+                        }
+                        dxr[c] = go * (1.0f - res);
+                        dres += go * (sg - rgb);
+                        d3[0][c] = go * res * sg * (1.0f - sg);
+                    }
+                    dxr[3] = dres;
+                    if (!(SPLIT && a.ext_grad)) a.pix_loss[p] = l1;
+                }
+                if (SPLIT && a.fwd_only) break;   // wave-uniform: colour only, no adjoint
+
+                // ---- backward through the MLP
+                f32x4 dh2[NH], dh1[NH], dx[NI];
+                for (int m = 0; m < NH; ++m) dh2[m] = dh1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int m = 0; m < NI; ++m) dx[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+                dense<K, NH, 1>(blobs, K::L3T, lane, d3, dh2);
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) dh2[m][r] = h2[m][r] > 0.f ? dh2[m][r] : 0.f;
+                dense<K, NH, NH>(blobs, K::L2T, lane, dh2, dh1);
+                for (int m = 0; m < NH; ++m)
+                    for (int r = 0; r < 4; ++r) dh1[m][r] = h1[m][r] > 0.f ? dh1[m][r] : 0.f;
+                dense<K, NI, NH>(blobs, K::L1T, lane, dh1, dx);
+                if (g == 0)
+                    for (int r = 0; r < 4; ++r) dx[0][r] += dxr[r];
+
+                // ---- scatter the input gradient onto the triangle's gradient row.
+                // Lane (g, j) holds dX of pixel j for 4 channels of one level and
+                // that level's lattice corners.  The lattices of levels 0-2 (3,
+                // 6 and 15 samples) are shared by most pixels of a tile: there
+                // the scatter is a matrix product, below.  Finer levels add into
+                // the LDS row.  LDS float atomics are slow there (the LDS stays
+                // busy ~2 cycles per lane), so each lane first claims its
+                // 4-channel block (it writes its lane id into the block's claim
+                // word and reads it back); the one lane that reads back its own
+                // id owns the block and adds with a plain 16-byte
+                // read-modify-write; the losers claim again (CLAIM_ROUNDS) and
+                // what is left adds with atomics.
+    #pragma unroll
+                for (int tt = 0; tt < NI; ++tt) {
+                    const int f0 = 16 * tt + 4 * g;
+                    const bool in = valid && f0 < DF;
+                    const int lv = in ? f0 / C : 0, c = f0 % C;
+                    const Corners& cr = crs[tt];
+                    if constexpr (K::SH || C != CP) {
+                        // only feature inputs scatter (not the SH2 values; no padding channel)
+                        for (int r = 0; r < 4; ++r)
+                            if (f0 + r >= DF || c + r >= C) dx[tt][r] = 0.f;
+                    }
+                    if constexpr (DIRECT) {
+                        if (in) {
+                            float* gg = a.grad_feat + (long long)tA * row + loff[lv] * C + c;
+                            for (int k = 0; k < 3; ++k)
+                                for (int r = 0; r < 4; ++r)
+                                    if (c + r < C) unsafeAtomicAdd(gg + cr.idx[k] * C + r, cr.w[k] * dx[tt][r]);
+                        }
+                        continue;
+                    }
+                    bool fine;
+                    if constexpr (MF) {
+                        fine = in && a.lo + lv > kMfmaTopLevel;
                     } else {
-                        const float d = out - tg[c];
-                        loss += d * d;
-                        l1 += fabsf(d);
-                        go = a.grad_scale * d;
-                        col[c] = out;
-                        gc[c] = go;
+                        // a pixel of the next triangle (a tile crossing into it)
+                        // scatters its coarse levels like the fine ones
+                        const bool coarse = tt == 0 && a.lo + lv <= 1 && inA;
+                        if (in && coarse) {
+                            for (int vtx = 0; vtx < 6; ++vtx) {
+                                const float wv = (cr.idx[0] == (uint32_t)vtx ? cr.w[0] : 0.f) +
+                                                 (cr.idx[1] == (uint32_t)vtx ? cr.w[1] : 0.f) +
+                                                 (cr.idx[2] == (uint32_t)vtx ? cr.w[2] : 0.f);
+                                for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
+                            }
+                        }
+                        fine = in && !coarse;
                     }
-                    dxr[c] = go * (1.0f - res);
-                    dres += go * (sg - rgb);
-                    d3[0][c] = go * res * sg * (1.0f - sg);
-                }
-                dxr[3] = dres;
-                if (!(SPLIT && a.ext_grad)) a.pix_loss[p] = l1;
-            }
-            if (SPLIT && a.fwd_only) {   // wave-uniform: colour only, no adjoint
-                rec = next;
-                rtg = ntg;
-                continue;
-            }
-
-            // ---- backward through the MLP
-            f32x4 dh2[NH], dh1[NH], dx[NI];
-            for (int m = 0; m < NH; ++m) dh2[m] = dh1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-            for (int m = 0; m < NI; ++m) dx[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-            dense<K, NH, 1>(blobs, K::L3T, lane, d3, dh2);
-            for (int m = 0; m < NH; ++m)
-                for (int r = 0; r < 4; ++r) dh2[m][r] = h2[m][r] > 0.f ? dh2[m][r] : 0.f;
-            dense<K, NH, NH>(blobs, K::L2T, lane, dh2, dh1);
-            for (int m = 0; m < NH; ++m)
-                for (int r = 0; r < 4; ++r) dh1[m][r] = h1[m][r] > 0.f ? dh1[m][r] : 0.f;
-            dense<K, NI, NH>(blobs, K::L1T, lane, dh1, dx);
-            if (g == 0)
-                for (int r = 0; r < 4; ++r) dx[0][r] += dxr[r];
-
-            // ---- scatter the input gradient onto the triangle's gradient row.
-            // Lane (g, j) holds dX of pixel j for 4 channels of one level and
-            // that level's lattice corners.  The lattices of levels 0-2 (3,
-            // 6 and 15 samples) are shared by most pixels of a tile: there
-            // the scatter is a matrix product, below.  Finer levels add into
-            // the LDS row.  LDS float atomics are slow there (the LDS stays
-            // busy ~2 cycles per lane), so each lane first claims its
-            // 4-channel block (it writes its lane id into the block's claim
-            // word and reads it back); the one lane that reads back its own
-            // id owns the block and adds with a plain 16-byte
-            // read-modify-write; the losers claim again (CLAIM_ROUNDS) and
-            // what is left adds with atomics.
-#pragma unroll
-            for (int tt = 0; tt < NI; ++tt) {
-                const int f0 = 16 * tt + 4 * g;
-                const bool in = valid && f0 < DF;
-                const int lv = in ? f0 / C : 0, c = f0 % C;
-                const Corners& cr = crs[tt];
-                if constexpr (K::SH || C != CP) {
-                    // only feature inputs scatter (not the SH2 values; no padding channel)
-                    for (int r = 0; r < 4; ++r)
-                        if (f0 + r >= DF || c + r >= C) dx[tt][r] = 0.f;
-                }
-                if constexpr (DIRECT) {
-                    if (in) {
-                        float* gg = a.grad_feat + (long long)t * row + loff[lv] * C + c;
+                    if constexpr (!P::CLAIM) {
+                        // no room for the claim table: LDS atomics
+                        if (fine)
+                            for (int k = 0; k < 3; ++k) {
+                                float* gl = grow_l + grad_slot<CP>(loff[lv] + cr.idx[k], c);
+                                for (int r = 0; r < 4; ++r) atomicAdd(gl + r, cr.w[k] * dx[tt][r]);
+                            }
+                        continue;
+                    }
+                    if (__ballot(fine) == 0ull) continue;   // wave-uniform
+                    int slot[3];
+                    for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<CP>(loff[lv] + cr.idx[k], c) : 0;
+                    // each round: every pending lane claims its blocks, the owners
+                    // add and are done, the losers try again in the next round
+                    bool pend[3] = {fine, fine, fine};
+    #pragma unroll
+                    for (int round = 0; round < K::CLAIM_ROUNDS; ++round) {
+                        if (round > 0 && __ballot(pend[0] || pend[1] || pend[2]) == 0ull) break;   // wave-uniform
                         for (int k = 0; k < 3; ++k)
-                            for (int r = 0; r < 4; ++r)
-                                if (c + r < C) unsafeAtomicAdd(gg + cr.idx[k] * C + r, cr.w[k] * dx[tt][r]);
+                            if (pend[k])
+                                __hip_atomic_store(claim_l + (slot[k] >> 2), lane, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        wave_lds_sync();
+                        bool own[3];
+                        for (int k = 0; k < 3; ++k)
+                            own[k] = pend[k] && __hip_atomic_load(claim_l + (slot[k] >> 2), __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WAVEFRONT) == lane;
+                        for (int k = 0; k < 3; ++k)
+                            if (own[k]) {
+                                float4* q = reinterpret_cast<float4*>(grow_l + slot[k]);
+                                float4 v = *q;
+                                v.x += cr.w[k] * dx[tt][0];
+                                v.y += cr.w[k] * dx[tt][1];
+                                v.z += cr.w[k] * dx[tt][2];
+                                v.w += cr.w[k] * dx[tt][3];
+                                *q = v;
+                                pend[k] = false;
+                            }
+                        wave_lds_sync();
                     }
-                    continue;
-                }
-                bool fine;
-                if constexpr (MF) {
-                    fine = in && a.lo + lv > kMfmaTopLevel;
-                } else {
-                    const bool coarse = tt == 0 && a.lo + lv <= 1;
-                    if (in && coarse) {
-                        for (int vtx = 0; vtx < 6; ++vtx) {
-                            const float wv = (cr.idx[0] == (uint32_t)vtx ? cr.w[0] : 0.f) +
-                                             (cr.idx[1] == (uint32_t)vtx ? cr.w[1] : 0.f) +
-                                             (cr.idx[2] == (uint32_t)vtx ? cr.w[2] : 0.f);
-                            for (int r = 0; r < 4; ++r) racc[vtx][r] += wv * dx[0][r];
-                        }
-                    }
-                    fine = in && !coarse;
-                }
-                if constexpr (!P::CLAIM) {
-                    // no room for the claim table: LDS atomics
-                    if (fine)
-                        for (int k = 0; k < 3; ++k) {
-                            float* gl = grow + grad_slot<CP>(loff[lv] + cr.idx[k], c);
-                            for (int r = 0; r < 4; ++r) atomicAdd(gl + r, cr.w[k] * dx[tt][r]);
-                        }
-                    continue;
-                }
-                if (__ballot(fine) == 0ull) continue;   // wave-uniform
-                int slot[3];
-                for (int k = 0; k < 3; ++k) slot[k] = fine ? grad_slot<CP>(loff[lv] + cr.idx[k], c) : 0;
-                // each round: every pending lane claims its blocks, the owners
-                // add and are done, the losers try again in the next round
-                bool pend[3] = {fine, fine, fine};
-#pragma unroll
-                for (int round = 0; round < K::CLAIM_ROUNDS; ++round) {
-                    if (round > 0 && __ballot(pend[0] || pend[1] || pend[2]) == 0ull) break;   // wave-uniform
                     for (int k = 0; k < 3; ++k)
                         if (pend[k])
-                            __hip_atomic_store(claim + (slot[k] >> 2), lane, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    wave_lds_sync();
-                    bool own[3];
-                    for (int k = 0; k < 3; ++k)
-                        own[k] = pend[k] && __hip_atomic_load(claim + (slot[k] >> 2), __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WAVEFRONT) == lane;
-                    for (int k = 0; k < 3; ++k)
-                        if (own[k]) {
-                            float4* q = reinterpret_cast<float4*>(grow + slot[k]);
-                            float4 v = *q;
-                            v.x += cr.w[k] * dx[tt][0];
-                            v.y += cr.w[k] * dx[tt][1];
-                            v.z += cr.w[k] * dx[tt][2];
-                            v.w += cr.w[k] * dx[tt][3];
-                            *q = v;
-                            pend[k] = false;
+                            for (int r = 0; r < 4; ++r) atomicAdd(grow_l + slot[k] + r, cr.w[k] * dx[tt][r]);
+                }
+
+                // ---- levels 0-2: G[s][f] += sum_p W[s][p] dX[p][f] on the fp32
+                // MFMA, samples s on M, the level's input tile's 16 feature
+                // columns f on N, the tile's pixels p on K (four steps of four).
+                // W[s][p] is pixel p's barycentric weight at sample s (the sum
+                // over its corners that are s).  dX passes through the stage
+                // region transposed ([pixel][f], rows of 20 floats) and the
+                // corners through a per-pixel table, so that lane (g, i) of step
+                // st can form W[i][4 st + g] and read dX[4 st + g][i].  Columns
+                // of other levels sharing the tile accumulate unused values.
+                const int nmf = kMfmaTopLevel + 1 - a.lo;   // levels scattered by MFMA (wave-uniform)
+                if constexpr (!DIRECT && MF) {
+                    float* trn = reinterpret_cast<float*>(stage);
+                    int4* cwi = reinterpret_cast<int4*>(trn + 16 * 20);
+                    float4* cwf = reinterpret_cast<float4*>(cwi + 16);
+    #pragma unroll
+                    for (int q = 0; q < NMF; ++q) {
+                        if (q >= nmf) break;
+                        const int tq = q * C / 16, c0 = q * C - 16 * tq;
+                        if (q == 0 || tq != (q - 1) * C / 16)
+                            *reinterpret_cast<f32x4*>(trn + j * 20 + 4 * g) = dx[tq];
+                        if (4 * g == (c0 & ~3)) {
+                            const Corners& cr = crs[tq];
+                            cwi[j] = make_int4(static_cast<int>(cr.idx[0]), static_cast<int>(cr.idx[1]),
+                                               static_cast<int>(cr.idx[2]), 0);
+                            cwf[j] = make_float4(cr.w[0], cr.w[1], cr.w[2], 0.f);
                         }
-                    wave_lds_sync();
+                        wave_lds_sync();
+    #pragma unroll
+                        for (int st = 0; st < 4; ++st) {
+                            const int p = 4 * st + g;
+                            const int4 ci = cwi[p];
+                            const float4 cf = cwf[p];
+                            const float w = (ci.x == j ? cf.x : 0.f) + (ci.y == j ? cf.y : 0.f) + (ci.z == j ? cf.z : 0.f);
+                            sacc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, trn[p * 20 + j], sacc[q], 0, 0, 0);
+                        }
+                        wave_lds_sync();
+                    }
                 }
-                for (int k = 0; k < 3; ++k)
-                    if (pend[k])
-                        for (int r = 0; r < 4; ++r) atomicAdd(grow + slot[k] + r, cr.w[k] * dx[tt][r]);
-            }
+                if constexpr (!MF) {
+                    // bias gradients per lane until the end
+                    for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
+                    gb3 += d3[0];
+                }
 
-            // ---- levels 0-2: G[s][f] += sum_p W[s][p] dX[p][f] on the fp32
-            // MFMA, samples s on M, the level's input tile's 16 feature
-            // columns f on N, the tile's pixels p on K (four steps of four).
-            // W[s][p] is pixel p's barycentric weight at sample s (the sum
-            // over its corners that are s).  dX passes through the stage
-            // region transposed ([pixel][f], rows of 20 floats) and the
-            // corners through a per-pixel table, so that lane (g, i) of step
-            // st can form W[i][4 st + g] and read dX[4 st + g][i].  Columns
-            // of other levels sharing the tile accumulate unused values.
-            const int nmf = kMfmaTopLevel + 1 - a.lo;   // levels scattered by MFMA (wave-uniform)
-            if constexpr (!DIRECT && MF) {
-                float* trn = reinterpret_cast<float*>(stage);
-                int4* cwi = reinterpret_cast<int4*>(trn + 16 * 20);
-                float4* cwf = reinterpret_cast<float4*>(cwi + 16);
-#pragma unroll
-                for (int q = 0; q < NMF; ++q) {
-                    if (q >= nmf) break;
-                    const int tq = q * C / 16, c0 = q * C - 16 * tq;
-                    if (q == 0 || tq != (q - 1) * C / 16)
-                        *reinterpret_cast<f32x4*>(trn + j * 20 + 4 * g) = dx[tq];
-                    if (4 * g == (c0 & ~3)) {
-                        const Corners& cr = crs[tq];
-                        cwi[j] = make_int4(static_cast<int>(cr.idx[0]), static_cast<int>(cr.idx[1]),
-                                           static_cast<int>(cr.idx[2]), 0);
-                        cwf[j] = make_float4(cr.w[0], cr.w[1], cr.w[2], 0.f);
+                // ---- weight gradients: transpose (delta, activation) pairs through
+                // LDS one at a time and contract over the 16 pixels
+                {
+                    constexpr int DINP = K::DINP;
+                    ST* sD = stage;
+                    ST* sA = stage + K::region(DH);
+                    for (int m = 0; m < NH; ++m) stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh1[m]);
+                    for (int tt = 0; tt < NI; ++tt) stage_put<K>(sA, DINP, 16 * tt + 4 * g, j, x[tt]);
+                    wave_lds_sync();
+                    for (int m = 0; m < NH; ++m) {
+                        for (int nn = 0; nn < NI; ++nn)
+                            gW1[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DINP, 16 * nn, lane, gW1[m][nn]);
+                        if constexpr (MF) gbias = stage_sum<K>(sD, DH, 16 * m, m, lane, gbias);
                     }
                     wave_lds_sync();
-#pragma unroll
-                    for (int st = 0; st < 4; ++st) {
-                        const int p = 4 * st + g;
-                        const int4 ci = cwi[p];
-                        const float4 cf = cwf[p];
-                        const float w = (ci.x == j ? cf.x : 0.f) + (ci.y == j ? cf.y : 0.f) + (ci.z == j ? cf.z : 0.f);
-                        sacc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, trn[p * 20 + j], sacc[q], 0, 0, 0);
+                    for (int m = 0; m < NH; ++m) {
+                        stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh2[m]);
+                        stage_put<K>(sA, DH, 16 * m + 4 * g, j, h1[m]);
                     }
                     wave_lds_sync();
+                    for (int m = 0; m < NH; ++m) {
+                        for (int nn = 0; nn < NH; ++nn)
+                            gW2[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DH, 16 * nn, lane, gW2[m][nn]);
+                        if constexpr (MF) gbias = stage_sum<K>(sD, DH, 16 * m, NH + m, lane, gbias);
+                    }
+                    wave_lds_sync();
+                    ST* sD3 = stage;
+                    ST* sH2 = stage + K::region(16);
+                    stage_put<K>(sD3, 16, 4 * g, j, d3[0]);
+                    for (int m = 0; m < NH; ++m) stage_put<K>(sH2, DH, 16 * m + 4 * g, j, h2[m]);
+                    wave_lds_sync();
+                    for (int m = 0; m < NH; ++m)
+                        gW3[m] = stage_outer<K>(sD3, 16, 0, sH2, DH, 16 * m, lane, gW3[m]);
+                    if constexpr (MF) gbias = stage_sum<K>(sD3, 16, 0, 2 * NH, lane, gbias);
+                    wave_lds_sync();
                 }
-            }
-            if constexpr (!MF) {
-                // bias gradients per lane until the end
-                for (int m = 0; m < NH; ++m) { gb1[m] += dh1[m]; gb2[m] += dh2[m]; }
-                gb3 += d3[0];
-            }
-
-            // ---- weight gradients: transpose (delta, activation) pairs through
-            // LDS one at a time and contract over the 16 pixels
-            {
-                constexpr int DINP = K::DINP;
-                ST* sD = stage;
-                ST* sA = stage + K::region(DH);
-                for (int m = 0; m < NH; ++m) stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh1[m]);
-                for (int tt = 0; tt < NI; ++tt) stage_put<K>(sA, DINP, 16 * tt + 4 * g, j, x[tt]);
-                wave_lds_sync();
-                for (int m = 0; m < NH; ++m) {
-                    for (int nn = 0; nn < NI; ++nn)
-                        gW1[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DINP, 16 * nn, lane, gW1[m][nn]);
-                    if constexpr (MF) gbias = stage_sum<K>(sD, DH, 16 * m, m, lane, gbias);
-                }
-                wave_lds_sync();
-                for (int m = 0; m < NH; ++m) {
-                    stage_put<K>(sD, DH, 16 * m + 4 * g, j, dh2[m]);
-                    stage_put<K>(sA, DH, 16 * m + 4 * g, j, h1[m]);
-                }
-                wave_lds_sync();
-                for (int m = 0; m < NH; ++m) {
-                    for (int nn = 0; nn < NH; ++nn)
-                        gW2[m][nn] = stage_outer<K>(sD, DH, 16 * m, sA, DH, 16 * nn, lane, gW2[m][nn]);
-                    if constexpr (MF) gbias = stage_sum<K>(sD, DH, 16 * m, NH + m, lane, gbias);
-                }
-                wave_lds_sync();
-                ST* sD3 = stage;
-                ST* sH2 = stage + K::region(16);
-                stage_put<K>(sD3, 16, 4 * g, j, d3[0]);
-                for (int m = 0; m < NH; ++m) stage_put<K>(sH2, DH, 16 * m + 4 * g, j, h2[m]);
-                wave_lds_sync();
-                for (int m = 0; m < NH; ++m)
-                    gW3[m] = stage_outer<K>(sD3, 16, 0, sH2, DH, 16 * m, lane, gW3[m]);
-                if constexpr (MF) gbias = stage_sum<K>(sD3, 16, 0, 2 * NH, lane, gbias);
-                wave_lds_sync();
-            }
+            } while (false);
             rec = next;
             rtg = ntg;
         }
+        pos = end;
 
-        if (SPLIT && a.fwd_only) {
-            wave_lds_sync();
-            t = t_next;
-            te = te_next;
-            continue;
-        }
-
-        if constexpr (DIRECT) {   // the gradient went straight to global memory
-            t = t_next;
-            te = te_next;
-            continue;
-        }
-
-        // ---- the coarse levels' blocks into the LDS row
-        if constexpr (MF) {
-            add_mf_blocks<K, NMF>(sacc, kMfmaTopLevel + 1 - a.lo, a.lo, loff, grow, g, j);
-            for (int q = 0; q < NMF; ++q) sacc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-            wave_lds_sync();
-        } else {
-            // the two coarsest levels' sums: reduce over the 16 pixel lanes of
-            // each lane group, then one lane per group adds them to the LDS row
-            const int f0 = 4 * g;
-            const int lv = f0 / C, c = f0 % C;
-            const bool coarse = f0 < DF && a.lo + lv <= 1;
-            const int S = coarse ? static_cast<int>(feats_at_level(a.lo + lv)) : 0;
-            for (int vtx = 0; vtx < 6; ++vtx)
-                for (int r = 0; r < 4; ++r)
-                    racc[vtx][r] = row16_sum(racc[vtx][r]);
-            wave_lds_sync();
-            if (coarse && j == 0) {
-                for (int vtx = 0; vtx < 6; ++vtx)
-                    if (vtx < S) {
-                        float* gl = grow + grad_slot<CP>(loff[lv] + vtx, c);
-                        for (int r = 0; r < 4; ++r) gl[r] += racc[vtx][r];
+        // ---- the triangles this tile completed: A, and (a tile that ran
+        // through all of N) N after it; at the range's end A is done
+        // whether or not its pixels continue in the next wave's range
+        while (pos >= min(teA, r1)) {
+            if (!(SPLIT && a.fwd_only) && !DIRECT) {
+                float* grow = sA ? grow1 : grow0;
+                // ---- the coarse levels' blocks into the LDS row
+                if constexpr (MF) {
+                    add_mf_blocks<K, NMF>(sacc, kMfmaTopLevel + 1 - a.lo, a.lo, loff, grow, g, j);
+                    for (int q = 0; q < NMF; ++q) sacc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    wave_lds_sync();
+                } else {
+                    // the two coarsest levels' sums: reduce over the 16 pixel
+                    // lanes of each lane group, then one lane per group adds
+                    // them to the LDS row
+                    const int f0 = 4 * g;
+                    const int lv = f0 / C, c = f0 % C;
+                    const bool coarse = f0 < DF && a.lo + lv <= 1;
+                    const int S = coarse ? static_cast<int>(feats_at_level(a.lo + lv)) : 0;
+                    for (int vtx = 0; vtx < 6; ++vtx)
+                        for (int r = 0; r < 4; ++r) {
+                            racc[vtx][r] = row16_sum(racc[vtx][r]);
+                        }
+                    wave_lds_sync();
+                    if (coarse && j == 0) {
+                        for (int vtx = 0; vtx < 6; ++vtx)
+                            if (vtx < S) {
+                                float* gl = grow + grad_slot<CP>(loff[lv] + vtx, c);
+                                for (int r = 0; r < 4; ++r) gl[r] += racc[vtx][r];
+                            }
                     }
-            }
-            wave_lds_sync();
-        }
+                    for (int vtx = 0; vtx < 6; ++vtx)
+                        for (int r = 0; r < 4; ++r) racc[vtx][r] = 0.f;
+                    wave_lds_sync();
+                }
 
-        // ---- PF: the next segment's bf16 row goes out before this segment's
-        // gradient row: loads issued ahead of the stores complete without
-        // waiting for them (the memory counter is in order), so the row's
-        // latency overlaps the stores and the row clearing
-        if constexpr (PF) {
-            if (te < r1 && row / 4 <= PF_N * kWave) {
-                const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)t_next * row);
+                // ---- PF: the next triangle's bf16 row goes out before this
+                // triangle's gradient row: loads issued ahead of the stores
+                // complete without waiting for them (the memory counter is in
+                // order), so the row's latency overlaps the stores and the row
+                // clearing
+                if constexpr (PF) {
+                    if (teA < r1 && row / 4 <= PF_N * kWave) {
+                        const uint2* src = reinterpret_cast<const uint2*>(a.feat_bf16 + (long long)tN * row);
 #pragma unroll
-                for (int u = 0; u < PF_N; ++u)
-                    pf[u] = lane + u * kWave < row / 4 ? src[lane + u * kWave] : make_uint2(0u, 0u);
-                have_pf = true;
-            }
-        }
+                        for (int u = 0; u < PF_N; ++u)
+                            pf[u] = lane + u * kWave < row / 4 ? src[lane + u * kWave] : make_uint2(0u, 0u);
+                        have_pf = true;
+                    }
+                }
 
-        // ---- the triangle's feature gradient row
-        if (owned && C == CP) {
-            float4* dst = reinterpret_cast<float4*>(a.grad_feat + (long long)t * row);
-            for (int i = lane; i < row / 4; i += kWave)
-                dst[i] = *reinterpret_cast<const float4*>(grow + grad_slot<CP>(4 * i / C, 4 * i % C));
-        } else {
-            float* dst = a.grad_feat + (long long)t * row;
-            for (int i = lane; i < row; i += kWave) {
-                const int sm = i / C, c = i - sm * C;
-                const float gv = grow[grad_slot<CP>(sm, c & ~3) + (c & 3)];
-                if (owned) dst[i] = gv;
-                else if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
+                // ---- the triangle's feature gradient row
+                if (ownA && C == CP) {
+                    float4* dst = reinterpret_cast<float4*>(a.grad_feat + (long long)tA * row);
+                    for (int i = lane; i < row / 4; i += kWave)
+                        dst[i] = *reinterpret_cast<const float4*>(grow + grad_slot<CP>(4 * i / C, 4 * i % C));
+                } else {
+                    float* dst = a.grad_feat + (long long)tA * row;
+                    for (int i = lane; i < row; i += kWave) {
+                        const int sm = i / C, c = i - sm * C;
+                        const float gv = grow[grad_slot<CP>(sm, c & ~3) + (c & 3)];
+                        if (ownA) dst[i] = gv;
+                        else if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
+                    }
+                }
+                wave_lds_sync();
+            }
+            // at the range's end, a staged N (its first pixels were in the
+            // last tile) still has its row to add
+            if (pos >= r1 && !(XT && haveN)) break;
+            // ---- N becomes A
+            tA = tN;
+            teA = teN;
+            ownA = teA <= r1;
+            if (XT && haveN) {
+                sA = 1 - sA;
+                haveN = false;
+            } else {
+                begin_row(sA, tA);
+            }
+            if (teA < r1) {
+                tN = a.list[teA].tri;
+                teN = a.list[teA].seg_end;
             }
         }
-        wave_lds_sync();
-        t = t_next;
-        te = te_next;
     }
 
     if constexpr (!MF) {
