@@ -163,10 +163,10 @@ hipError_t launch_edge_grad(const Views& v, int C, const float* color, const flo
 // The shading waves' ranges of the triangle-grouped pixel list, and the
 // gradient rows the shading does not store whole.  The ranges split an
 // estimated cost evenly, not the pixel count: a pixel costs 1 and each
-// triangle's segment kSegCost more (its row staging, coarse-level reduction
-// and row store, and on average half a tile of idle lanes), with the triangle
-// index standing in for the number of non-empty segments before it.
-// Cost(t) = tri_off[t] + kSegCost t is monotone, so a 64-way search over
+// triangle's segment seg_cost more (its row staging, coarse-level reduction
+// and row store, and where tiles stop at a triangle's end, on average half a
+// tile of idle lanes), with the triangle index standing in for the number of
+// non-empty segments before it.  Cost(t) = tri_off[t] + seg_cost t is monotone, so a 64-way search over
 // triangles (one vector load per level) finds the segment holding each
 // boundary.  bounds[b], b in [0, n_waves], is where shading wave b's range
 // starts (bounds[0] = 0, bounds[n_waves] = the list length).  A triangle
@@ -181,8 +181,8 @@ struct ShadeRanges {
     int* bounds;          // out [n_waves + 1]
     float* grad;          // [T, row] rows to clear, or null
     int row;
+    long long seg_cost;   // the shading configuration's (shade_seg_cost)
 };
-constexpr long long kSegCost = 16;
 
 // Wave-level jobs, wave-uniform: jobs [0, n_waves] are the boundaries (a
 // boundary inside a triangle's segment also clears that row), then one job
@@ -215,13 +215,13 @@ __device__ __forceinline__ void shade_range_job(const ShadeRanges& r, long long 
     }
     const int b = static_cast<int>(job);
     const int total = r.tri_off[T];
-    const long long q = (total + kSegCost * T) * b / r.n_waves;
+    const long long q = (total + r.seg_cost * T) * b / r.n_waves;
     // largest t in [0, T] with Cost(t) <= q
     int lo = 0, hi = T + 1;
     while (hi - lo > 1) {
         const int st = (hi - lo + 63) / 64;
         const int pr = lo + lane * st;
-        const bool ok = pr < hi && r.tri_off[pr] + kSegCost * pr <= q;
+        const bool ok = pr < hi && r.tri_off[pr] + r.seg_cost * pr <= q;
         // lane 0 probes lo, which always qualifies
         lo += (63 - __builtin_clzll(__ballot(ok))) * st;
         hi = min(hi, lo + st);
@@ -235,7 +235,7 @@ __device__ __forceinline__ void shade_range_job(const ShadeRanges& r, long long 
     } else {
         // the boundary's pixel inside that triangle's segment
         const int tb = r.tri_off[lo], tn = r.tri_off[lo + 1];
-        const long long over = q - (tb + kSegCost * lo) - kSegCost;
+        const long long over = q - (tb + r.seg_cost * lo) - r.seg_cost;
         at = over <= 0 ? tb : static_cast<int>(min<long long>(tn, tb + over));
         split = at > tb && at < tn;
     }

@@ -113,6 +113,8 @@ int shade_nparam(int nl, int c, int dh, bool sh);
 // Resident shading workgroups per compute unit of a compiled configuration
 // (the kernel's register budget: two up to hidden width 32, else one).
 int shade_blocks_per_cu(int nl, int c, int dh, bool sh);
+// Cost of a triangle's segment in pixels, for the shading waves' ranges.
+long long shade_seg_cost(int nl, int c, int dh, bool sh);
 
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
                         hipStream_t s);

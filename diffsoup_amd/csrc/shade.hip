@@ -106,6 +106,13 @@ struct Cfg {
     // 159 us against 177-192 us with the MFMA forms): fp32 MFMA issue is
     // what the smaller networks are short of.
     static constexpr bool MF = MIN_BLOCKS == 1;
+    // a triangle's segment cost, in pixels, in the waves' range split (row
+    // staging, coarse reduction, row store; where tiles stop at a triangle's
+    // end, also half a tile of idle lanes).  Measured: 6 / 8 / 10 / 12 / 16 /
+    // 24 give 145 / 141 / 138 / 138 / 141 / 147 us on eight 800x800 views
+    // (tiles crossing triangles), and 0 / 2 / 4 / 8 / 16 give 1.61 / 1.49 /
+    // 1.48 / 1.51 / 1.55 ms for the 80->64->64->3 kernel on 100K triangles
+    static constexpr long long SEG_COST = MF ? 4 : 12;
     // per-wave LDS staging for the weight-gradient transposes: one (delta,
     // activation) pair at a time - (D1, X), (D2, H1), (D3, H2).  The f32
     // products stage floats as [pixel][unit] rows (width + 4 apart); the bf16
@@ -1107,6 +1114,14 @@ int shade_blocks_per_cu(int nl, int c, int dh, bool sh) {
     DS_SHADE_CONFIGS(X)
 #undef X
     return 1;
+}
+
+long long shade_seg_cost(int nl, int c, int dh, bool sh) {
+#define X(NL, C, DH, SH) \
+    if (nl == NL && c == C && dh == DH && sh == SH) return Cfg<NL, C, DH, false, SH>::SEG_COST;
+    DS_SHADE_CONFIGS(X)
+#undef X
+    return 16;
 }
 
 int shade_nparam(int nl, int c, int dh, bool sh) {

@@ -674,7 +674,8 @@ ShadeRanges shade_ranges(const ds_step_desc* d, const Step& st, bool clear_rows)
     const bool acc = d->accumulate != 0;
     return ShadeRanges{st.tri_off, d->T, L.nblocks * kShadeWavesPerBlock, st.bounds,
                        clear_rows && !L.direct ? (acc ? st.gfeat_acc : d->grad_feat) : nullptr,
-                       static_cast<int>(L.S_shade) * d->channels};
+                       static_cast<int>(L.S_shade) * d->channels,
+                       shade_seg_cost(L.nl, d->channels, d->hidden, L.sh)};
 }
 
 // Prologue, binning, resolve (with background shading) and the covered-pixel
