@@ -173,20 +173,22 @@ struct Plan {
     // (CLAIM) NS claim tables of the gradient rows' 4-channel blocks (one int
     // each: rowp bytes)
     static constexpr int NS = K::MF ? 1 : 2;
-    __host__ __device__ static constexpr int base_bytes(int rowp) {
-        return NS * (frow_bytes(rowp) + 4 * rowp) + stage_bytes();
+    __host__ __device__ static constexpr int base_bytes(int rowp, int ns = NS) {
+        return ns * (frow_bytes(rowp) + 4 * rowp) + stage_bytes();
     }
     static constexpr int ROW0 = (row_floats_from_zero(K::NL, K::CP) + 3) & ~3;
     static constexpr int LDS_MAX = 160 * 1024, BLOB = 4 * K::BLOB_ELEMS;
     // preference, for rows starting at level 0: operands and claim table in
     // LDS; claim table only; operands only; neither
-    static constexpr bool FIT_BOTH = BLOB + kWavesPerBlock * (base_bytes(ROW0) + NS * ROW0) <= LDS_MAX;
-    static constexpr bool FIT_CLAIM = kWavesPerBlock * (base_bytes(ROW0) + NS * ROW0) <= LDS_MAX;
-    static constexpr bool FIT_BLOB = BLOB + kWavesPerBlock * base_bytes(ROW0) <= LDS_MAX;
+    static constexpr bool FIT_BOTH = BLOB + kWavesPerBlock * (base_bytes(ROW0, NS) + NS * ROW0) <= LDS_MAX;
+    static constexpr bool FIT_CLAIM = kWavesPerBlock * (base_bytes(ROW0, NS) + NS * ROW0) <= LDS_MAX;
+    static constexpr bool FIT_BLOB = BLOB + kWavesPerBlock * base_bytes(ROW0, NS) <= LDS_MAX;
     static constexpr bool CLAIM = FIT_BOTH || FIT_CLAIM;
     static constexpr bool LB = FIT_BOTH || (!FIT_CLAIM && FIT_BLOB);
     static constexpr int BLOB_BYTES = LB ? BLOB : 0;
-    __host__ __device__ static constexpr int wave_bytes(int rowp) { return base_bytes(rowp) + (CLAIM ? NS * rowp : 0); }
+    __host__ __device__ static constexpr int wave_bytes(int rowp, int ns = NS) {
+        return base_bytes(rowp, ns) + (CLAIM ? ns * rowp : 0);
+    }
 };
 
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -401,7 +403,7 @@ __device__ __forceinline__ void add_mf_blocks(const f32x4 (&acc)[NMF], int n, in
 // its gradient is added with global atomics into the zeroed table.
 template <class K, bool FB, bool SPLIT, bool DIRECT>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock, K::MIN_BLOCKS)
-k_shade(ShadeArgs a, int rowp) {
+k_shade(ShadeArgs a, int rowp, int ns) {
     constexpr int NI = K::NI, NH = K::NH, C = K::C, CP = K::CP, DF = K::DF, DIN = K::DIN, DH = K::DH;
     constexpr bool MF = K::MF;
     constexpr int NMF = MF ? MfPlan<K>::NMF : 1;   // levels that may scatter by MFMA
@@ -414,16 +416,16 @@ k_shade(ShadeArgs a, int rowp) {
     const float* blobs = P::LB ? smem : a.blobs;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
-    unsigned char* wbase = reinterpret_cast<unsigned char*>(smem) + P::BLOB_BYTES + wave * P::wave_bytes(rowp);
+    unsigned char* wbase = reinterpret_cast<unsigned char*>(smem) + P::BLOB_BYTES + wave * P::wave_bytes(rowp, ns);
     // per wave: NS slots of feature row and gradient row, the transpose
     // stage, NS claim tables
     FT* frow0 = reinterpret_cast<FT*>(wbase);
-    FT* frow1 = reinterpret_cast<FT*>(wbase + (P::NS - 1) * P::frow_bytes(rowp));
-    float* grow0 = reinterpret_cast<float*>(wbase + P::NS * P::frow_bytes(rowp));
-    float* grow1 = grow0 + (P::NS - 1) * rowp;
-    ST* stage = reinterpret_cast<ST*>(wbase + P::NS * (P::frow_bytes(rowp) + 4 * rowp));
-    int* claim0 = reinterpret_cast<int*>(wbase + P::base_bytes(rowp));
-    int* claim1 = claim0 + (P::NS - 1) * (rowp / 4);
+    FT* frow1 = reinterpret_cast<FT*>(wbase + (ns - 1) * P::frow_bytes(rowp));
+    float* grow0 = reinterpret_cast<float*>(wbase + ns * P::frow_bytes(rowp));
+    float* grow1 = grow0 + (ns - 1) * rowp;
+    ST* stage = reinterpret_cast<ST*>(wbase + ns * (P::frow_bytes(rowp) + 4 * rowp));
+    int* claim0 = reinterpret_cast<int*>(wbase + P::base_bytes(rowp, ns));
+    int* claim1 = claim0 + (ns - 1) * (rowp / 4);
     const int row = a.S_tot * C;
 
     // stage the weight blobs
@@ -466,7 +468,7 @@ k_shade(ShadeArgs a, int rowp) {
     // lanes stay full (on eight 800x800 views 73.5% of the lanes held a
     // pixel when tiles stopped at each triangle's end); the wave's current
     // triangle is A, in slot sA, the next is N.
-    constexpr bool XT = !K::MF && !DIRECT;
+    const bool XT = !K::MF && !DIRECT && ns == 2;   // wave-uniform
     const int wid = blockIdx.x * kWavesPerBlock + wave;
     const int r0 = a.bounds[wid], r1 = a.bounds[wid + 1];
     const float4* recs4 = reinterpret_cast<const float4*>(a.list);
@@ -552,7 +554,7 @@ k_shade(ShadeArgs a, int rowp) {
         // this tile: up to 16 pixels, within A, or (XT) running on into N
         int end = min(pos + 16, r1);
         if (end > teA) {
-            if constexpr (XT) {
+            if (XT) {
                 if (!haveN) {
                     begin_row(1 - sA, tN);
                     haveN = true;
@@ -1031,7 +1033,7 @@ k_shade(ShadeArgs a, int rowp) {
 }
 
 template <class K, bool FB, bool SPLIT, bool DIRECT>
-hipError_t shade_launch(const ShadeArgs& a, int nblocks, int rowp, size_t bytes, hipStream_t s) {
+hipError_t shade_launch(const ShadeArgs& a, int nblocks, int rowp, int ns, size_t bytes, hipStream_t s) {
     static size_t opted_in = 65536;   // per instance: more LDS needs the kernel's opt-in
     if (bytes > opted_in) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_shade<K, FB, SPLIT, DIRECT>),
@@ -1040,28 +1042,33 @@ hipError_t shade_launch(const ShadeArgs& a, int nblocks, int rowp, size_t bytes,
         if (e != hipSuccess) return e;
         opted_in = bytes;
     }
-    k_shade<K, FB, SPLIT, DIRECT><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp);
+    k_shade<K, FB, SPLIT, DIRECT><<<nblocks, kWave * kWavesPerBlock, bytes, s>>>(a, rowp, ns);
     return hipGetLastError();
 }
 
 template <class K, bool FB, bool DIRECT>
-hipError_t shade_forms(const ShadeArgs& a, int nblocks, int rowp, size_t bytes, hipStream_t s) {
-    if (a.fwd_only || a.ext_grad) return shade_launch<K, FB, true, DIRECT>(a, nblocks, rowp, bytes, s);
-    return shade_launch<K, FB, false, DIRECT>(a, nblocks, rowp, bytes, s);
+hipError_t shade_forms(const ShadeArgs& a, int nblocks, int rowp, int ns, size_t bytes, hipStream_t s) {
+    if (a.fwd_only || a.ext_grad) return shade_launch<K, FB, true, DIRECT>(a, nblocks, rowp, ns, bytes, s);
+    return shade_launch<K, FB, false, DIRECT>(a, nblocks, rowp, ns, bytes, s);
 }
 
 // Staged rows when the LDS plan fits; single-level rows that do not fit are
-// gathered and scattered in global memory instead (DIRECT).
+// gathered and scattered in global memory instead (DIRECT).  The kernels
+// without MF take a second row slot (tiles running on into the next
+// triangle) when it keeps their workgroups per CU.
 template <class K, bool FB>
 hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
     using P = Plan<K, FB>;
     const int rowp = (a.S_tot * K::CP + 3) & ~3;
     const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
-    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
-    if (std::max(lds, red) <= 160 * 1024) return shade_forms<K, FB, false>(a, nblocks, rowp, std::max(lds, red), s);
+    const auto lds_of = [&](int rp, int ns) {
+        return std::max(red, P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rp, ns));
+    };
+    if (P::NS == 2 && lds_of(rowp, 2) <= 160u * 1024u / K::MIN_BLOCKS)
+        return shade_forms<K, FB, false>(a, nblocks, rowp, 2, lds_of(rowp, 2), s);
+    if (lds_of(rowp, 1) <= 160 * 1024) return shade_forms<K, FB, false>(a, nblocks, rowp, 1, lds_of(rowp, 1), s);
     if constexpr (K::NL == 1 && !FB) {
-        const size_t lds0 = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(0);
-        if (std::max(lds0, red) <= 160 * 1024) return shade_forms<K, FB, true>(a, nblocks, 0, std::max(lds0, red), s);
+        if (lds_of(0, 1) <= 160 * 1024) return shade_forms<K, FB, true>(a, nblocks, 0, 1, lds_of(0, 1), s);
     }
     return hipErrorInvalidValue;
 }
@@ -1135,7 +1142,7 @@ bool direct_rows(int S_tot) {
     using P = Plan<K, FB>;
     const int rowp = (S_tot * K::CP + 3) & ~3;
     const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
-    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp);
+    const size_t lds = P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rowp, 1);
     return std::max(lds, red) > 160 * 1024;
 }
 template <int NL, int C, int DH, bool SH>
