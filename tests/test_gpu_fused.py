@@ -58,6 +58,17 @@ def medium():
     return scenes.sphere_torus_soup(0)
 
 
+@pytest.fixture(scope="module")
+def small_l4(small):
+    """The small soup with its 8 feature channels at lattice level 4 (153
+    samples a triangle): rows large enough that the shading keeps one row
+    slot (tiles stop at each triangle's end) to hold two workgroups per CU."""
+    import dataclasses
+    rng = np.random.Generator(np.random.PCG64(11))
+    feat = rng.normal(0, 0.1, size=(small.T, O.feats_at_level(4), 8)).astype(np.float32)
+    return dataclasses.replace(small, feat=feat, levels=(4, 4))
+
+
 def _bf16_round(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(torch.bfloat16).to(torch.float32).numpy()
 
@@ -72,11 +83,12 @@ def _bf16_rows_as_read(feat, lo):
     return out
 
 
-@pytest.mark.parametrize("which,bf16,bf16_feat", [("small", False, False), ("medium", False, False),
-                                                  ("medium", True, False), ("medium", True, True)])
-def test_fused_step_matches_reference(dev, small, medium, which, bf16, bf16_feat):
+@pytest.mark.parametrize("which,bf16,bf16_feat", [("small", False, False), ("small_l4", False, False),
+                                                  ("medium", False, False), ("medium", True, False),
+                                                  ("medium", True, True)])
+def test_fused_step_matches_reference(dev, small, small_l4, medium, which, bf16, bf16_feat):
     import dataclasses
-    s0 = small if which == "small" else medium
+    s0 = {"small": small, "small_l4": small_l4, "medium": medium}[which]
     print(f"fused step {s0.name} bf16={bf16} bf16 features={bf16_feat}", flush=True)
     g = _run(dev, s0, bf16, bf16_features=bf16_feat)
     # with bf16 feature rows the reference sees the features as the kernel
