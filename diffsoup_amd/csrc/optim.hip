@@ -19,13 +19,16 @@ namespace {
 
 constexpr int kMaxGroups = 16;
 constexpr int kThreads = 256;
-// Two launch shapes (measured over 9.1M and 93M parameters): up to about
-// 16M parameters one float4 per thread per chunk and a grid of 16 blocks per
-// CU (46 us vs 49 us at 8 per CU); above, two float4s per thread, issued
-// together, and 4 blocks per CU (522 us vs 585 us).
-constexpr int kChunkSmall = 4 * 1 * kThreads, kChunkLarge = 4 * 2 * kThreads;   // units per chunk
-constexpr int kBlocksSmall = 256 * 16, kBlocksLarge = 256 * 4;
-constexpr long long kLargeUnits = 16ll << 20;
+// Three launch shapes by parameter count (measured over 9.1M, 28M, 93M,
+// 355M and 1.07G parameters): up to about 16M parameters one float4 per
+// thread per chunk and a grid of 16 blocks per CU (46 us vs 49 us at 8 per
+// CU); up to 128M, two float4s per thread, issued together, and 4 blocks per
+// CU (522 us vs 585 us at 93M; 186 vs 211 us at 28M against four float4s);
+// above, four float4s per thread and 2 blocks per CU (355M: 2.26 -> 2.14 ms,
+// 1.07G: 6.73 -> 6.44 ms).
+constexpr int kChunkSmall = 4 * 1 * kThreads, kChunkLarge = 4 * 2 * kThreads, kChunkHuge = 4 * 4 * kThreads;
+constexpr int kBlocksSmall = 256 * 16, kBlocksLarge = 256 * 4, kBlocksHuge = 256 * 2;
+constexpr long long kLargeUnits = 16ll << 20, kHugeUnits = 128ll << 20;
 
 struct Group {
     const float* p;
@@ -224,8 +227,8 @@ extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* 
     tab.n = n_groups;
     long long units = 0;
     for (int i = 0; i < n_groups; ++i) units += groups[i].n;
-    const bool large = units > kLargeUnits;
-    const int chunk = large ? kChunkLarge : kChunkSmall;
+    const int shape = units > kHugeUnits ? 2 : units > kLargeUnits ? 1 : 0;
+    const int chunk = shape == 2 ? kChunkHuge : shape == 1 ? kChunkLarge : kChunkSmall;
     long long blocks = 0;
     for (int i = 0; i < n_groups; ++i) {
         const ds_adam_group& s = groups[i];
@@ -260,7 +263,10 @@ extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* 
     hipStream_t st = static_cast<hipStream_t>(stream);
     long long* t = reinterpret_cast<long long*>(step);
     if (blocks > 0) {
-        if (large)
+        if (shape == 2)
+            k_adam<4><<<static_cast<unsigned>(std::min<long long>(blocks, kBlocksHuge)), kThreads, 0, st>>>(tab, blocks,
+                                                                                                           t);
+        else if (shape == 1)
             k_adam<2><<<static_cast<unsigned>(std::min<long long>(blocks, kBlocksLarge)), kThreads, 0, st>>>(tab, blocks,
                                                                                                             t);
         else

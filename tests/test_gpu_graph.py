@@ -69,6 +69,32 @@ def test_fused_adam_matches_torch_and_vector_adam(dev):
         np.testing.assert_array_equal(a.view(torch.int16).cpu().numpy(), b.to(torch.bfloat16).view(torch.int16).cpu().numpy())
 
 
+@pytest.mark.parametrize("n", [(1 << 20) + 3, (20 << 20) + 5, (130 << 20) + 7])
+def test_fused_adam_launch_shapes(dev, n):
+    """The update at the three launch shapes (up to 16M units, up to 128M,
+    above: one, two and four float4s per thread), against torch.optim.Adam
+    over two steps; the odd sizes take the scalar tail."""
+    from diffsoup_amd.optimize import FusedAdam
+    g = torch.Generator(device=dev).manual_seed(1)
+    p0 = torch.randn(n, generator=g, device=dev)
+    ours, grad = p0.clone(), torch.empty_like(p0)
+    opt = FusedAdam([dict(param=ours, grad=grad, lr=5e-2)])
+    ref = p0.clone().requires_grad_(True)
+    ta = torch.optim.Adam([ref], lr=5e-2, foreach=False)
+    for it in range(2):
+        x = torch.randn(n, generator=g, device=dev)
+        grad.copy_(x)
+        ref.grad = x
+        before = ref.detach().clone()
+        opt.step()
+        ta.step()
+        torch.cuda.synchronize()
+        du, dr = ours - before, ref.detach() - before
+        assert float((du - dr).abs().max()) <= 1e-4 * float(dr.abs().max()), f"step {it}"
+        del before, du, dr
+    assert int(opt.step_count.item()) == 2
+
+
 def _snap(p):
     return {k: getattr(p, k).detach().clone() for k in ("verts", "alpha_logits", "feat", "w1", "b3")}
 
