@@ -17,6 +17,48 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Wave-wide reductions and scans on the VALU's data-parallel primitives
+// (DPP) instead of LDS permutes.  DPP controls used: row_shr:n = 0x110 + n
+// and row_ror:n = 0x120 + n, both inside each 16-lane row.  Every lane of the
+// wave must be active.
+
+// Sum over the 64 lanes, the same value in every lane: each row's total by
+// four rotations, then the four row totals in a fixed order.
+__device__ __forceinline__ int wave_sum_i32(int x) {
+    x += __builtin_amdgcn_mov_dpp(x, 0x128, 0xf, 0xf, false);
+    x += __builtin_amdgcn_mov_dpp(x, 0x124, 0xf, 0xf, false);
+    x += __builtin_amdgcn_mov_dpp(x, 0x122, 0xf, 0xf, false);
+    x += __builtin_amdgcn_mov_dpp(x, 0x121, 0xf, 0xf, false);
+    return (__builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16)) +
+           (__builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_sum_f32(float x) {
+    x += dpp_f32<0x128>(x);
+    x += dpp_f32<0x124>(x);
+    x += dpp_f32<0x122>(x);
+    x += dpp_f32<0x121>(x);
+    auto at = [&](int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); };
+    return (at(0) + at(16)) + (at(32) + at(48));
+}
+
+// Inclusive prefix sum over lanes 0..31 (lanes 32..63 receive unspecified
+// values): a shift-and-add scan inside each row, whose shifted-in lanes read
+// zero, then row 0's total added to row 1.
+__device__ __forceinline__ int scan32_inclusive(int x, int lane) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    const int row0 = __builtin_amdgcn_readlane(x, 15);
+    return lane >= 16 ? x + row0 : x;
+}
+
 // Views per launch: the view table below travels as a kernel argument
 // (about 5.3 KB at 256 views, the largest argument block of any kernel here).
 constexpr int kMaxViews = 256;

@@ -162,6 +162,13 @@ __host__ __device__ __forceinline__ uint32_t feats_in_levels(uint32_t lo, uint32
 
 struct Corners { uint32_t idx[3]; float w[3]; };
 
+// Index of lattice sample (x, y).  Levels are at most 10 (R = 1024), so
+// x + y < 2^23 and the product takes the 24-bit multiplier.
+__host__ __device__ __forceinline__ uint32_t lattice_index(uint32_t x, uint32_t y) {
+    const uint32_t n = (x + y) & 0x7fffffu;
+    return ((n * (n + 1u)) >> 1) + y;
+}
+
 __host__ __device__ __forceinline__ Corners lattice_corners(float b0, float b1, uint32_t level) {
     const uint32_t R = 1u << level;
     const float Rf = static_cast<float>(R);
@@ -177,9 +184,9 @@ __host__ __device__ __forceinline__ Corners lattice_corners(float b0, float b1, 
     const uint32_t x1 = x, y1 = y + 1u;
     const uint32_t x2 = x + fu, y2 = min(y + fu, R - x2);
     Corners c;
-    c.idx[0] = (x0 + y0) * (x0 + y0 + 1u) / 2u + y0;
-    c.idx[1] = (x1 + y1) * (x1 + y1 + 1u) / 2u + y1;
-    c.idx[2] = (x2 + y2) * (x2 + y2 + 1u) / 2u + y2;
+    c.idx[0] = lattice_index(x0, y0);
+    c.idx[1] = lattice_index(x1, y1);
+    c.idx[2] = lattice_index(x2, y2);
     c.w[0] = (1.f - ff) * u + ff * (1.f - v);
     c.w[1] = (1.f - ff) * v + ff * (1.f - u);
     c.w[2] = 1.f - c.w[0] - c.w[1];

@@ -227,9 +227,10 @@ __device__ __forceinline__ void stage_chunk(TileStage<TS>& st, const TileWalk& w
 // stay busy: a triangle covers ~10 of a quad's 64 pixels.
 struct CandGroup {
     int start[32];    // exclusive prefix of the candidate counts
-    // (x0 | y0 << 16, width | staged index << 4 | ceil(1024 / width) << 20)
-    // of each intersection; for a candidate number n < 64 and width w <= 8,
-    // (n * ceil(1024 / w)) >> 10 == n / w exactly (checked over all n, w)
+    // (x0 | y0 << 16, width | staged index << 4 | M << 20) of each
+    // intersection, M = ceil(1024 * rcp(width)) from the hardware reciprocal;
+    // for a candidate number n < 64 and width w <= 8, (n * M) >> 10 == n / w
+    // exactly whenever rcp(w) is within 2 ulp of 1 / w (checked over all n, w)
     uint2 box[32];
     float ndcx[8];    // pixel-centre NDC of the quad's columns / rows
     float ndcy[8];
@@ -264,17 +265,13 @@ __device__ __forceinline__ void for_each_candidate(const TileStage<TS>& st, int 
                 area = wd * (iy1 - iy0);
             }
         }
-        int inc = area;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int o = __shfl_up(inc, d, kWave);
-            if (lane >= d) inc += o;
-        }
-        const int total = __shfl(inc, 31, kWave);
+        const int inc = scan32_inclusive(area, lane);
+        const int total = __builtin_amdgcn_readlane(inc, 31);
         if (lane < 32) {
             c.start[lane] = inc - area;
             c.box[lane] = make_uint2(static_cast<uint32_t>(ix0) | static_cast<uint32_t>(iy0) << 16,
                                      static_cast<uint32_t>(wd) | static_cast<uint32_t>(j) << 4 |
-                                         static_cast<uint32_t>((1024 + wd - 1) / wd) << 20);
+                                         static_cast<uint32_t>(ceilf(1024.f * __builtin_amdgcn_rcpf(static_cast<float>(wd)))) << 20);
         }
         wave_lds_sync();
         for (int k0 = 0; k0 < total; k0 += kWave) {
@@ -299,11 +296,6 @@ __device__ __forceinline__ void for_each_candidate(const TileStage<TS>& st, int 
     }
 }
 
-__device__ __forceinline__ long long wave_sum_ll(long long x) {
-    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    return x;
-}
-
 // Eight waves per SIMD for 8x8 and 16x16 tiles: the record emission of the
 // training step would otherwise take the walk past 64 VGPRs (seven waves);
 // at 64 nothing spills (32x32 tiles would spill, and keep seven).
@@ -322,13 +314,7 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
     __shared__ CandGroup cg[NW];
     __shared__ unsigned long long zb[NW][kWave];   // per-wave z-buffer of packed keys
     const TileWalk w = tile_walk<TS>(v, T, tile_off, stats, true);
-    const unsigned long long seed = mseed.device ? *mseed.device : mseed.value;
-    const uint64_t view_hash = mask_view_hash(seed, static_cast<uint32_t>(w.b));
     const int wl = threadIdx.x % kWave, wq = threadIdx.x / kWave;
-    CandGroup& cgw = cg[wq];
-    unsigned long long* zw = zb[wq];
-    cand_setup(cgw, w, wl);
-    zw[wl] = static_cast<unsigned long long>(kKeyEmpty);
     const bool inside = w.px < v.W[w.b] && w.py < v.H[w.b];
     const long long p = v.pix_base[w.b] + (long long)w.py * v.W[w.b] + w.px;
     // the background shading's target values are loaded now, so that their
@@ -338,10 +324,55 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
         if (inside)
             for (int c = 0; c < 3; ++c) tg[c] = bg.target[p * 3 + c];
     }
+    const bool emitting = SHADE && bg.aux_rec != nullptr;
+    // Background pixels are shaded here: constant colour, the MSE term's
+    // gradient and the per-pixel L1 for the opacity gradient.  Returns the
+    // squared error.
+    auto shade_background = [&]() {
+        float* col = bg.color + p * 3;
+        float* gc = bg.grad_color + p * 3;
+        float l = 0.f, l1 = 0.f;
+        for (int c = 0; c < 3; ++c) {
+            const float d = bg.background - tg[c];
+            col[c] = bg.background;
+            gc[c] = bg.grad_scale * d;
+            l1 += fabsf(d);
+            l += d * d;
+        }
+        bg.pix_loss[p] = l1;
+        return l;
+    };
+
+    if (w.n == 0) {
+        // No triangle touches the tile (block-uniform): every pixel is
+        // background, no fragment is covered and no record is emitted.
+        if (emitting && wl == 0) bg.aux_n[(long long)w.tile * NW + wq] = 0;
+        if (inside) rast[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if constexpr (SHADE) {
+            const float l = wave_sum_f32(inside ? shade_background() : 0.f);
+            if (wl == 0) wave_loss[wq] = l;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                float sum = 0.f;
+                for (int k = 0; k < NW; ++k) sum += wave_loss[k];
+                bg.loss_part[w.tile] = sum;
+                bg.tile_ncov[w.tile] = 0;
+            }
+        }
+        if (threadIdx.x == 0) tile_cov[w.tile] = 0;
+        return;
+    }
+
+    const unsigned long long seed = mseed.device ? *mseed.device : mseed.value;
+    const uint64_t view_hash = mask_view_hash(seed, static_cast<uint32_t>(w.b));
+    CandGroup& cgw = cg[wq];
+    unsigned long long* zw = zb[wq];
+    cand_setup(cgw, w, wl);
+    zw[wl] = static_cast<unsigned long long>(kKeyEmpty);
 
     float bb0 = 0.f, bb1 = 0.f, bd = 0.f;
     int bt = -1;
-    long long covered = 0;
+    int covered = 0;
 
     // Fragment records for the opacity gradient (training step): a covered
     // fragment whose rounded depth is not beyond its pixel's current best is
@@ -350,7 +381,6 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
     // among them (the gradient pass applies the exact test).  Each wave owns
     // its quad's slots and stores a pass's records after the pass, in lane
     // order, behind a wave-uniform count.
-    const bool emitting = SHADE && bg.aux_rec != nullptr;
     const long long aux_base = emitting ? ((long long)w.tile * NW + wq) * bg.aux_per_quad : 0;
     int aux_n = 0;
     bool emit = false;
@@ -422,24 +452,10 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
     }
     const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
     if constexpr (SHADE) {
-        // Background pixels are shaded here (constant colour, MSE term and its
-        // gradient, per-pixel L1 for the opacity gradient).  Covered pixels are
-        // listed per tile in lane order, each with its rank among the covered
-        // pixels of its triangle (one atomic per distinct triangle per wave).
-        float l = 0.f;
-        if (inside && bt < 0) {
-            float* col = bg.color + p * 3;
-            float* gc = bg.grad_color + p * 3;
-            float l1 = 0.f;
-            for (int c = 0; c < 3; ++c) {
-                const float d = bg.background - tg[c];
-                col[c] = bg.background;
-                gc[c] = bg.grad_scale * d;
-                l1 += fabsf(d);
-                l += d * d;
-            }
-            bg.pix_loss[p] = l1;
-        }
+        // Covered pixels are listed per tile in lane order, each with its rank
+        // among the covered pixels of its triangle (one atomic per distinct
+        // triangle per wave).
+        const float l = wave_sum_f32(inside && bt < 0 ? shade_background() : 0.f);
         const bool cov = inside && bt >= 0;
         const unsigned long long m = __ballot(cov);
         const unsigned long long below = m & ((1ull << lane) - 1ull);
@@ -450,7 +466,7 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
         unsigned long long todo = m;
         while (todo) {
             const int l0 = __ffsll(static_cast<long long>(todo)) - 1;
-            const int tl = __shfl(bt, l0, kWave);
+            const int tl = __builtin_amdgcn_readlane(bt, l0);
             const unsigned long long same = __ballot(cov && bt == tl);
             if (cov && bt == tl) {
                 leader = l0;
@@ -463,7 +479,6 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
         if (cov && lane == leader) base = atomicAdd(&bg.tri_count[bt], group);
         base = __shfl(base, leader < 0 ? lane : leader, kWave);
         const int rank = base + local;
-        for (int d = kWave / 2; d > 0; d >>= 1) l += __shfl_xor(l, d, kWave);
         if (lane == 0) {
             wave_loss[wv] = l;
             wave_cnt[wv] = __popcll(m);
@@ -480,8 +495,7 @@ k_resolve(Views v, int T, AlphaSource a, int mask_mode, MaskSeed mseed,
             bg.tile_ncov[w.tile] = n;
         }
     }
-    int cv = static_cast<int>(covered);
-    for (int d = kWave / 2; d > 0; d >>= 1) cv += __shfl_xor(cv, d, kWave);
+    const int cv = wave_sum_i32(covered);
     if (lane == 0) wave_cov[wv] = cv;
     __syncthreads();
     if (threadIdx.x == 0) {
