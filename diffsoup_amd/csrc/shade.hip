@@ -43,6 +43,8 @@ constexpr int kWavesPerBlock = kShadeWavesPerBlock;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // Four fp32 values to bf16, rounded to nearest even, by the hardware's
 // packed conversion (v_cvt_pk_bf16_f32, two values per instruction).
@@ -263,6 +265,19 @@ struct Mma<true> {
         const i16x4 a = reinterpret_cast<const i16x4*>(blob)[lane];
         return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, in, acc, 0, 0, 0);
     }
+    // Two adjacent input tiles t, t+1 in one 16x16x32 product: lane (i, g)
+    // supplies K slots 8g .. 8g+7 as tile t's four values (units 4g .. 4g+3)
+    // followed by tile t+1's; A and B use the same slot order, so the sum
+    // over K is the two 16x16x16 products' sum.
+    __device__ static __forceinline__ f32x4 tile_pair(const float* blob0, const float* blob1, int lane, i16x4 in0,
+                                                      i16x4 in1, f32x4 acc) {
+        const i16x4 a0 = reinterpret_cast<const i16x4*>(blob0)[lane];
+        const i16x4 a1 = reinterpret_cast<const i16x4*>(blob1)[lane];
+        const i16x8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+        const i16x8 b = __builtin_shufflevector(in0, in1, 0, 1, 2, 3, 4, 5, 6, 7);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                       acc, 0, 0, 0);
+    }
     // staged bf16 values, [unit][pixel] rows of spx: A(i, k) = At[i][k], B(k, j) = Bt[j][k]
     __device__ static __forceinline__ f32x4 outer(const unsigned short* At, const unsigned short* Bt, int spx,
                                                   int lane, f32x4 acc) {
@@ -317,9 +332,13 @@ __device__ __forceinline__ void dense(const float* blobs, int base, int lane, co
     if constexpr (K::BF16) {
         i16x4 pin[NIN];
         for (int t = 0; t < NIN; ++t) pin[t] = Mma<true>::pack(in[t]);
-        for (int m = 0; m < NOUT; ++m)
-            for (int t = 0; t < NIN; ++t)
-                out[m] = Mma<true>::tile_packed(blob_at<K>(blobs, base, m, NIN, t), lane, pin[t], out[m]);
+        for (int m = 0; m < NOUT; ++m) {
+            int t = 0;
+            for (; t + 1 < NIN; t += 2)
+                out[m] = Mma<true>::tile_pair(blob_at<K>(blobs, base, m, NIN, t), blob_at<K>(blobs, base, m, NIN, t + 1),
+                                              lane, pin[t], pin[t + 1], out[m]);
+            if (t < NIN) out[m] = Mma<true>::tile_packed(blob_at<K>(blobs, base, m, NIN, t), lane, pin[t], out[m]);
+        }
     } else {
         for (int m = 0; m < NOUT; ++m)
             for (int t = 0; t < NIN; ++t)
