@@ -570,18 +570,24 @@ __global__ void __launch_bounds__(NT) k_fill_lists(Views v, const int2* __restri
 // loss over the probability of the fragment's mask outcome (kept for the
 // visible triangle, dropped for any other), scattered onto the triangle's
 // opacity lattice (src/cuda/multires.cu:178-202).
+// The estimator's factor of one fragment: loss / alpha for the pixel's
+// visible triangle, -loss / (1 - alpha) for any other (both clamped at 1e-5).
+__device__ __forceinline__ float aux_factor(const AlphaSource& a, int t, const Corners& c, int tri_pix, float loss) {
+    constexpr float eps = 1e-5f;
+    const float* src = a.alpha + (long long)t * a.S;
+    float alpha = 0.f;
+    alpha += c.w[0] * src[c.idx[0]];
+    alpha += c.w[1] * src[c.idx[1]];
+    alpha += c.w[2] * src[c.idx[2]];
+    if (tri_pix >= 0 && t == tri_pix) return loss / (alpha > eps ? alpha : eps);
+    const float om = 1.f - alpha;
+    return -(loss / (om > eps ? om : eps));
+}
+
 __device__ __forceinline__ void aux_fragment(const AlphaSource& a, int t, float b0, float b1, int tri_pix, float loss,
                                              float* __restrict__ grad_alpha) {
-    constexpr float eps = 1e-5f;
-    const float alpha = lattice_alpha(a, t, b0, b1);
-    float g;
-    if (tri_pix >= 0 && t == tri_pix) {
-        g = loss / (alpha > eps ? alpha : eps);
-    } else {
-        const float om = 1.f - alpha;
-        g = -(loss / (om > eps ? om : eps));
-    }
     const Corners c = lattice_corners(b0, b1, static_cast<uint32_t>(a.level));
+    const float g = aux_factor(a, t, c, tri_pix, loss);
     float* dst = grad_alpha + (long long)t * a.S;
     unsafeAtomicAdd(&dst[c.idx[0]], g * c.w[0]);
     unsafeAtomicAdd(&dst[c.idx[1]], g * c.w[1]);
