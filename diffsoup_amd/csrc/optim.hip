@@ -118,6 +118,58 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
         coef[threadIdx.x][4] = static_cast<float>(lr);
     }
     __syncthreads();
+    // kPerThread float4s of each operand per thread, all loaded before any
+    // is used: u0 is a multiple of kChunk, so the accesses stay 16-byte aligned
+    struct Regs {
+        float4 p[kPerThread], g[kPerThread], m[kPerThread], v[kPerThread];
+    };
+    auto load = [&](const Group& G, long long u0, long long n4, Regs& r) {
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const long long q = threadIdx.x + k * kThreads;
+            if (q < n4) {
+                const long long e = u0 + 4 * q;
+                r.p[k] = *reinterpret_cast<const float4*>(G.p + e);
+                r.g[k] = *reinterpret_cast<const float4*>(G.g + e);
+                r.m[k] = *reinterpret_cast<const float4*>(G.m + e);
+                r.v[k] = *reinterpret_cast<const float4*>(G.v + e);
+            }
+        }
+    };
+    auto update = [&](const Group& G, int gi, long long u0, long long n4, Regs& r) {
+        const float step_size = coef[gi][0];
+        const float bc2_sqrt = coef[gi][1];
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const long long q = threadIdx.x + k * kThreads;
+            if (q < n4) {
+                const long long e = u0 + 4 * q;
+                float4 o;
+                o.x = adam_update(G, step_size, bc2_sqrt, r.p[k].x, r.g[k].x, r.m[k].x, r.v[k].x);
+                o.y = adam_update(G, step_size, bc2_sqrt, r.p[k].y, r.g[k].y, r.m[k].y, r.v[k].y);
+                o.z = adam_update(G, step_size, bc2_sqrt, r.p[k].z, r.g[k].z, r.m[k].z, r.v[k].z);
+                o.w = adam_update(G, step_size, bc2_sqrt, r.p[k].w, r.g[k].w, r.m[k].w, r.v[k].w);
+                st_nt(G.m + e, r.m[k]);
+                st_nt(G.v + e, r.v[k]);
+                st_nt(G.out + e, o);
+                if (G.out16) {
+                    uint2 h;
+                    h.x = bf16_bits(o.x) | (bf16_bits(o.y) << 16);
+                    h.y = bf16_bits(o.z) | (bf16_bits(o.w) << 16);
+                    st_nt2(G.out16 + e, h);
+                }
+            }
+        }
+    };
+    // With two float4s per thread the next chunk of the same group is loaded
+    // before this chunk's stores are issued: the memory counter retires in
+    // order, so its loads then complete without waiting for them (28M
+    // parameters: 185 -> 165 us).  At four float4s per thread the second
+    // register set costs more than it hides (355M: 2.13 -> 2.25 ms), and at
+    // one the grid's sixteen blocks per CU already hide the stores.
+    constexpr bool kPipe = kPerThread == 2;
+    Regs cur, nxt;
+    bool have = false;   // cur already holds chunk c (loaded during the previous chunk)
     for (long long c = blockIdx.x; c < n_chunks; c += gridDim.x) {
         int gi = 0;
         while (gi + 1 < tab.n && c >= tab.grp[gi + 1].block_base) ++gi;
@@ -133,42 +185,20 @@ __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks
                              (reinterpret_cast<uintptr_t>(G.out16) & 7u) == 0;
             long long i = u0;
             if (vec) {
-                // kPerThread float4s per thread, all loaded before any is
-                // used: u0 is a multiple of kChunk, so the accesses stay
-                // 16-byte aligned
                 const long long n4 = (u1 - u0) / 4;
-                float4 p[kPerThread], g[kPerThread], m[kPerThread], v[kPerThread];
-#pragma unroll
-                for (int k = 0; k < kPerThread; ++k) {
-                    const long long q = threadIdx.x + k * kThreads;
-                    if (q < n4) {
-                        const long long e = u0 + 4 * q;
-                        p[k] = *reinterpret_cast<const float4*>(G.p + e);
-                        g[k] = *reinterpret_cast<const float4*>(G.g + e);
-                        m[k] = *reinterpret_cast<const float4*>(G.m + e);
-                        v[k] = *reinterpret_cast<const float4*>(G.v + e);
+                if (!have) load(G, u0, n4, cur);
+                have = false;
+                if constexpr (kPipe) {
+                    const long long cn = c + gridDim.x;
+                    if (cn < n_chunks && (gi + 1 >= tab.n || cn < tab.grp[gi + 1].block_base)) {
+                        const long long u0n = (cn - G.block_base) * kChunk;
+                        load(G, u0n, (min(u0n + kChunk, G.units) - u0n) / 4, nxt);
+                        have = true;
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < kPerThread; ++k) {
-                    const long long q = threadIdx.x + k * kThreads;
-                    if (q < n4) {
-                        const long long e = u0 + 4 * q;
-                        float4 o;
-                        o.x = adam_update(G, step_size, bc2_sqrt, p[k].x, g[k].x, m[k].x, v[k].x);
-                        o.y = adam_update(G, step_size, bc2_sqrt, p[k].y, g[k].y, m[k].y, v[k].y);
-                        o.z = adam_update(G, step_size, bc2_sqrt, p[k].z, g[k].z, m[k].z, v[k].z);
-                        o.w = adam_update(G, step_size, bc2_sqrt, p[k].w, g[k].w, m[k].w, v[k].w);
-                        st_nt(G.m + e, m[k]);
-                        st_nt(G.v + e, v[k]);
-                        st_nt(G.out + e, o);
-                        if (G.out16) {
-                            uint2 h;
-                            h.x = bf16_bits(o.x) | (bf16_bits(o.y) << 16);
-                            h.y = bf16_bits(o.z) | (bf16_bits(o.w) << 16);
-                            st_nt2(G.out16 + e, h);
-                        }
-                    }
+                update(G, gi, u0, n4, cur);
+                if constexpr (kPipe) {
+                    if (have) cur = nxt;
                 }
                 i = u0 + 4 * n4;
             }
