@@ -177,10 +177,12 @@ hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const
                               hipStream_t s);
 // The opacity gradient from the records a training-step resolve emitted
 // (BackgroundShade::aux_rec); the quads of a tile whose records overflowed
-// walk its bins instead (the same result).
+// walk its bins instead (the same result).  skip_visible: the fragments of
+// each pixel's visible triangle are left out (the shading kernel added them).
 hipError_t launch_opacity_aux_records(const Views& v, int T, const AlphaSource& a, const float* rast,
                                       const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
-                                      const int4* rec, const int* aux_n, int per_quad, hipStream_t s);
+                                      const int4* rec, const int* aux_n, int per_quad, bool skip_visible,
+                                      hipStream_t s);
 hipError_t launch_pixel_l1(long long P, int C, const float* color, const float* target,
                            float* pix_loss, hipStream_t s);
 

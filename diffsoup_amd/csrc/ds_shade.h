@@ -105,6 +105,17 @@ struct ShadeArgs {
     // where each wave's range of list starts: [nblocks * kShadeWavesPerBlock
     // + 1], the ShadeRanges bounds for the same nblocks
     const int* bounds;
+    // The visible fragments' opacity gradient (alpha null: not taken here):
+    // the post-sigmoid opacity lattice [T, Sa] at alpha_level and its
+    // gradient table (zeroed before).  For every covered pixel the fused
+    // single-pass kernel adds the estimator's term of the pixel's own
+    // triangle, L1 / max(alpha, 1e-5) on the three lattice corners of its
+    // barycentrics, into an LDS row per triangle that is stored once; the
+    // opacity-gradient pass then takes only the fragments of other triangles.
+    const float* alpha;
+    float* grad_alpha;
+    int alpha_level;
+    int Sa;
 };
 
 bool shade_supported(int nl, int c, int dh, bool sh);
@@ -116,8 +127,11 @@ int shade_blocks_per_cu(int nl, int c, int dh, bool sh);
 // Cost of a triangle's segment in pixels, for the shading waves' ranges.
 long long shade_seg_cost(int nl, int c, int dh, bool sh);
 
+// visible_alpha (optional) receives whether this launch took the visible
+// fragments' opacity gradient (a.alpha set, the fused single-pass form, and
+// the per-wave LDS rows fitting the plan).
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
-                        hipStream_t s);
+                        hipStream_t s, bool* visible_alpha = nullptr);
 // The waves' ranges of the triangle-grouped list (before any shading launch).
 
 // Whether a configuration gathers and scatters its rows in global memory

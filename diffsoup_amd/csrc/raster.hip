@@ -611,7 +611,7 @@ __device__ void aux_walk_tile(AuxShared<TS>& sh, const Views& v, int T, const Al
                               const TriRecord* __restrict__ recs, const int* __restrict__ tile_off,
                               const int* __restrict__ bins, const long long* __restrict__ stats,
                               const float4* __restrict__ rast, const float* __restrict__ pix_loss,
-                              float* __restrict__ grad_alpha, int tile, bool walk) {
+                              float* __restrict__ grad_alpha, int tile, bool walk, bool skip_visible) {
     constexpr int NT = TS * TS;
     const TileWalk w = tile_walk_at<TS>(v, T, tile_off, stats, tile);
     const int wl = threadIdx.x % kWave, wq = threadIdx.x / kWave;
@@ -645,6 +645,7 @@ __device__ void aux_walk_tile(AuxShared<TS>& sh, const Views& v, int T, const Al
             if (!fragment_depth(h, d)) return;
             const int tri_pix = sh.pix_tri[wq][pi];
             if (tri_pix >= 0 && d > sh.pix_depth[wq][pi]) return;
+            if (skip_visible && sh.st.tri[j] == tri_pix) return;
             aux_fragment(a, sh.st.tri[j], h.b0, h.b1, tri_pix, sh.pix_l1[wq][pi], grad_alpha);
         });
     }
@@ -665,14 +666,14 @@ k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
               const long long* __restrict__ stats, const float4* __restrict__ rast,
               const float* __restrict__ pix_loss, float* __restrict__ grad_alpha,
               const int4* __restrict__ aux_rec, const int* __restrict__ aux_n, int per_quad,
-              int tiles_per_block) {
+              int tiles_per_block, int skip_visible) {
     constexpr int NT = TS * TS;
     constexpr int NW = NT / kWave;
     constexpr int QPR = TS / 8;
     __shared__ AuxShared<TS> sh;
     if (!aux_rec) {
         aux_walk_tile<TS>(sh, v, T, a, recs, tile_off, bins, stats, rast, pix_loss, grad_alpha,
-                          static_cast<int>(blockIdx.x), true);
+                          static_cast<int>(blockIdx.x), true, false);
         return;
     }
     __shared__ int seg_off[NT];
@@ -723,6 +724,8 @@ k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
         const float4 rp = rast[p];
         const int tri_pix = static_cast<int>(rp.w) - 1;
         if (tri_pix >= 0 && __int_as_float(rec.w) > rp.z) continue;
+        // the visible fragment's term was added by the shading kernel
+        if (skip_visible && (rec.x & 0xffffff) == tri_pix) continue;
         aux_fragment(a, rec.x & 0xffffff, __int_as_float(rec.y), __int_as_float(rec.z), tri_pix, pix_loss[p],
                      grad_alpha);
     }
@@ -731,7 +734,7 @@ k_opacity_aux(Views v, int T, AlphaSource a, const TriRecord* __restrict__ recs,
         for (int k = 0; k < NW; ++k) any |= seg_over[t0 * NW + k];
         if (any)
             aux_walk_tile<TS>(sh, v, T, a, recs, tile_off, bins, stats, rast, pix_loss, grad_alpha, tile0 + t0,
-                              seg_over[t0 * NW + wq] != 0);
+                              seg_over[t0 * NW + wq] != 0, skip_visible != 0);
     }
 }
 
@@ -843,7 +846,7 @@ hipError_t launch_raster_resolve_shade(const Views& v, int T, const AlphaSource&
 
 static hipError_t aux_launch(const Views& v, int T, const AlphaSource& a, const float* rast, const float* pix_loss,
                              float* grad_alpha, const RasterWorkspace& w, const int4* rec, const int* aux_n,
-                             int per_quad, hipStream_t s) {
+                             int per_quad, bool skip_visible, hipStream_t s) {
     const int n_tiles = v.tile_base[v.n];
     if (n_tiles == 0) return hipSuccess;
     const float4* r4 = reinterpret_cast<const float4*>(rast);
@@ -858,15 +861,15 @@ static hipError_t aux_launch(const Views& v, int T, const AlphaSource& a, const 
     switch (v.tile) {
         case 8:
             k_opacity_aux<8><<<grid, 64, 0, s>>>(v, T, a, w.recs, w.tile_off, w.bins, w.stats, r4, pix_loss,
-                                                 grad_alpha, rec, aux_n, per_quad, tpb);
+                                                 grad_alpha, rec, aux_n, per_quad, tpb, skip_visible ? 1 : 0);
             break;
         case 16:
             k_opacity_aux<16><<<grid, 256, 0, s>>>(v, T, a, w.recs, w.tile_off, w.bins, w.stats, r4, pix_loss,
-                                                   grad_alpha, rec, aux_n, per_quad, tpb);
+                                                   grad_alpha, rec, aux_n, per_quad, tpb, skip_visible ? 1 : 0);
             break;
         case 32:
             k_opacity_aux<32><<<grid, 1024, 0, s>>>(v, T, a, w.recs, w.tile_off, w.bins, w.stats, r4, pix_loss,
-                                                    grad_alpha, rec, aux_n, per_quad, tpb);
+                                                    grad_alpha, rec, aux_n, per_quad, tpb, skip_visible ? 1 : 0);
             break;
         default:
             return hipErrorInvalidValue;
@@ -877,13 +880,14 @@ static hipError_t aux_launch(const Views& v, int T, const AlphaSource& a, const 
 hipError_t launch_opacity_aux(const Views& v, int T, const AlphaSource& a, const float* rast,
                               const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
                               hipStream_t s) {
-    return aux_launch(v, T, a, rast, pix_loss, grad_alpha, w, nullptr, nullptr, 0, s);
+    return aux_launch(v, T, a, rast, pix_loss, grad_alpha, w, nullptr, nullptr, 0, false, s);
 }
 
 hipError_t launch_opacity_aux_records(const Views& v, int T, const AlphaSource& a, const float* rast,
                                       const float* pix_loss, float* grad_alpha, const RasterWorkspace& w,
-                                      const int4* rec, const int* aux_n, int per_quad, hipStream_t s) {
-    return aux_launch(v, T, a, rast, pix_loss, grad_alpha, w, rec, aux_n, per_quad, s);
+                                      const int4* rec, const int* aux_n, int per_quad, bool skip_visible,
+                                      hipStream_t s) {
+    return aux_launch(v, T, a, rast, pix_loss, grad_alpha, w, rec, aux_n, per_quad, skip_visible, s);
 }
 
 // Workgroups for the shading ranges' jobs: one wave per job, at most 4096 waves.

@@ -135,6 +135,10 @@ struct Cfg {
 
 constexpr int up16(int b) { return (b + 15) & ~15; }
 
+// Floats per row slot of the visible-opacity rows: the triangle's Sa opacity
+// values, then their Sa gradients, padded to whole float4s.
+__host__ __device__ constexpr int va_floats(int Sa) { return (2 * Sa + 3) & ~3; }
+
 // Lattice levels up to this one (3, 6 and 15 samples: one 16-row MFMA block
 // each) scatter their feature gradient by MFMA; finer levels by claims.
 constexpr int kMfmaTopLevel = 2;
@@ -435,7 +439,17 @@ k_shade(ShadeArgs a, int rowp, int ns) {
     const float* blobs = P::LB ? smem : a.blobs;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int g = lane / 16, j = lane % 16;
-    unsigned char* wbase = reinterpret_cast<unsigned char*>(smem) + P::BLOB_BYTES + wave * P::wave_bytes(rowp, ns);
+    // VA: the visible fragments' opacity gradient (the fused single pass;
+    // see ShadeArgs), in NS slots of va_floats(Sa) after the claim tables.
+    // Compiled for the two-waves-per-SIMD kernels only: the one-wave kernel
+    // has no LDS margin for the rows beside its weight operands, and with the
+    // operands in L2 instead it spills.
+    constexpr bool VA_ON = !SPLIT && !DIRECT && !K::MF;
+    const bool VA = VA_ON && a.alpha != nullptr;   // wave-uniform
+    const int Sa = a.Sa;
+    const int vaf = VA ? va_floats(Sa) : 0;
+    unsigned char* wbase =
+        reinterpret_cast<unsigned char*>(smem) + P::BLOB_BYTES + wave * (P::wave_bytes(rowp, ns) + 4 * ns * vaf);
     // per wave: NS slots of feature row and gradient row, the transpose
     // stage, NS claim tables
     FT* frow0 = reinterpret_cast<FT*>(wbase);
@@ -445,6 +459,8 @@ k_shade(ShadeArgs a, int rowp, int ns) {
     ST* stage = reinterpret_cast<ST*>(wbase + ns * (P::frow_bytes(rowp) + 4 * rowp));
     int* claim0 = reinterpret_cast<int*>(wbase + P::base_bytes(rowp, ns));
     int* claim1 = claim0 + (ns - 1) * (rowp / 4);
+    float* va0 = reinterpret_cast<float*>(wbase + P::wave_bytes(rowp, ns));
+    float* va1 = va0 + (ns - 1) * vaf;
     const int row = a.S_tot * C;
 
     // stage the weight blobs
@@ -506,6 +522,17 @@ k_shade(ShadeArgs a, int rowp, int ns) {
     auto begin_row = [&](int sl, int t) {
         FT* frow = sl ? frow1 : frow0;
         float4* g4 = reinterpret_cast<float4*>(sl ? grow1 : grow0);
+        if constexpr (VA_ON) {
+            if (VA) {
+                // the triangle's opacity values and a cleared gradient row
+                float* av = sl ? va1 : va0;
+                const float* src = a.alpha + (long long)t * Sa;
+                for (int i = lane; i < Sa; i += kWave) {
+                    av[i] = src[i];
+                    av[Sa + i] = 0.f;
+                }
+            }
+        }
         for (int i = lane; i < rowp / 4; i += kWave) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if constexpr (DIRECT) {
             // nothing staged
@@ -706,6 +733,22 @@ k_shade(ShadeArgs a, int rowp, int ns) {
                     }
                     dxr[3] = dres;
                     if (!(SPLIT && a.ext_grad)) a.pix_loss[p] = l1;
+                    if constexpr (VA_ON) {
+                        if (VA) {
+                            // the pixel's own triangle's opacity term (src/cuda/rasterize.cu:923-971
+                            // for the kept fragment), added into its slot's LDS gradient row
+                            const Corners ca = lattice_corners(b0, b1, static_cast<uint32_t>(a.alpha_level));
+                            float* av = sl ? va1 : va0;
+                            float al = 0.f;
+                            al += ca.w[0] * av[ca.idx[0]];
+                            al += ca.w[1] * av[ca.idx[1]];
+                            al += ca.w[2] * av[ca.idx[2]];
+                            const float ga = l1 / (al > 1e-5f ? al : 1e-5f);
+                            atomicAdd(av + Sa + ca.idx[0], ga * ca.w[0]);
+                            atomicAdd(av + Sa + ca.idx[1], ga * ca.w[1]);
+                            atomicAdd(av + Sa + ca.idx[2], ga * ca.w[2]);
+                        }
+                    }
                 }
                 if (SPLIT && a.fwd_only) break;   // wave-uniform: colour only, no adjoint
 
@@ -968,6 +1011,20 @@ k_shade(ShadeArgs a, int rowp, int ns) {
                         else if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
                     }
                 }
+                // ---- the triangle's visible-fragment opacity gradient row
+                // (the table is zeroed before the step; the opacity-gradient
+                // pass adds the other triangles' fragments after this kernel)
+                if constexpr (VA_ON) {
+                    if (VA) {
+                        const float* ag = (sA ? va1 : va0) + Sa;
+                        float* dst = a.grad_alpha + (long long)tA * Sa;
+                        for (int i = lane; i < Sa; i += kWave) {
+                            const float gv = ag[i];
+                            if (ownA) dst[i] = gv;
+                            else if (gv != 0.f) unsafeAtomicAdd(&dst[i], gv);
+                        }
+                    }
+                }
                 wave_lds_sync();
             }
             // at the range's end, a staged N (its first pixels were in the
@@ -1075,19 +1132,40 @@ hipError_t shade_forms(const ShadeArgs& a, int nblocks, int rowp, int ns, size_t
 // gathered and scattered in global memory instead (DIRECT).  The kernels
 // without MF take a second row slot (tiles running on into the next
 // triangle) when it keeps their workgroups per CU.
+//
+// The visible-opacity rows (the fused single pass with a.alpha set) are
+// taken when they fit beside the chosen slots: two slots with them, two
+// without, one with, one without.
 template <class K, bool FB>
-hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
+hipError_t shade_dispatch(const ShadeArgs& a0, int nblocks, hipStream_t s, bool* va_out) {
     using P = Plan<K, FB>;
-    const int rowp = (a.S_tot * K::CP + 3) & ~3;
+    const int rowp = (a0.S_tot * K::CP + 3) & ~3;
     const size_t red = sizeof(float) * kWavesPerBlock * K::NPARAM;
-    const auto lds_of = [&](int rp, int ns) {
-        return std::max(red, P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) * P::wave_bytes(rp, ns));
+    // (the kernel compiles the rows for the two-waves-per-SIMD forms only)
+    const bool want_va = !K::MF && a0.alpha && a0.grad_alpha && !a0.fwd_only && !a0.ext_grad && a0.Sa > 0;
+    const auto lds_of = [&](int rp, int ns, bool va) {
+        return std::max(red, P::BLOB_BYTES + static_cast<size_t>(kWavesPerBlock) *
+                                                 (P::wave_bytes(rp, ns) + (va ? 4 * ns * va_floats(a0.Sa) : 0)));
     };
-    if (P::NS == 2 && lds_of(rowp, 2) <= 160u * 1024u / K::MIN_BLOCKS)
-        return shade_forms<K, FB, false>(a, nblocks, rowp, 2, lds_of(rowp, 2), s);
-    if (lds_of(rowp, 1) <= 160 * 1024) return shade_forms<K, FB, false>(a, nblocks, rowp, 1, lds_of(rowp, 1), s);
+    ShadeArgs a = a0;
+    const auto go = [&](int ns, bool va) {
+        if (!va) a.alpha = nullptr;
+        if (va_out) *va_out = va;
+        return shade_forms<K, FB, false>(a, nblocks, rowp, ns, lds_of(rowp, ns, va), s);
+    };
+    if (P::NS == 2) {
+        const size_t lim = 160u * 1024u / K::MIN_BLOCKS;
+        if (want_va && lds_of(rowp, 2, true) <= lim) return go(2, true);
+        if (lds_of(rowp, 2, false) <= lim) return go(2, false);
+    }
+    if (want_va && lds_of(rowp, 1, true) <= 160 * 1024) return go(1, true);
+    if (lds_of(rowp, 1, false) <= 160 * 1024) return go(1, false);
     if constexpr (K::NL == 1 && !FB) {
-        if (lds_of(0, 1) <= 160 * 1024) return shade_forms<K, FB, true>(a, nblocks, 0, 1, lds_of(0, 1), s);
+        if (lds_of(0, 1, false) <= 160 * 1024) {
+            a.alpha = nullptr;
+            if (va_out) *va_out = false;
+            return shade_forms<K, FB, true>(a, nblocks, 0, 1, lds_of(0, 1, false), s);
+        }
     }
     return hipErrorInvalidValue;
 }
@@ -1095,13 +1173,13 @@ hipError_t shade_dispatch(const ShadeArgs& a, int nblocks, hipStream_t s) {
 // One configuration: fp32 MLP, bf16 MLP on fp32 rows, or bf16 MLP on bf16
 // rows (whole 4-channel blocks only).
 template <int NL, int C, int DH, bool SH>
-hipError_t shade_config(bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s) {
-    if (!bf16) return shade_dispatch<Cfg<NL, C, DH, false, SH>, false>(a, nblocks, s);
+hipError_t shade_config(bool bf16, const ShadeArgs& a, int nblocks, hipStream_t s, bool* va) {
+    if (!bf16) return shade_dispatch<Cfg<NL, C, DH, false, SH>, false>(a, nblocks, s, va);
     if (a.feat_bf16) {
-        if constexpr (C % 4 == 0) return shade_dispatch<Cfg<NL, C, DH, true, SH>, true>(a, nblocks, s);
+        if constexpr (C % 4 == 0) return shade_dispatch<Cfg<NL, C, DH, true, SH>, true>(a, nblocks, s, va);
         return hipErrorInvalidValue;
     }
-    return shade_dispatch<Cfg<NL, C, DH, true, SH>, false>(a, nblocks, s);
+    return shade_dispatch<Cfg<NL, C, DH, true, SH>, false>(a, nblocks, s, va);
 }
 
 // The compiled configurations: (levels, channels per level, hidden width,
@@ -1181,9 +1259,11 @@ bool shade_direct_rows(int nl, int c, int dh, bool sh, bool bf16, bool feat_bf16
 }
 
 hipError_t launch_shade(int nl, int c, int dh, bool sh, bool bf16, const ShadeArgs& a, int nblocks,
-                        hipStream_t s) {
+                        hipStream_t s, bool* visible_alpha) {
+    if (visible_alpha) *visible_alpha = false;
 #define X(NL, C, DH, SH) \
-    if (nl == NL && c == C && dh == DH && sh == SH) return shade_config<NL, C, DH, SH>(bf16, a, nblocks, s);
+    if (nl == NL && c == C && dh == DH && sh == SH) \
+        return shade_config<NL, C, DH, SH>(bf16, a, nblocks, s, visible_alpha);
     DS_SHADE_CONFIGS(X)
 #undef X
     return hipErrorInvalidValue;

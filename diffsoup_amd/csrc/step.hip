@@ -543,6 +543,9 @@ struct Step {
     void *scan_tmp, *scan_tmp2;
     float inv_n;   // 1 / (3 P): the MSE mean and the opacity term's 1 / numel(color)
     hipStream_t s;
+    // the shading launch took the visible fragments' opacity gradient: the
+    // opacity-gradient pass then skips them
+    bool va = false;
 };
 
 int step_prepare(const ds_step_desc* d, void* stream, Step& st);
@@ -742,9 +745,10 @@ int enqueue_shade(const ds_step_desc* d, Step& st, int fwd_only, const float* ex
                  acc ? nullptr : reinterpret_cast<const unsigned short*>(d->feat_bf16),
                  acc ? st.gfeat_acc : d->grad_feat, static_cast<int>(L.S_shade), acc ? d->feat_hi : d->feat_lo,
                  st.blobs, d->b1, d->b2, d->b3, d->color, d->grad_color, d->pix_loss, 2.0f * st.inv_n,
-                 st.partials, ext_grad, fwd_only, st.v, d->inv_mvp, st.bounds};
+                 st.partials, ext_grad, fwd_only, st.v, d->inv_mvp, st.bounds, d->alpha, st.galpha,
+                 d->alpha_level, static_cast<int>(L.Sa)};
     hipError_t e;
-    DS_TRY(launch_shade(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, sa, L.nblocks, st.s), "shade");
+    DS_TRY(launch_shade(L.nl, d->channels, d->hidden, L.sh, d->bf16 != 0, sa, L.nblocks, st.s, &st.va), "shade");
     return DS_OK;
 }
 
@@ -763,7 +767,7 @@ int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, b
     DS_TRY(mark(d, s, DS_STAGE_AUX, true), "event");
     AlphaSource a{d->alpha, d->alpha_level, static_cast<int>(L.Sa)};
     DS_TRY(launch_opacity_aux_records(st.v, d->T, a, d->rast, d->pix_loss, st.galpha, st.rw, st.aux_rec,
-                                      st.aux_n, L.aux_per_quad, s),
+                                      st.aux_n, L.aux_per_quad, st.va, s),
            "opacity aux");
     DS_TRY(mark(d, s, DS_STAGE_AUX, false), "event");
     DS_TRY(mark(d, s, DS_STAGE_FINAL, true), "event");
