@@ -383,7 +383,10 @@ k_edge_grad_sorted(Views v, int T, int C, const float* __restrict__ color, const
         unsigned long long todo = __ballot(act);
         while (todo) {
             const int leader = __ffsll(static_cast<long long>(todo)) - 1;
-            const long long k = __shfl(key, leader, 64);
+            // the leader is wave-uniform: its key by two scalar lane reads
+            const long long k = static_cast<long long>(
+                static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(key), leader))) |
+                static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(key >> 32), leader))) << 32);
             const unsigned long long same = __ballot(act && key == k);
             if (act && key == k) grp = ng;
             if (lane == leader)
