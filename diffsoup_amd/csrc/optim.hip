@@ -94,14 +94,16 @@ __device__ __forceinline__ float adam_update(const Group& G, float step_size, fl
 // resident wave of blocks and no tail of stragglers is left.
 template <int kPerThread>
 __global__ void __launch_bounds__(kThreads) k_adam(Table tab, long long n_chunks,
-                                                   const long long* __restrict__ step_before) {
+                                                   const long long* __restrict__ step_before, int advanced) {
     constexpr int kChunk = 4 * kPerThread * kThreads;
     // per group, once per block: the bias corrections (in double, as torch
     // forms beta ** step on the host) and the fp32 factors derived from them
     __shared__ float coef[kMaxGroups][5];   // lr / bc1, sqrt(bc2), bc1, bc2, lr
     if (threadIdx.x < (unsigned)tab.n) {
         const Group& G = tab.grp[threadIdx.x];
-        const double t = static_cast<double>(*step_before + 1);
+        // the iteration: the counter + 1, or the counter itself when an
+        // earlier kernel of the stream has already advanced it
+        const double t = static_cast<double>(*step_before + (advanced ? 0 : 1));
         const double bc1 = 1.0 - pow(G.b1, t), bc2 = 1.0 - pow(G.b2, t);
         // the learning rate of this iteration: the schedule's multiplier
         // final ^ (i / steps), i = t clamped to [1, steps], then the caller's
@@ -248,8 +250,23 @@ int optim_fail(int code, const char* msg);
 
 }  // namespace ds
 
+namespace ds {
+namespace {
+int adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream, bool advanced);
+}  // namespace
+}  // namespace ds
+
 extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream) {
-    using namespace ds;
+    return ds::adam_step(groups, n_groups, step, stream, false);
+}
+
+extern "C" int ds_adam_step_advanced(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream) {
+    return ds::adam_step(groups, n_groups, step, stream, true);
+}
+
+namespace ds {
+namespace {
+int adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream, bool advanced) {
     if (n_groups < 1 || n_groups > kMaxGroups)
         return optim_fail(DS_ERR_INVALID, "adam: 1..16 parameter groups");
     if (!step) return optim_fail(DS_ERR_INVALID, "adam: step counter is null");
@@ -295,15 +312,17 @@ extern "C" int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* 
     if (blocks > 0) {
         if (shape == 2)
             k_adam<4><<<static_cast<unsigned>(std::min<long long>(blocks, kBlocksHuge)), kThreads, 0, st>>>(tab, blocks,
-                                                                                                           t);
+                                                                                                           t, advanced);
         else if (shape == 1)
             k_adam<2><<<static_cast<unsigned>(std::min<long long>(blocks, kBlocksLarge)), kThreads, 0, st>>>(tab, blocks,
-                                                                                                            t);
+                                                                                                            t, advanced);
         else
             k_adam<1><<<static_cast<unsigned>(std::min<long long>(blocks, kBlocksSmall)), kThreads, 0, st>>>(tab, blocks,
-                                                                                                            t);
+                                                                                                            t, advanced);
     }
-    k_tick<<<1, 64, 0, st>>>(t);
+    if (!advanced) k_tick<<<1, 64, 0, st>>>(t);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? DS_OK : optim_fail(DS_ERR_HIP, hipGetErrorString(e));
 }
+}  // namespace
+}  // namespace ds

@@ -347,7 +347,8 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
                                                    const float* __restrict__ galpha, float* __restrict__ glogits,
                                                    const int* __restrict__ tile_cov, int n_tiles,
                                                    long long* __restrict__ stats,
-                                                   unsigned long long* __restrict__ seed_advance) {
+                                                   unsigned long long* __restrict__ seed_advance,
+                                                   long long* __restrict__ counter_advance) {
     __shared__ float acc[16][65];
     __shared__ long long cov[16];
     const int R = (nparam + 63) / 64;
@@ -413,6 +414,8 @@ __global__ void __launch_bounds__(1024) k_epilogue(const float* __restrict__ par
             stats[kStatCovered] = t;
             // the mask seed's last reader (the resolve) has run
             if (seed_advance) *seed_advance += 1ull;
+            // the optimiser that follows reads its counter as advanced
+            if (counter_advance) *counter_advance += 1ll;
         }
         return;
     }
@@ -782,7 +785,8 @@ int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, b
             st.rw.stats,
             d->seed_device && d->advance_seed
                 ? const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(d->seed_device))
-                : nullptr);
+                : nullptr,
+            reinterpret_cast<long long*>(d->advance_counter));
         DS_TRY(hipGetLastError(), "epilogue");
         if (acc && d->T > 0) {
             DS_TRY(accum_lds_opt_in(), "accumulate LDS");

@@ -49,6 +49,7 @@ class StepDesc(ctypes.Structure):
         ("loss_kind", ctypes.c_int), ("advance_seed", ctypes.c_int),
         ("accumulate", ctypes.c_int), ("alpha_lo", ctypes.c_int), ("sh2", ctypes.c_int),
         ("inv_mvp", ctypes.c_void_p), ("aux_records", ctypes.c_int64),
+        ("advance_counter", ctypes.c_void_p),
     ]
 
 
@@ -533,6 +534,11 @@ class CapturedStep:
         torch.cuda.synchronize(step.dev)
         step.desc.seed_device = self.seed_dev.data_ptr()
         step.desc.advance_seed = 1
+        # a FusedAdam's step counter is advanced by the step's last kernel,
+        # and the update reads it as advanced: no launch for the counter
+        fused_counter = getattr(optimizer, "step_count", None) is not None and hasattr(optimizer, "step_advanced")
+        if fused_counter:
+            step.desc.advance_counter = optimizer.step_count.data_ptr()
         self.graph = torch.cuda.CUDAGraph()
         self._workspace = step._ws   # the replays write into this allocation
         step._captured = True
@@ -540,10 +546,14 @@ class CapturedStep:
             with torch.cuda.graph(self.graph):
                 step.run(params, mvp, target, 0, inv_mvp)
                 if optimizer is not None:
-                    optimizer.step()
+                    if fused_counter:
+                        optimizer.step_advanced()
+                    else:
+                        optimizer.step()
         finally:
             step.desc.seed_device = None
             step.desc.advance_seed = 0
+            step.desc.advance_counter = None
         self.set_seed(seed)
 
     def set_seed(self, seed: int) -> None:

@@ -57,6 +57,7 @@ class AdamGroup(ctypes.Structure):
 
 
 _lib.register("ds_adam_step", [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+_lib.register("ds_adam_step_advanced", [ctypes.POINTER(AdamGroup), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 
 
 class FusedAdam:
@@ -157,3 +158,10 @@ class FusedAdam:
     def step(self):
         _lib.check(_lib.lib().ds_adam_step(self._arr, len(self.groups), _lib.ptr(self.step_count),
                                            _lib.stream_ptr(self.device)), "ds_adam_step")
+
+    def step_advanced(self):
+        """The update for a counter that an earlier kernel on the stream has
+        already advanced (a fused step given ``advance_counter``): iteration
+        ``step_count``, and the counter is left as it is."""
+        _lib.check(_lib.lib().ds_adam_step_advanced(self._arr, len(self.groups), _lib.ptr(self.step_count),
+                                                    _lib.stream_ptr(self.device)), "ds_adam_step_advanced")

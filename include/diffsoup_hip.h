@@ -273,6 +273,13 @@ typedef struct ds_step_desc {
                                     fragments overflow its slots walks its
                                     tile's bins again instead (the same
                                     gradient, more time)                     */
+    int64_t* advance_counter;    /* if non-null, a device int64 the step adds 1
+                                    to in its last kernel: the step counter of
+                                    the optimiser update that follows it on the
+                                    same stream, which then reads it as
+                                    already advanced (ds_adam_step_advanced),
+                                    so a captured iteration needs no launch of
+                                    its own for the counter                  */
 } ds_step_desc;
 
 #define DS_LOSS_MSE 0
@@ -425,6 +432,13 @@ typedef struct ds_adam_group {
 
 /* At most 16 groups. */
 int ds_adam_step(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream);
+
+/* ds_adam_step for a counter that something earlier on the stream has
+ * already advanced for this update (ds_step_desc.advance_counter): the
+ * update is iteration *step (not *step + 1) and the counter is left as it
+ * is.  No counterpart in the reference: for callers that capture the step and
+ * the update in one graph and would otherwise spend a launch on the counter. */
+int ds_adam_step_advanced(const ds_adam_group* groups, int n_groups, int64_t* step, void* stream);
 
 /* Per-triangle pixel counts of a raster rast [P,4] (channel 3 = triangle id
  * + 1, 0 = background): counts [T] int64 receives the bincount of the ids
