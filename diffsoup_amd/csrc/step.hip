@@ -186,9 +186,12 @@ __device__ __forceinline__ void accum_row_fwd(const AccumJob& J, int t, const fl
 // the sigmoid's output row is stored in one coalesced pass.  A thread takes
 // a fine vertex and all its channels; per level two LDS reads per channel at
 // most, against three corners per level at every target vertex in the
-// corner form.  Features only: the opacities keep the corner form above.
-template <int C>
-__device__ __forceinline__ void accum_feat_fwd(const AccumJob& J, int t, float* row) {
+// corner form.  With OP the opacity row (one channel, the same levels) goes
+// through the same passes as one more channel, reusing each fine vertex's
+// indices, and its sigmoid is evaluated in double and rounded once; without
+// it the opacities take the corner form above.
+template <int C, bool OP>
+__device__ __forceinline__ void accum_feat_fwd(const AccumJob& J, int t, float* row, const AccumJob& A, float* arow) {
     const int c = C > 0 ? C : J.C;
     const int n_dst = J.S_T * c;
     constexpr int CA = C > 0 ? C : 16;
@@ -214,6 +217,7 @@ __device__ __forceinline__ void accum_feat_fwd(const AccumJob& J, int t, float* 
                 for (int jj = 0; jj < CA; ++jj)
                     if (C > 0 || j0 + jj < c) q[j0 + jj] += 0.5f * (pa[j0 + jj] + pb[j0 + jj]);
             }
+            if constexpr (OP) arow[off_f + k] += 0.5f * (arow[off + static_cast<int>(ka)] + arow[off + static_cast<int>(kb)]);
         }
         __syncthreads();
         off = off_f;
@@ -221,22 +225,35 @@ __device__ __forceinline__ void accum_feat_fwd(const AccumJob& J, int t, float* 
     float* dst = J.dst + (long long)t * n_dst;
     const float* r = row + off * c;
     for (int i = threadIdx.x; i < n_dst; i += blockDim.x) dst[i] = 1.0f / (1.0f + expf(-r[i]));
+    if constexpr (OP) {
+        float* da = A.dst + (long long)t * A.S_T;
+        for (int k = threadIdx.x; k < A.S_T; k += blockDim.x)
+            da[k] = static_cast<float>(1.0 / (1.0 + exp(-static_cast<double>(arow[off + k]))));
+    }
 }
 
 // Both source rows of a triangle are staged together (features, then the
-// opacities after them), so a triangle waits out one round of loads.
-__global__ void __launch_bounds__(256) k_accum_sigmoid(int T, AccumJob f, AccumJob a) {
+// opacities after them), so a triangle waits out one round of loads.  With
+// merge (the host's accum_merge: the opacities over the features' levels, 7
+// or 8 feature channels) the opacities ride along as one more channel.
+__global__ void __launch_bounds__(256) k_accum_sigmoid(int T, AccumJob f, AccumJob a, int merge) {
     extern __shared__ float lds_row[];
     const int nf = f.S_tot * f.C, na = a.S_tot;
     float* frow = lds_row;
     float* arow = lds_row + nf;
+    const bool same = merge != 0;
     for (int t = blockIdx.x; t < T; t += gridDim.x) {
         stage_rows2(frow, f.src + (long long)t * nf, nf, arow, a.src + (long long)t * na, na);
         __syncthreads();
-        if (f.C == 7) accum_feat_fwd<7>(f, t, frow);
-        else if (f.C == 8) accum_feat_fwd<8>(f, t, frow);
-        else accum_feat_fwd<0>(f, t, frow);
-        accum_row_fwd<1>(a, t, arow, true);
+        if (same) {
+            if (f.C == 7) accum_feat_fwd<7, true>(f, t, frow, a, arow);
+            else accum_feat_fwd<8, true>(f, t, frow, a, arow);
+        } else {
+            if (f.C == 7) accum_feat_fwd<7, false>(f, t, frow, a, arow);
+            else if (f.C == 8) accum_feat_fwd<8, false>(f, t, frow, a, arow);
+            else accum_feat_fwd<0, false>(f, t, frow, a, arow);
+            accum_row_fwd<1>(a, t, arow, true);
+        }
         __syncthreads();   // the next triangle's staging overwrites both rows
     }
 }
@@ -319,9 +336,117 @@ __device__ __forceinline__ void accum_row_bwd(const AccumJob& J, int t, float* l
     if (J.lo == J.hi) __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) k_accum_sigmoid_bwd(int T, AccumJob f, AccumJob a) {
+// The same restriction for the features (C channels) and the opacities (one
+// channel over the same levels) together, vertex-major: in LDS a vertex holds
+// CP = C + 1 values rounded up to whole float4s, the opacity after the
+// features, and a thread takes one coarse vertex at a time, forms its seven
+// fine indices once and reads each fine vertex as float4s for every channel.
+// Each value is formed by the arithmetic of accum_row_bwd in the same order.
+// The top level's gradient is stored from registers as it is staged, each
+// coarse level as it is formed, and the levels alternate between two LDS
+// regions, so a level costs one barrier.
+template <int C>
+__device__ __forceinline__ void accum_rows_bwd_op(const AccumJob& F, const AccumJob& A, int t, float* lds) {
+    constexpr int CP = (C + 1 + 3) & ~3, NQ = CP / 4;
+    const int S_T = F.S_T, n_f = S_T * C, n = n_f + S_T;
+    int off_hi = 0;   // first sample of level hi in a row
+    for (int l = F.lo; l < F.hi; ++l) off_hi += static_cast<int>(feats_at_level(static_cast<uint32_t>(l)));
+    const float* gdf = F.grad_dst + (long long)t * n_f;
+    const float* sdf = F.dst + (long long)t * n_f;
+    const float* gda = A.grad_dst + (long long)t * S_T;
+    const float* sda = A.dst + (long long)t * S_T;
+    float* outf = F.grad_src + (long long)t * F.S_tot * C;
+    float* outa = A.grad_src + (long long)t * A.S_tot;
+    float* fine = lds;                   // [S_T][CP]
+    float* coarse = lds + S_T * CP;      // [S_(hi-1)][CP]
+    {
+        constexpr int U = 8;   // loads in flight per thread: 2 U
+        const int nt = blockDim.x;
+        auto put = [&](int e, float ge, float se) {
+            if (e < n_f) {
+                const float r = F.scale * ge * (se * (1.0f - se));
+                fine[(e / C) * CP + e % C] = r;
+                outf[off_hi * C + e] = r;
+            } else {
+                const float r = A.scale * ge * (se * (1.0f - se));
+                fine[(e - n_f) * CP + C] = r;
+                outa[off_hi + e - n_f] = r;
+            }
+        };
+        int i = threadIdx.x;
+        for (; i + (U - 1) * nt < n; i += U * nt) {
+            float a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = i + u * nt;
+                a[u] = e < n_f ? gdf[e] : gda[e - n_f];
+                b[u] = e < n_f ? sdf[e] : sda[e - n_f];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) put(i + u * nt, a[u], b[u]);
+        }
+        for (; i < n; i += nt) put(i, i < n_f ? gdf[i] : gda[i - n_f], i < n_f ? sdf[i] : sda[i - n_f]);
+    }
+    __syncthreads();
+    int off = off_hi;
+    for (int l = F.hi - 1; l >= F.lo; --l) {
+        const int S = static_cast<int>(feats_at_level(static_cast<uint32_t>(l)));
+        const int Rf = 1 << (l + 1);   // the finer level's edge
+        off -= S;
+        for (int k = threadIdx.x; k < S; k += blockDim.x) {
+            uint32_t X, Y;
+            lattice_xy(static_cast<uint32_t>(k), X, Y);
+            const int x = 2 * static_cast<int>(X), y = 2 * static_cast<int>(Y);
+            // fine vertex index of (px, py), or -1 outside the lattice
+            auto idx = [&](int px, int py) -> int {
+                if (px < 0 || py < 0 || px + py > Rf) return -1;
+                const int m = px + py;
+                return ((m * (m + 1)) >> 1) + py;
+            };
+            const int i0 = idx(x, y), i1 = idx(x + 1, y), i2 = idx(x - 1, y), i3 = idx(x, y + 1),
+                      i4 = idx(x, y - 1), i5 = idx(x + 1, y - 1), i6 = idx(x - 1, y + 1);
+            auto row = [&](int v, int q) -> float4 {
+                return v < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : *reinterpret_cast<const float4*>(fine + v * CP + 4 * q);
+            };
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const float4 c0 = row(i0, q), c1 = row(i1, q), c2 = row(i2, q), c3 = row(i3, q), c4 = row(i4, q),
+                             c5 = row(i5, q), c6 = row(i6, q);
+                float r[4];
+                const float v0[4] = {c0.x, c0.y, c0.z, c0.w}, v1[4] = {c1.x, c1.y, c1.z, c1.w},
+                            v2[4] = {c2.x, c2.y, c2.z, c2.w}, v3[4] = {c3.x, c3.y, c3.z, c3.w},
+                            v4[4] = {c4.x, c4.y, c4.z, c4.w}, v5[4] = {c5.x, c5.y, c5.z, c5.w},
+                            v6[4] = {c6.x, c6.y, c6.z, c6.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float nb = ((v1[e] + v2[e]) + (v3[e] + v4[e])) + (v5[e] + v6[e]);
+                    r[e] = v0[e] + 0.5f * nb;
+                }
+                *reinterpret_cast<float4*>(coarse + k * CP + 4 * q) = make_float4(r[0], r[1], r[2], r[3]);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int j = 4 * q + e;
+                    if (j < C) outf[(off + k) * C + j] = r[e];
+                    else if (j == C) outa[off + k] = r[e];
+                }
+            }
+        }
+        __syncthreads();
+        float* tmp = fine;
+        fine = coarse;
+        coarse = tmp;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_accum_sigmoid_bwd(int T, AccumJob f, AccumJob a, int merge) {
     extern __shared__ float lds_row[];
+    const bool same = merge != 0;
     for (int t = blockIdx.x; t < T; t += gridDim.x) {
+        if (same) {
+            if (f.C == 7) accum_rows_bwd_op<7>(f, a, t, lds_row);
+            else accum_rows_bwd_op<8>(f, a, t, lds_row);
+            continue;
+        }
         if (f.C == 7) accum_row_bwd<7>(f, t, lds_row);
         else if (f.C == 8) accum_row_bwd<8>(f, t, lds_row);
         else accum_row_bwd<0>(f, t, lds_row);
@@ -642,12 +767,25 @@ hipError_t mark(const ds_step_desc* d, hipStream_t s, int stage, bool begin) {
 
 // LDS of the accumulation kernels: both source rows (forward), or the
 // target row plus the level below it of the larger job (backward).
-size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L) {
+size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L, bool merge) {
     const size_t below = d->feat_hi > 0 ? feats_at_level(d->feat_hi - 1) : 0;
     const size_t below_a = d->alpha_level > 0 ? feats_at_level(d->alpha_level - 1) : 0;
-    return sizeof(float) * std::max<size_t>((size_t)L.S_src * d->channels + (size_t)L.Sa_src,
-                                            std::max<size_t>((size_t)(L.S_shade + below) * d->channels,
-                                                             (size_t)L.Sa + below_a));
+    const size_t bwd = merge ? (size_t)(L.S_shade + below) * ((d->channels + 4) & ~3)
+                             : std::max<size_t>((size_t)(L.S_shade + below) * d->channels, (size_t)L.Sa + below_a);
+    return sizeof(float) * std::max<size_t>((size_t)L.S_src * d->channels + (size_t)L.Sa_src, bwd);
+}
+
+// Whether the accumulation kernels carry the opacities as one more channel
+// of the features' passes: the same source levels and target level, a
+// feature width those passes are compiled for (7 or 8), and their backward
+// row within the LDS.
+bool accum_merge(const ds_step_desc* d, const StepLayout& L) {
+    return d->alpha_lo == d->feat_lo && d->alpha_level == d->feat_hi && L.Sa == L.S_shade && L.Sa_src == L.S_src &&
+           (d->channels == 7 || d->channels == 8) && accum_lds_bytes(d, L, true) <= 160 * 1024;
+}
+
+size_t accum_lds_bytes(const ds_step_desc* d, const StepLayout& L) {
+    return accum_lds_bytes(d, L, accum_merge(d, L));
 }
 
 // Rows beyond the default 64 KB of LDS need each kernel's opt-in, once.
@@ -709,8 +847,8 @@ int enqueue_forward(const ds_step_desc* d, Step& st, bool clear_rows) {
     DS_TRY(hipGetLastError(), "step prep");
     if (acc && d->T > 0) {
         DS_TRY(accum_lds_opt_in(), "accumulate LDS");
-        k_accum_sigmoid<<<std::min(d->T, 4096), 256, accum_lds_bytes(d, L), s>>>(d->T, feat_job(d, st),
-                                                                                alpha_job(d, st));
+        k_accum_sigmoid<<<std::min(d->T, 4096), 256, accum_lds_bytes(d, L), s>>>(
+            d->T, feat_job(d, st), alpha_job(d, st), accum_merge(d, L) ? 1 : 0);
         DS_TRY(hipGetLastError(), "accumulate + sigmoid");
     }
     DS_TRY(mark(d, s, DS_STAGE_PREP, false), "event");
@@ -790,8 +928,8 @@ int enqueue_backward(const ds_step_desc* d, Step& st, const float* grad_color, b
         DS_TRY(hipGetLastError(), "epilogue");
         if (acc && d->T > 0) {
             DS_TRY(accum_lds_opt_in(), "accumulate LDS");
-            k_accum_sigmoid_bwd<<<std::min(d->T, 4096), 256, accum_lds_bytes(d, L), s>>>(d->T, feat_job(d, st),
-                                                                                        alpha_job(d, st));
+            k_accum_sigmoid_bwd<<<std::min(d->T, 4096), 256, accum_lds_bytes(d, L), s>>>(
+                d->T, feat_job(d, st), alpha_job(d, st), accum_merge(d, L) ? 1 : 0);
             DS_TRY(hipGetLastError(), "accumulate + sigmoid backward");
         }
     }
