@@ -31,6 +31,8 @@
 namespace ds {
 namespace {
 
+constexpr int kWave = 64;
+
 // Zero n floats at p (16-byte aligned) from a grid-stride loop: float4 body,
 // scalar tail.
 __device__ __forceinline__ void zero_floats(float* p, long long n, long long i0, long long stride) {
@@ -98,31 +100,21 @@ struct AccumJob {
     int lo, hi, target, C, S_tot, S_T;
 };
 
-// Two source rows into LDS as one: n0 floats of s0 to d0, then n1 of s1 to
-// d1, with eight loads in flight per thread across both before the first is
-// waited on (a row is a few thousand floats, so one load per iteration would
-// leave each thread waiting out the memory latency once per element).
-__device__ __forceinline__ void stage_rows2(float* d0, const float* __restrict__ s0, int n0, float* d1,
-                                            const float* __restrict__ s1, int n1) {
-    constexpr int U = 8;
-    const int nt = blockDim.x, n = n0 + n1;
-    int i = threadIdx.x;
-    for (; i + (U - 1) * nt < n; i += U * nt) {
-        float v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = i + u * nt;
-            v[u] = e < n0 ? s0[e] : s1[e - n0];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = i + u * nt;
-            if (e < n0) d0[e] = v[u]; else d1[e - n0] = v[u];
-        }
+// Two source rows into LDS as one, n0 floats of s0 then n1 of s1 at dst,
+// by loads that write LDS directly (global_load_lds_dword: lane l of a wave
+// writes the dword at the wave's LDS base + 4 l, the base taken from the
+// first active lane, so every lane passes its wave's base).  No registers
+// hold the data, so all of a row's loads are in flight at once; the
+// vector-memory counter is drained before the caller's barrier.
+// blockDim.x must be a multiple of the wave.
+__device__ __forceinline__ void stage_rows_lds(float* dst, const float* __restrict__ s0, int n0,
+                                               const float* __restrict__ s1, int n1) {
+    const int lane = threadIdx.x % kWave;
+    for (int e = threadIdx.x; e < n0 + n1; e += blockDim.x) {
+        const float* g = e < n0 ? s0 + e : s1 + (e - n0);
+        __builtin_amdgcn_global_load_lds(g, dst + (e - lane), 4, 0, 0);
     }
-    for (; i < n; i += nt) {
-        if (i < n0) d0[i] = s0[i]; else d1[i - n0] = s1[i - n0];
-    }
+    __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0); expcnt, lgkmcnt not waited on
 }
 
 // Forward of both accumulations, one workgroup per triangle (grid-stride):
@@ -243,7 +235,7 @@ __global__ void __launch_bounds__(256) k_accum_sigmoid(int T, AccumJob f, AccumJ
     float* arow = lds_row + nf;
     const bool same = merge != 0;
     for (int t = blockIdx.x; t < T; t += gridDim.x) {
-        stage_rows2(frow, f.src + (long long)t * nf, nf, arow, a.src + (long long)t * na, na);
+        stage_rows_lds(lds_row, f.src + (long long)t * nf, nf, a.src + (long long)t * na, na);
         __syncthreads();
         if (same) {
             if (f.C == 7) accum_feat_fwd<7, true>(f, t, frow, a, arow);
