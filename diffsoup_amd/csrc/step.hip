@@ -117,14 +117,15 @@ __device__ __forceinline__ void stage_rows_lds(float* dst, const float* __restri
     __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0); expcnt, lgkmcnt not waited on
 }
 
-// Forward of both accumulations, one workgroup per triangle (grid-stride):
-// the triangle's source row is staged in LDS, then each thread takes target
-// samples, forms the sample's three corners at every level once and sums all
-// C channels in the order of ds_accumulate_fwd.  Opacities go through a
-// sigmoid evaluated in double and rounded once (their last bit decides mask
-// ties, as in the step's prologue); features through the fp32 sigmoid (the
-// reference applies torch's fp32 sigmoid).  C is a template argument so the
-// index arithmetic folds; CG is the generic runtime-width fallback.
+// The corner form of one accumulation, used for the opacities when their
+// levels differ from the features': with the triangle's source row staged
+// in LDS, each thread takes target samples, forms the sample's three corners
+// at every level once and sums all C channels in the order of
+// ds_accumulate_fwd.  Opacities go through a sigmoid evaluated in double and
+// rounded once (their last bit decides mask ties, as in the step's
+// prologue); features through the fp32 sigmoid (the reference applies
+// torch's fp32 sigmoid).  C is a template argument so the index arithmetic
+// folds; C = 0 is the runtime-width fallback.
 template <int C>
 __device__ __forceinline__ void accum_row_fwd(const AccumJob& J, int t, const float* row, bool exact_sigmoid) {
     const int c = C > 0 ? C : J.C;
