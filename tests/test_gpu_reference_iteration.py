@@ -41,11 +41,14 @@ def _inputs(dev, s):
             torch.from_numpy(s.inv_mvp).to(dev))
 
 
-@pytest.mark.parametrize("levels", [(0, 0), (2, 5)])
-def test_reference_iteration_matches_oracle(dev, levels):
-    s = scenes.reference_training_soup(0, levels=levels, alpha_levels=levels)
-    print(f"reference iteration: levels {levels}, feat {s.feat.shape}, alpha {s.alpha_logits.shape}, "
-          f"views {s.resolutions}", flush=True)
+# opacities over the features' levels (the driver's two phases: the
+# accumulation carries them as one more channel of the feature passes), and
+# over other levels (their own corner-form and restriction passes)
+@pytest.mark.parametrize("levels,alpha_levels", [((0, 0), (0, 0)), ((2, 5), (2, 5)), ((2, 5), (1, 3))])
+def test_reference_iteration_matches_oracle(dev, levels, alpha_levels):
+    s = scenes.reference_training_soup(0, levels=levels, alpha_levels=alpha_levels)
+    print(f"reference iteration: levels {levels}, opacity levels {alpha_levels}, feat {s.feat.shape}, "
+          f"alpha {s.alpha_logits.shape}, views {s.resolutions}", flush=True)
     prm, mvp, target, inv = _inputs(dev, s)
     st = _step(dev, s)
     loss = float(st.run(prm, mvp, target, 21, inv_mvp=inv).item())
