@@ -17,10 +17,13 @@
 //
 //   k_ssim_maps  one workgroup per 60x16 tile of windows: x and y of all
 //                channels with a 10-pixel halo staged in LDS in one pass over
-//                contiguous row segments, then per channel the horizontal pass
-//                of the five sums into LDS, the vertical pass in registers, S
-//                (a per-tile partial sum) and the maps a, b, c written
-//                channel-planar at the window's top-left pixel.
+//                contiguous row segments, then per channel the vertical pass
+//                of the five sums over the 16 window rows and every staged
+//                column into LDS, the horizontal pass in registers, S (a
+//                per-tile partial sum) and the maps a, b, c written
+//                channel-planar at the window's top-left pixel (vertical
+//                first: the first pass covers 16 rows of 70 columns instead
+//                of 26 rows of 60).
 //   k_ssim_grad  one workgroup per 60x16 tile of pixels: per channel the
 //                three maps with their halo in LDS (zero outside the windows)
 //                and the same two passes (the window is symmetric, so the
@@ -32,7 +35,7 @@
 // row segments of all channels at once: a per-channel stride would fetch
 // and write back each cache line once per channel.  In LDS the filter passes
 // are register-blocked: a horizontal pass forms four adjacent outputs from
-// fourteen staged values read as four float4s, a vertical pass two rows from
+// fourteen values read as four float4s, a vertical pass two rows from
 // twelve, so each staged value is read about a third as often as with one
 // output per thread.  The 60-pixel tile width keeps two 8-wave workgroups per
 // CU within the LDS once the staged rows are padded to a 16-byte stride.
@@ -97,7 +100,7 @@ __device__ __forceinline__ float wave_sum(float x) {
 // k_ssim_grad stages the three maps of one channel with their halo, their
 // horizontal sums, the tile's x and y (all channels) and its gradient.
 template <int C>
-constexpr int maps_lds_floats() { return 2 * C * kLH * kLWp + 5 * kLH * kTW; }
+constexpr int maps_lds_floats() { return 2 * C * kLH * kLWp + 5 * kTH * kLWp; }
 template <int C>
 constexpr int grad_lds_floats() { return 3 * kLH * kLWp + 3 * kLH * kTW + 3 * C * kTH * kTW; }
 
@@ -105,83 +108,77 @@ constexpr int grad_lds_floats() { return 3 * kLH * kLWp + 3 * kLH * kTW + 3 * C 
 // or 1 on a side shorter than the window), with every tap loop unrolled so
 // that a thread's LDS reads issue back to back.
 
-// Horizontal sums of x, y, x^2, y^2, xy over `rows` staged rows; a thread
-// forms kQ adjacent outputs from kQ + KW - 1 staged values read as float4s.
-template <int KW>
-__device__ __forceinline__ void moments_h(const float* sx, const float* sy, float* hs, const float (&g)[kMaxTaps],
-                                          int rows, int tid) {
-    constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
-    // one row per 16 lanes (the last idle): a float4 read serves 16 lanes per
-    // LDS pass, and a pass within one row touches each bank group once
-    for (int i = tid; i < rows * kLanesPerRow; i += kThreads) {
-        const int r = i / kLanesPerRow, quad = i - r * kLanesPerRow;
-        if (quad >= kTW / kQ) continue;
-        const int j0 = quad * kQ;
-        float xv[4 * N4], yv[4 * N4];
-#pragma unroll
-        for (int q = 0; q < N4; ++q) {
-            const float4 a = *reinterpret_cast<const float4*>(sx + r * kLWp + j0 + 4 * q);
-            const float4 b = *reinterpret_cast<const float4*>(sy + r * kLWp + j0 + 4 * q);
-            xv[4 * q] = a.x; xv[4 * q + 1] = a.y; xv[4 * q + 2] = a.z; xv[4 * q + 3] = a.w;
-            yv[4 * q] = b.x; yv[4 * q + 1] = b.y; yv[4 * q + 2] = b.z; yv[4 * q + 3] = b.w;
-        }
-        // the products once per staged value, shared by the kQ outputs
-        float xx[NV], yy[NV], xy[NV];
+// Vertical sums of x, y, x^2, y^2, xy for the tile's kTH window rows over
+// every staged column (the horizontal pass then needs no halo rows); a
+// thread forms kR rows of one column from kR + KH - 1 staged values, the
+// products once per staged value.
+template <int KH>
+__device__ __forceinline__ void moments_v(const float* sx, const float* sy, float* vs, const float (&g)[kMaxTaps],
+                                          int tid) {
+    constexpr int NV = kR + KH - 1;
+    for (int i = tid; i < (kTH / kR) * kLW; i += kThreads) {
+        const int rg = i / kLW, j = i - rg * kLW;
+        const int r0 = rg * kR;
+        float xv[NV], yv[NV], xx[NV], yy[NV], xy[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
+            xv[k] = sx[(r0 + k) * kLWp + j];
+            yv[k] = sy[(r0 + k) * kLWp + j];
             xx[k] = xv[k] * xv[k];
             yy[k] = yv[k] * yv[k];
             xy[k] = xv[k] * yv[k];
         }
-        float ax[kQ], ay[kQ], axx[kQ], ayy[kQ], axy[kQ];
 #pragma unroll
-        for (int o = 0; o < kQ; ++o) {
-            ax[o] = ay[o] = axx[o] = ayy[o] = axy[o] = 0.f;
+        for (int rr = 0; rr < kR; ++rr) {
+            float m[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k = 0; k < KW; ++k) {
-                const float wk = KW == 1 ? 1.f : g[k];
-                ax[o] += wk * xv[o + k];
-                ay[o] += wk * yv[o + k];
-                axx[o] += wk * xx[o + k];
-                ayy[o] += wk * yy[o + k];
-                axy[o] += wk * xy[o + k];
+            for (int k = 0; k < KH; ++k) {
+                const float wk = KH == 1 ? 1.f : g[k];
+                m[0] += wk * xv[rr + k];
+                m[1] += wk * yv[rr + k];
+                m[2] += wk * xx[rr + k];
+                m[3] += wk * yy[rr + k];
+                m[4] += wk * xy[rr + k];
             }
+#pragma unroll
+            for (int q = 0; q < 5; ++q) vs[(q * kTH + r0 + rr) * kLWp + j] = m[q];
         }
-        *reinterpret_cast<float4*>(hs + (0 * kLH + r) * kTW + j0) = make_float4(ax[0], ax[1], ax[2], ax[3]);
-        *reinterpret_cast<float4*>(hs + (1 * kLH + r) * kTW + j0) = make_float4(ay[0], ay[1], ay[2], ay[3]);
-        *reinterpret_cast<float4*>(hs + (2 * kLH + r) * kTW + j0) = make_float4(axx[0], axx[1], axx[2], axx[3]);
-        *reinterpret_cast<float4*>(hs + (3 * kLH + r) * kTW + j0) = make_float4(ayy[0], ayy[1], ayy[2], ayy[3]);
-        *reinterpret_cast<float4*>(hs + (4 * kLH + r) * kTW + j0) = make_float4(axy[0], axy[1], axy[2], axy[3]);
     }
 }
 
-// Vertical sums, S and the three derivative maps of channel c for the
-// tile's windows, kR rows per thread; returns this thread's sum of S.
-template <int C, int KH>
-__device__ __forceinline__ float ssim_v(const float* hs, const float (&g)[kMaxTaps], int c, float c1, float c2,
+// Horizontal sums of the five vertical sums, S and the three derivative maps
+// of channel c for the tile's windows, kQ adjacent windows per thread from
+// kQ + KW - 1 values of each sum read as float4s; returns this thread's sum
+// of S.
+template <int C, int KW>
+__device__ __forceinline__ float ssim_h(const float* vs, const float (&g)[kMaxTaps], int c, float c1, float c2,
                                         int Hv, int Wv, int y0, int x0, int W, long long base,
                                         float* __restrict__ maps, long long P, int tid) {
-    constexpr int NV = kR + KH - 1;
+    constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
     float acc = 0.f;
-    for (int i = tid; i < (kTH / kR) * kTW; i += kThreads) {
-        const int rg = i / kTW, j = i - rg * kTW;
-        const int r0 = rg * kR;
-        if (y0 + r0 >= Hv || x0 + j >= Wv) continue;
-        float h[5][NV];
+    for (int i = tid; i < kTH * kLanesPerRow; i += kThreads) {   // one row per 16 lanes, as above
+        const int r = i / kLanesPerRow, quad = i - r * kLanesPerRow;
+        if (quad >= kTW / kQ) continue;
+        const int j0 = quad * kQ, oy = y0 + r;
+        if (oy >= Hv || x0 + j0 >= Wv) continue;
+        float v[5][4 * N4];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
 #pragma unroll
-            for (int k = 0; k < NV; ++k) h[q][k] = hs[(q * kLH + r0 + k) * kTW + j];
+            for (int n = 0; n < N4; ++n) {
+                const float4 a = *reinterpret_cast<const float4*>(vs + (q * kTH + r) * kLWp + j0 + 4 * n);
+                v[q][4 * n] = a.x; v[q][4 * n + 1] = a.y; v[q][4 * n + 2] = a.z; v[q][4 * n + 3] = a.w;
+            }
 #pragma unroll
-        for (int rr = 0; rr < kR; ++rr) {
-            const int oy = y0 + r0 + rr, ox = x0 + j;
-            if (oy >= Hv) break;
+        for (int o = 0; o < kQ; ++o) {
+            const int ox = x0 + j0 + o;
+            if (ox >= Wv) break;
             float m[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
                 m[q] = 0.f;
 #pragma unroll
-                for (int k = 0; k < KH; ++k) m[q] += (KH == 1 ? 1.f : g[k]) * h[q][rr + k];
+                for (int k = 0; k < KW; ++k) m[q] += (KW == 1 ? 1.f : g[k]) * v[q][o + k];
             }
             const float mx = m[0], my = m[1], exx = m[2], eyy = m[3], exy = m[4];
             const float sxx = exx - mx * mx, syy = eyy - my * my, sxy = exy - mx * my;
@@ -192,11 +189,11 @@ __device__ __forceinline__ float ssim_v(const float* hs, const float (&g)[kMaxTa
             const float rb1 = __builtin_amdgcn_rcpf(b1), rb2 = __builtin_amdgcn_rcpf(b2);
             const float lum = a1 * rb1, cs = a2 * rb2;
             acc += lum * cs;
-            const long long o = base + (long long)oy * W + ox;
+            const long long o_ = base + (long long)oy * W + ox;
             // dS/dmy = dL/dmy cs + L dcs/dmy; dS/deyy = -L cs / b2; dS/dexy = 2 L / b2
-            maps[(0 * C + c) * P + o] = 2.f * ((mx - lum * my) * rb1 * cs + lum * (my * cs - mx) * rb2);
-            maps[(1 * C + c) * P + o] = -lum * cs * rb2;
-            maps[(2 * C + c) * P + o] = 2.f * lum * rb2;
+            maps[(0 * C + c) * P + o_] = 2.f * ((mx - lum * my) * rb1 * cs + lum * (my * cs - mx) * rb2);
+            maps[(1 * C + c) * P + o_] = -lum * cs * rb2;
+            maps[(2 * C + c) * P + o_] = 2.f * lum * rb2;
         }
     }
     return acc;
@@ -271,7 +268,7 @@ __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float
     extern __shared__ float lds[];
     float* xs = lds;                       // [C][kLH][kLWp]
     float* ys = xs + C * kLH * kLWp;       // [C][kLH][kLWp]
-    float* hs = ys + C * kLH * kLWp;       // [5][kLH][kTW]
+    float* hs = ys + C * kLH * kLWp;       // [5][kTH][kLWp]  vertical sums
     __shared__ float red[kThreads / 64];
     const int tile = blockIdx.x;
     const int b = view_of_tile(v, tile);
@@ -316,11 +313,11 @@ __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float
     for (int c = 0; c < C; ++c) {
         const float* sx = xs + c * kLH * kLWp;
         const float* sy = ys + c * kLH * kLWp;
-        if (kw == 1) moments_h<1>(sx, sy, hs, g, kTH + kh - 1, tid);
-        else moments_h<kMaxTaps>(sx, sy, hs, g, kTH + kh - 1, tid);
+        if (kh == 1) moments_v<1>(sx, sy, hs, g, tid);
+        else moments_v<kMaxTaps>(sx, sy, hs, g, tid);
         __syncthreads();
-        float acc = kh == 1 ? ssim_v<C, 1>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid)
-                            : ssim_v<C, kMaxTaps>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid);
+        float acc = kw == 1 ? ssim_h<C, 1>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid)
+                            : ssim_h<C, kMaxTaps>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid);
         acc = wave_sum(acc);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
         __syncthreads();   // also orders this channel's reads of hs before the next one's writes
