@@ -26,10 +26,11 @@
 //                of 26 rows of 60).
 //   k_ssim_grad  one workgroup per 60x16 tile of pixels: per channel the
 //                three maps with their halo in LDS (zero outside the windows)
-//                and the same two passes (the window is symmetric, so the
-//                transposed filter is the filter); scale (a + 2 y b + x c)
-//                plus the L1 term l1 sign(y - x) collects in LDS for all
-//                channels and is stored (or added) in one coalesced pass.
+//                and the same two passes, vertical first (the window is
+//                symmetric, so the transposed filter is the filter); scale
+//                (a + 2 y b + x c) plus the L1 term l1 sign(y - x) collects
+//                in LDS for all channels and is stored (or added) in one
+//                coalesced pass.
 //
 // The images interleave their channels, so every global access walks whole
 // row segments of all channels at once: a per-channel stride would fetch
@@ -102,7 +103,7 @@ __device__ __forceinline__ float wave_sum(float x) {
 template <int C>
 constexpr int maps_lds_floats() { return 2 * C * kLH * kLWp + 5 * kTH * kLWp; }
 template <int C>
-constexpr int grad_lds_floats() { return 3 * kLH * kLWp + 3 * kLH * kTW + 3 * C * kTH * kTW; }
+constexpr int grad_lds_floats() { return 3 * kLH * kLWp + 3 * kTH * kLWp + 3 * C * kTH * kTW; }
 
 // The separable passes, one per axis and tap count (the window's 11 taps,
 // or 1 on a side shorter than the window), with every tap loop unrolled so
@@ -199,58 +200,59 @@ __device__ __forceinline__ float ssim_h(const float* vs, const float (&g)[kMaxTa
     return acc;
 }
 
-// Horizontal sums of the three maps of one channel, kQ outputs per thread.
-template <int KW>
-__device__ __forceinline__ void maps_h(const float* sm, float* hs, const float (&g)[kMaxTaps], int rows, int tid) {
-    constexpr int NV = kQ + KW - 1, N4 = (NV + 3) / 4;
-    for (int i = tid; i < rows * kLanesPerRow; i += kThreads) {   // one row per 16 lanes, as above
-        const int r = i / kLanesPerRow, quad = i - r * kLanesPerRow;
-        if (quad >= kTW / kQ) continue;
-        const int j0 = quad * kQ;
+// Vertical sums of the three maps of one channel for the tile's kTH pixel
+// rows over every staged column, kR rows of one column per thread.
+template <int KH>
+__device__ __forceinline__ void maps_v(const float* sm, float* vs, const float (&g)[kMaxTaps], int tid) {
+    constexpr int NV = kR + KH - 1;
+    for (int i = tid; i < (kTH / kR) * kLW; i += kThreads) {
+        const int rg = i / kLW, j = i - rg * kLW;
+        const int r0 = rg * kR;
 #pragma unroll
         for (int mm = 0; mm < 3; ++mm) {
-            float v[4 * N4];
+            float h[NV];
 #pragma unroll
-            for (int q = 0; q < N4; ++q) {
-                const float4 a = *reinterpret_cast<const float4*>(sm + (mm * kLH + r) * kLWp + j0 + 4 * q);
-                v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+            for (int k = 0; k < NV; ++k) h[k] = sm[(mm * kLH + r0 + k) * kLWp + j];
+#pragma unroll
+            for (int rr = 0; rr < kR; ++rr) {
+                float m = 0.f;
+#pragma unroll
+                for (int k = 0; k < KH; ++k) m += (KH == 1 ? 1.f : g[k]) * h[rr + k];
+                vs[(mm * kTH + r0 + rr) * kLWp + j] = m;
             }
-            float out[kQ];
-#pragma unroll
-            for (int o = 0; o < kQ; ++o) {
-                out[o] = 0.f;
-#pragma unroll
-                for (int k = 0; k < KW; ++k) out[o] += (KW == 1 ? 1.f : g[k]) * v[o + k];
-            }
-            *reinterpret_cast<float4*>(hs + (mm * kLH + r) * kTW + j0) = make_float4(out[0], out[1], out[2], out[3]);
         }
     }
 }
 
-// Vertical sums and the gradient of channel c for the tile's pixels, kR
-// rows per thread, into tg.
-template <int C, int KH>
-__device__ __forceinline__ void grad_v(const float* hs, const float (&g)[kMaxTaps], int c, float sc, float l1s,
+// Horizontal sums of the three vertical sums and the gradient of channel c
+// for the tile's pixels, kQ adjacent pixels per thread from kQ + KW - 1
+// values of each sum read as float4s, into tg.
+template <int C, int KW>
+__device__ __forceinline__ void grad_h(const float* vs, const float (&g)[kMaxTaps], int c, float sc, float l1s,
                                        const float* tx_, const float* ty_, float* tg, int tid) {
-    constexpr int rowf = kTW * C, NV = kR + KH - 1;
-    for (int i = tid; i < (kTH / kR) * kTW; i += kThreads) {
-        const int rg = i / kTW, j = i - rg * kTW;
-        const int r0 = rg * kR;
-        float h[3][NV];
+    constexpr int rowf = kTW * C, NV = kQ + KW - 1, N4 = (NV + 3) / 4;
+    for (int i = tid; i < kTH * kLanesPerRow; i += kThreads) {   // one row per 16 lanes, as above
+        const int r = i / kLanesPerRow, quad = i - r * kLanesPerRow;
+        if (quad >= kTW / kQ) continue;
+        const int j0 = quad * kQ;
+        float v[3][4 * N4];
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
+        for (int mm = 0; mm < 3; ++mm)
 #pragma unroll
-            for (int k = 0; k < NV; ++k) h[q][k] = hs[(q * kLH + r0 + k) * kTW + j];
+            for (int n = 0; n < N4; ++n) {
+                const float4 a = *reinterpret_cast<const float4*>(vs + (mm * kTH + r) * kLWp + j0 + 4 * n);
+                v[mm][4 * n] = a.x; v[mm][4 * n + 1] = a.y; v[mm][4 * n + 2] = a.z; v[mm][4 * n + 3] = a.w;
+            }
 #pragma unroll
-        for (int rr = 0; rr < kR; ++rr) {
+        for (int o = 0; o < kQ; ++o) {
             float m[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                m[q] = 0.f;
+            for (int mm = 0; mm < 3; ++mm) {
+                m[mm] = 0.f;
 #pragma unroll
-                for (int k = 0; k < KH; ++k) m[q] += (KH == 1 ? 1.f : g[k]) * h[q][rr + k];
+                for (int k = 0; k < KW; ++k) m[mm] += (KW == 1 ? 1.f : g[k]) * v[mm][o + k];
             }
-            const int e = (r0 + rr) * rowf + j * C + c;
+            const int e = r * rowf + (j0 + o) * C + c;
             const float xv = tx_[e], yv = ty_[e];
             const float d = yv - xv;
             const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
@@ -338,8 +340,8 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
                                                         int accumulate, float* __restrict__ l1_partial) {
     extern __shared__ float lds[];
     float* sm = lds;                       // [3][kLH][kLWp]  one channel's maps with the halo
-    float* hs = sm + 3 * kLH * kLWp;       // [3][kLH][kTW]   their horizontal sums
-    float* tx_ = hs + 3 * kLH * kTW;       // [kTH][kTW * C]  the tile's x, pixel-major
+    float* hs = sm + 3 * kLH * kLWp;       // [3][kTH][kLWp]  their vertical sums
+    float* tx_ = hs + 3 * kTH * kLWp;      // [kTH][kTW * C]  the tile's x, pixel-major
     float* ty_ = tx_ + C * kTH * kTW;      // [kTH][kTW * C]  its y
     float* tg = ty_ + C * kTH * kTW;       // [kTH][kTW * C]  its gradient
     __shared__ float red[kThreads / 64];
@@ -418,12 +420,12 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
         }
         __syncthreads();
         if (c + 1 < C) load_maps(c + 1);
-        if (kw == 1) maps_h<1>(sm, hs, g, kTH + hy, tid);
-        else maps_h<kMaxTaps>(sm, hs, g, kTH + hy, tid);
+        if (kh == 1) maps_v<1>(sm, hs, g, tid);
+        else maps_v<kMaxTaps>(sm, hs, g, tid);
         __syncthreads();
         const float sc = dscale ? v.scale[b * C + c] * dscale[b * C + c] : v.scale[b * C + c];
-        if (kh == 1) grad_v<C, 1>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
-        else grad_v<C, kMaxTaps>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
+        if (kw == 1) grad_h<C, 1>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
+        else grad_h<C, kMaxTaps>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
         __syncthreads();   // the next channel overwrites sm and hs
     }
     // the gradient of all channels, stored in row segments of kTW * C floats
