@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float
     extern __shared__ float lds[];
     float* xs = lds;                       // [C][kLH][kLWp]
     float* ys = xs + C * kLH * kLWp;       // [C][kLH][kLWp]
-    float* hs = ys + C * kLH * kLWp;       // [5][kTH][kLWp]  vertical sums
+    float* vsum = ys + C * kLH * kLWp;     // [5][kTH][kLWp]  vertical sums
     __shared__ float red[kThreads / 64];
     const int tile = blockIdx.x;
     const int b = view_of_tile(v, tile);
@@ -315,14 +315,14 @@ __global__ __launch_bounds__(kThreads) void k_ssim_maps(SsimTiles v, const float
     for (int c = 0; c < C; ++c) {
         const float* sx = xs + c * kLH * kLWp;
         const float* sy = ys + c * kLH * kLWp;
-        if (kh == 1) moments_v<1>(sx, sy, hs, g, tid);
-        else moments_v<kMaxTaps>(sx, sy, hs, g, tid);
+        if (kh == 1) moments_v<1>(sx, sy, vsum, g, tid);
+        else moments_v<kMaxTaps>(sx, sy, vsum, g, tid);
         __syncthreads();
-        float acc = kw == 1 ? ssim_h<C, 1>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid)
-                            : ssim_h<C, kMaxTaps>(hs, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid);
+        float acc = kw == 1 ? ssim_h<C, 1>(vsum, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid)
+                            : ssim_h<C, kMaxTaps>(vsum, g, c, c1, c2, Hv, Wv, y0, x0, W, base, maps, P, tid);
         acc = wave_sum(acc);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
-        __syncthreads();   // also orders this channel's reads of hs before the next one's writes
+        __syncthreads();   // also orders this channel's reads of vsum before the next one's writes
         if (tid == 0) {
             float t = 0.f;
             for (int w = 0; w < kThreads / 64; ++w) t += red[w];
@@ -340,8 +340,8 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
                                                         int accumulate, float* __restrict__ l1_partial) {
     extern __shared__ float lds[];
     float* sm = lds;                       // [3][kLH][kLWp]  one channel's maps with the halo
-    float* hs = sm + 3 * kLH * kLWp;       // [3][kTH][kLWp]  their vertical sums
-    float* tx_ = hs + 3 * kTH * kLWp;      // [kTH][kTW * C]  the tile's x, pixel-major
+    float* vsum = sm + 3 * kLH * kLWp;     // [3][kTH][kLWp]  their vertical sums
+    float* tx_ = vsum + 3 * kTH * kLWp;    // [kTH][kTW * C]  the tile's x, pixel-major
     float* ty_ = tx_ + C * kTH * kTW;      // [kTH][kTW * C]  its y
     float* tg = ty_ + C * kTH * kTW;       // [kTH][kTW * C]  its gradient
     __shared__ float red[kThreads / 64];
@@ -420,13 +420,13 @@ __global__ __launch_bounds__(kThreads) void k_ssim_grad(SsimTiles v, const float
         }
         __syncthreads();
         if (c + 1 < C) load_maps(c + 1);
-        if (kh == 1) maps_v<1>(sm, hs, g, tid);
-        else maps_v<kMaxTaps>(sm, hs, g, tid);
+        if (kh == 1) maps_v<1>(sm, vsum, g, tid);
+        else maps_v<kMaxTaps>(sm, vsum, g, tid);
         __syncthreads();
         const float sc = dscale ? v.scale[b * C + c] * dscale[b * C + c] : v.scale[b * C + c];
-        if (kw == 1) grad_h<C, 1>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
-        else grad_h<C, kMaxTaps>(hs, g, c, sc, l1s, tx_, ty_, tg, tid);
-        __syncthreads();   // the next channel overwrites sm and hs
+        if (kw == 1) grad_h<C, 1>(vsum, g, c, sc, l1s, tx_, ty_, tg, tid);
+        else grad_h<C, kMaxTaps>(vsum, g, c, sc, l1s, tx_, ty_, tg, tid);
+        __syncthreads();   // the next channel overwrites sm and vsum
     }
     // the gradient of all channels, stored in row segments of kTW * C floats
     float l1_acc = 0.f;   // sum of |y - x| over the tile's pixels and channels
